@@ -1,0 +1,154 @@
+/* C ABI of the MI355X (gfx950) implementation of J-UBIK's forward model and
+ * gradient: the spatially variant PSF (linear patching), exposure, read-out,
+ * Poisson likelihood and the correlated-field GP sky, plus a fused gradient
+ * evaluation and the MGVI metric product.
+ *
+ * Conventions
+ *   - Every entry point returns 0 on success or a negative status
+ *     (JUBIK_EINVAL, JUBIK_EUNSUPPORTED, JUBIK_EHIP, JUBIK_ENOMEM,
+ *     JUBIK_ENOTREADY); jubik_last_error() gives the message of the calling
+ *     thread's last failure.
+ *   - Array arguments are device pointers owned by the caller, C-contiguous,
+ *     typed by the plan's dtype (0 = float32, 1 = float64) unless stated.
+ *   - ``stream`` is a hipStream_t (NULL = the default stream).  Execution
+ *     entry points only enqueue work: no host synchronisation, no device
+ *     allocation.  Plans own their workspace; a plan must not be used from
+ *     two host threads at once.
+ *   - Citations name the reference function each entry point replaces
+ *     (paths relative to the J-UBIK source tree).
+ */
+#ifndef JUBIK_HIP_H
+#define JUBIK_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define JUBIK_OK 0
+#define JUBIK_EINVAL (-1)
+#define JUBIK_EUNSUPPORTED (-2)
+#define JUBIK_EHIP (-3)
+#define JUBIK_ENOMEM (-4)
+#define JUBIK_ENOTREADY (-5)
+
+#define JUBIK_FLOAT32 0
+#define JUBIK_FLOAT64 1
+
+#define JUBIK_KERNEL_FULL 0   /* (M, P, [E,] H, W) full-frame kernels */
+#define JUBIK_KERNEL_WINDOW 1 /* (M, P, [E,] 2dx, 2dx) cut windows */
+
+/* Bytes of the FP64 workspace jubik_poisson_nll_grad needs. */
+#define JUBIK_POISSON_WORK_BYTES 8192
+
+const char* jubik_last_error(void);
+int jubik_abi_version(void);
+
+/* ---------------------------------------------------------------- linpatch
+ * Replaces jubik/convolve.py:66-145 linpatch_convolve(x, domain, kernel,
+ * n_patches_per_axis, margin), including the module vmap of its callers
+ * (jubik/instruments/chandra/chandra_response.py:127-130,
+ * jubik/instruments/erosita/erosita_response.py:209-212).
+ * Grid (H, W), E energies, M modules.  kernel_per_energy: 1 if kernels are
+ * (M, P, E, ...), 0 if one kernel per patch serves all energies (the
+ * broadcast branch of convolve.py:167-171).  x_shared: 1 if one input
+ * (E, H, W) feeds all modules (the vmap in_axes=None of x), 0 if the input
+ * is (M, E, H, W).                                                        */
+typedef struct jubik_linpatch jubik_linpatch;
+
+int jubik_linpatch_create(jubik_linpatch** plan, int dtype, int64_t H, int64_t W, int64_t E,
+                          int64_t M, int64_t n_patches_per_axis, int64_t margin,
+                          int kernel_per_energy, int x_shared);
+int jubik_linpatch_destroy(jubik_linpatch* plan);
+/* out[0..7] = dx, window 2dx, FFT length L, P, slicing pad dx/2, jobs M*E,
+ * workspace bytes, spectra bytes. */
+int jubik_linpatch_info(const jubik_linpatch* plan, int64_t* out);
+/* Kernel preparation of convolve.py:104-131 (fftshift, centred cut, margin
+ * pad, ifftshift, unit-sum normalisation) and its FFT; kept in the plan. */
+int jubik_linpatch_set_kernel(jubik_linpatch* plan, const void* kernel, int layout,
+                              void* stream);
+/* y (M, E, H, W) = linpatch_convolve(scale * x). */
+int jubik_linpatch_forward(jubik_linpatch* plan, const void* x, void* y, double scale,
+                           void* stream);
+/* gx = scale * linpatch_convolve^T(gy); the VJP JAX derives for
+ * convolve.py:66-145 (sums over modules when x_shared). */
+int jubik_linpatch_adjoint(jubik_linpatch* plan, const void* gy, void* gx, double scale,
+                           void* stream);
+
+/* ---------------------------------------------------------------- response
+ * Exposure closure of jubik/response.py:12-45 (``exposures * x`` with
+ * numpy broadcasting of operands whose shapes are suffixes of the output:
+ * out[i] = e[i % n_e] * x[i % n_x], i < n_out).  The exposure cut
+ * (response.py:37-40) is applied on the host. */
+int jubik_exposure_apply(int dtype, const void* exposure, int64_t n_e, const void* x,
+                         int64_t n_x, void* out, int64_t n_out, void* stream);
+/* Read-out of jubik/response.py:94-110 (``x[i][~mask[i]]``) as a gather by
+ * the C-order flat indices of the kept pixels, and its transpose (scatter
+ * into a zero array of n elements). */
+int jubik_readout_gather(int dtype, const void* x, const int64_t* index, int64_t nnz, void* out,
+                         void* stream);
+int jubik_readout_scatter(int dtype, const void* v, const int64_t* index, int64_t nnz, int64_t n,
+                          void* out, void* stream);
+/* Poisson NLL of nifty.re's Poissonian(data) as composed at
+ * jubik/instruments/chandra/chandra_likelihood.py:37 and
+ * jubik/instruments/erosita/erosita_likelihood.py:89:
+ * nll (device double) = sum(lam - d log lam); grad = 1 - d / lam (may be
+ * NULL).  ``work``: JUBIK_POISSON_WORK_BYTES of device memory. */
+int jubik_poisson_nll_grad(int dtype, const void* lam, const void* data, int64_t n, double* nll,
+                           void* grad, void* work, void* stream);
+
+/* ---------------------------------------------------------------- GP sky
+ * Correlated-field sky of
+ * jubik/sky_model/multifrequency/spectral_product_mf_sky.py:171-298 (and
+ * the diffuse exp(CF) of jubik/sky_models.py:306-341) with a fixed
+ * amplitude field amp = hdvol * fluct * A[pd] (A[0] = 0), on an N x N GP
+ * grid cropped at the origin to the (H, W) sky:
+ *   s_e = exp(zm + HT(amp_ref xi_ref) + (HT(amp_alpha xi_alpha) + mu) dnu_e)
+ * HT = Re(FFT) + Im(FFT) (nifty.re hartley).  has_alpha = 0 gives the
+ * single-frequency sky exp(zm + HT(amp_ref xi_ref)).                      */
+typedef struct jubik_gp jubik_gp;
+
+int jubik_gp_create(jubik_gp** plan, int dtype, int64_t N, int64_t H, int64_t W, int64_t E,
+                    int has_alpha);
+int jubik_gp_destroy(jubik_gp* plan);
+/* sky (E, H, W).  dnu: device array of E relative log-frequencies. */
+int jubik_gp_forward(jubik_gp* plan, const void* amp_ref, const void* amp_alpha,
+                     const void* xi_ref, const void* xi_alpha, double zm, double mu,
+                     const void* dnu, void* sky, void* stream);
+/* VJP: given sky (E, H, W) and its cotangent, the cotangents of xi_ref and
+ * xi_alpha (N x N each; g_alpha ignored when has_alpha = 0). */
+int jubik_gp_vjp(jubik_gp* plan, const void* amp_ref, const void* amp_alpha, const void* dnu,
+                 const void* sky, const void* cot_sky, void* g_ref, void* g_alpha, void* stream);
+
+/* ---------------------------------------------------------------- fused model
+ * lam_M = Exposure_M(linpatch(a * s(xi), K_M)), read out where counts >= 0,
+ * NLL = sum (lam - d log lam): the likelihood of
+ * jubik/instruments/erosita/erosita_likelihood.py:73-89 and
+ * jubik/instruments/chandra/chandra_response.py:136-142 composed with the
+ * sky.  ``exposure`` (M, E, H, W) includes the effective area and is zero
+ * where cut; ``counts`` (M, E, H, W) int32 holds the data, -1 where masked.
+ * The model keeps pointers to the gp and linpatch plans and to the caller's
+ * exposure/counts buffers.                                                  */
+typedef struct jubik_model jubik_model;
+
+int jubik_model_create(jubik_model** model, jubik_gp* gp, jubik_linpatch* lp, double pixel_area);
+int jubik_model_destroy(jubik_model* model);
+int jubik_model_set_data(jubik_model* model, const void* exposure, const int32_t* counts);
+/* One gradient evaluation: nll (device double) and the gradient with
+ * respect to (xi_ref, xi_alpha).  Forward, Poisson residual and the
+ * hand-written adjoint chain; the sky and lam of this position stay in the
+ * model (for jubik_model_metric). */
+int jubik_model_grad(jubik_model* model, const void* amp_ref, const void* amp_alpha,
+                     const void* xi_ref, const void* xi_alpha, double zm, double mu,
+                     const void* dnu, double* nll, void* g_ref, void* g_alpha, void* stream);
+/* MGVI metric product J^T diag(1/lam) J v at the position of the last
+ * jubik_model_grad call (the metric nifty.re's sampling uses). */
+int jubik_model_metric(jubik_model* model, const void* amp_ref, const void* amp_alpha,
+                       const void* dnu, const void* v_ref, const void* v_alpha, void* out_ref,
+                       void* out_alpha, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* JUBIK_HIP_H */

@@ -1,0 +1,21 @@
+"""jubik_amd: J-UBIK's X-ray forward model and gradient on AMD MI355X (gfx950).
+
+Drop-in names (jubik/__init__.py:31-59): Domain, linpatch_convolve,
+integrate, build_exposure_function, build_readout_function; plus the
+Poisson likelihood, the GP sky and the fused gradient / metric model.  All
+array work runs in libjubik_hip.so (hand-written HIP kernels behind a C ABI,
+include/jubik_hip.h); there is no CPU fallback.
+"""
+from .data import Domain, Vector
+from .convolve import (LinpatchOperator, integrate, linpatch_convolve,
+                       linpatch_convolve_modules)
+from .response import build_exposure_function, build_readout_function
+from .likelihood import Poissonian
+from .sky import CorrelatedFieldSky, powerlaw_amplitude
+from .model import XrayModel, dense_counts
+
+__all__ = [
+    "Domain", "Vector", "LinpatchOperator", "integrate", "linpatch_convolve",
+    "linpatch_convolve_modules", "build_exposure_function", "build_readout_function",
+    "Poissonian", "CorrelatedFieldSky", "powerlaw_amplitude", "XrayModel", "dense_counts",
+]

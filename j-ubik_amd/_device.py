@@ -1,0 +1,63 @@
+"""Device-memory plumbing: torch tensors carry the buffers and the stream.
+
+Arrays may be given as numpy arrays or torch tensors; they are moved to the
+current HIP device as contiguous float32/float64 tensors.  Results are
+returned as the same kind of array as the primary input.
+"""
+import ctypes as C
+
+import numpy as np
+import torch
+
+from . import _lib
+
+
+def require_device():
+    if not torch.cuda.is_available():
+        raise RuntimeError(
+            "jubik_amd computes on an AMD GPU through libjubik_hip.so; no HIP "
+            "device is visible")
+    _lib.load()
+
+
+def float_dtype(*arrays):
+    """float32 if every floating input is float32, else float64 (jubik
+    enforces float64, jubik/enforce_float64.py:1-2)."""
+    seen = []
+    for a in arrays:
+        if a is None:
+            continue
+        dt = a.dtype
+        if isinstance(a, torch.Tensor):
+            seen.append(dt == torch.float32)
+        else:
+            seen.append(np.dtype(dt) == np.float32)
+    return torch.float32 if seen and all(seen) else torch.float64
+
+
+def code(dtype):
+    return _lib.FLOAT32 if dtype == torch.float32 else _lib.FLOAT64
+
+
+def to_device(a, dtype):
+    """Contiguous device tensor of ``dtype`` holding ``a``."""
+    if isinstance(a, torch.Tensor):
+        t = a.to(device="cuda", dtype=dtype)
+    else:
+        t = torch.as_tensor(np.asarray(a), dtype=dtype, device="cuda")
+    return t.contiguous()
+
+
+def like_input(t, ref):
+    """Return ``t`` as the kind of array ``ref`` is (torch stays on device)."""
+    if isinstance(ref, torch.Tensor):
+        return t
+    return t.detach().cpu().numpy()
+
+
+def ptr(t):
+    return C.c_void_p(t.data_ptr()) if t is not None else C.c_void_p(0)
+
+
+def stream():
+    return C.c_void_p(torch.cuda.current_stream().cuda_stream)

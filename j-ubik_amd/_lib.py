@@ -1,0 +1,84 @@
+"""ctypes binding of libjubik_hip.so (the C ABI declared in include/jubik_hip.h).
+
+The library is the only compute path: if it is missing or no HIP device is
+visible, every compute call raises.  Status codes map to the exceptions the
+reference raises (ValueError for invalid arguments).
+"""
+import ctypes as C
+import threading
+from pathlib import Path
+
+LIB_PATH = Path(__file__).resolve().parent / "libjubik_hip.so"
+
+OK, EINVAL, EUNSUPPORTED, EHIP, ENOMEM, ENOTREADY = 0, -1, -2, -3, -4, -5
+FLOAT32, FLOAT64 = 0, 1
+KERNEL_FULL, KERNEL_WINDOW = 0, 1
+POISSON_WORK_BYTES = 8192
+
+_vp, _i64, _int, _dbl = C.c_void_p, C.c_int64, C.c_int, C.c_double
+_dblp = C.POINTER(C.c_double)
+
+# name -> (argtypes); every function returns int except jubik_last_error.
+SIGNATURES = {
+    "jubik_abi_version": [],
+    "jubik_linpatch_create": [C.POINTER(_vp), _int, _i64, _i64, _i64, _i64, _i64, _i64, _int,
+                              _int],
+    "jubik_linpatch_destroy": [_vp],
+    "jubik_linpatch_info": [_vp, C.POINTER(_i64)],
+    "jubik_linpatch_set_kernel": [_vp, _vp, _int, _vp],
+    "jubik_linpatch_forward": [_vp, _vp, _vp, _dbl, _vp],
+    "jubik_linpatch_adjoint": [_vp, _vp, _vp, _dbl, _vp],
+    "jubik_exposure_apply": [_int, _vp, _i64, _vp, _i64, _vp, _i64, _vp],
+    "jubik_readout_gather": [_int, _vp, _vp, _i64, _vp, _vp],
+    "jubik_readout_scatter": [_int, _vp, _vp, _i64, _i64, _vp, _vp],
+    "jubik_poisson_nll_grad": [_int, _vp, _vp, _i64, _vp, _vp, _vp, _vp],
+    "jubik_gp_create": [C.POINTER(_vp), _int, _i64, _i64, _i64, _i64, _int],
+    "jubik_gp_destroy": [_vp],
+    "jubik_gp_forward": [_vp, _vp, _vp, _vp, _vp, _dbl, _dbl, _vp, _vp, _vp],
+    "jubik_gp_vjp": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
+    "jubik_model_create": [C.POINTER(_vp), _vp, _vp, _dbl],
+    "jubik_model_destroy": [_vp],
+    "jubik_model_set_data": [_vp, _vp, _vp],
+    "jubik_model_grad": [_vp, _vp, _vp, _vp, _vp, _dbl, _dbl, _vp, _vp, _vp, _vp, _vp],
+    "jubik_model_metric": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
+}
+
+_lock = threading.Lock()
+_lib = None
+
+
+def load():
+    """Load the shared library once; raise if it has not been built."""
+    global _lib
+    with _lock:
+        if _lib is not None:
+            return _lib
+        if not LIB_PATH.exists():
+            raise RuntimeError(
+                f"{LIB_PATH} is missing: build the HIP library first "
+                "(python -c 'import __graft_entry__ as g; g.build()')")
+        lib = C.CDLL(str(LIB_PATH))
+        lib.jubik_last_error.restype = C.c_char_p
+        lib.jubik_last_error.argtypes = []
+        for name, args in SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.argtypes = args
+            fn.restype = C.c_int
+        _lib = lib
+        return lib
+
+
+_EXC = {EINVAL: ValueError, EUNSUPPORTED: NotImplementedError, EHIP: RuntimeError,
+        ENOMEM: MemoryError, ENOTREADY: RuntimeError}
+
+
+def check(status):
+    """Raise the exception that corresponds to a C ABI status code."""
+    if status == OK:
+        return
+    msg = load().jubik_last_error().decode(errors="replace")
+    raise _EXC.get(status, RuntimeError)(msg)
+
+
+def call(name, *args):
+    check(getattr(load(), name)(*args))

@@ -1,0 +1,63 @@
+"""Build libjubik_hip.so for gfx950 with hipcc (in-tree, no JIT cache).
+
+Each ``csrc/*.hip`` translation unit is compiled in parallel into
+``_build/`` and linked into ``libjubik_hip.so`` next to this file.
+"""
+import os
+import subprocess
+import sys
+from concurrent.futures import ThreadPoolExecutor
+from pathlib import Path
+
+PKG = Path(__file__).resolve().parent
+SRC = PKG / "csrc"
+OUT = PKG / "_build"
+LIB = PKG / "libjubik_hip.so"
+ARCH = os.environ.get("JUBIK_OFFLOAD_ARCH", "gfx950")
+
+# Packed FP32 instructions stay off: only scalar/float2 code is written and
+# the compiler is told not to form v_pk_* FP32 operations from it.
+CFLAGS = [
+    "-std=c++17", "-O3", f"--offload-arch={ARCH}", "-fPIC",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wno-unused-command-line-argument",
+]
+
+
+def _hipcc():
+    for cand in (os.environ.get("HIPCC"), "/opt/rocm/bin/hipcc", "hipcc"):
+        if cand and (os.path.sep not in cand or os.path.exists(cand)):
+            return cand
+    raise RuntimeError("hipcc not found")
+
+
+def _compile(src):
+    obj = OUT / (src.stem + ".o")
+    cmd = [_hipcc(), *CFLAGS, "-c", str(src), "-o", str(obj)]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"hipcc failed for {src.name}:\n{res.stderr[-6000:]}")
+    return obj
+
+
+def build(verbose=True):
+    """Compile every HIP source and link the shared library; returns its path."""
+    OUT.mkdir(exist_ok=True)
+    sources = sorted(SRC.glob("*.hip"))
+    jobs = min(len(sources), max(1, int(os.environ.get("MAX_JOBS", os.cpu_count() or 4))))
+    with ThreadPoolExecutor(max_workers=jobs) as ex:
+        objs = list(ex.map(_compile, sources))
+    tmp = LIB.with_suffix(".so.tmp")
+    cmd = [_hipcc(), "-shared", "-fPIC", f"--offload-arch={ARCH}", *map(str, objs),
+           "-o", str(tmp)]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"link failed:\n{res.stderr[-6000:]}")
+    os.replace(tmp, LIB)
+    if verbose:
+        print(f"built {LIB}", file=sys.stderr)
+    return LIB
+
+
+if __name__ == "__main__":
+    build()

@@ -1,0 +1,211 @@
+"""Spatially variant PSF: drop-in for jubik/convolve.py on the GPU.
+
+``linpatch_convolve(x, domain, kernel, n_patches_per_axis, margin)`` keeps
+the reference signature (jubik/convolve.py:66) and error behaviour; the
+work runs in libjubik_hip.so (row FFT pass, column FFT pass with the kernel
+spectrum, gather-form overlap-add).  ``LinpatchOperator`` is the reusable
+plan (device workspace + prepared kernel spectra) behind it, with the
+adjoint used for gradients.
+"""
+import ctypes
+import math
+
+import numpy as np
+import torch
+
+from . import _device as dev
+from . import _lib
+from .data import is_domain
+
+
+def _geometry(shape, n, margin):
+    H, W = int(shape[-2]), int(shape[-1])
+    dx, dy = int(H / n), int(W / n)
+    return dict(H=H, W=W, dx=dx, dy=dy, w=2 * dx, L=2 * dx + 2 * margin)
+
+
+class LinpatchOperator:
+    """Plan for linpatch_convolve on one geometry.
+
+    shape        sky shape (*lead, H, W); lead axes (energies) are batched.
+    n_modules    number of kernel sets applied to the same input (the vmap
+                 over modules of jubik/instruments/*/..._response.py).
+    kernel_per_energy  kernels carry the lead axes (else one kernel per
+                 patch serves every energy, convolve.py:167-171).
+    x_shared     one input for all modules (else one input per module).
+    dtype        torch.float32 or torch.float64.
+    """
+
+    def __init__(self, shape, n_patches_per_axis, margin, n_modules=1,
+                 kernel_per_energy=True, x_shared=True, dtype=torch.float32):
+        dev.require_device()
+        self.shape = tuple(int(s) for s in shape)
+        self.lead = self.shape[:-2]
+        self.E = int(np.prod(self.lead)) if self.lead else 1
+        self.H, self.W = self.shape[-2:]
+        self.n = int(n_patches_per_axis)
+        self.margin = int(margin)
+        self.M = int(n_modules)
+        self.kernel_per_energy = bool(kernel_per_energy)
+        self.x_shared = bool(x_shared)
+        self.dtype = dtype
+        self._plan = ctypes.c_void_p()
+        _lib.call("jubik_linpatch_create", ctypes.byref(self._plan), dev.code(dtype),
+                  self.H, self.W, self.E, self.M, self.n, self.margin,
+                  int(self.kernel_per_energy), int(self.x_shared))
+        info = (ctypes.c_int64 * 8)()
+        _lib.call("jubik_linpatch_info", self._plan, info)
+        self.dx, self.window, self.L, self.P, self.pad, self.J = (int(v) for v in info[:6])
+        self.workspace_bytes, self.spectra_bytes = int(info[6]), int(info[7])
+        self._kernel_ref = None
+
+    def __del__(self):
+        plan = getattr(self, "_plan", None)
+        if plan is not None and plan.value:
+            try:
+                _lib.load().jubik_linpatch_destroy(plan)
+            except Exception:
+                pass
+            self._plan = None
+
+    # -------------------------------------------------------------- kernels
+    def kernel_shape(self, layout="full"):
+        side = (self.H, self.W) if layout == "full" else (self.window, self.window)
+        ek = self.lead if self.kernel_per_energy else ()
+        mods = (self.M,) if self.M > 1 else ()
+        return mods + (self.P,) + ek + side
+
+    def set_kernel(self, kernel, layout="full"):
+        """Prepare the kernel spectra (convolve.py:104-131).
+
+        layout "full": (M, P, [lead,] H, W) full-frame kernels as the
+        reference takes them; "window": the cut (M, P, [lead,] 2dx, 2dx)
+        windows, i.e. fftshift(kernel)[..., c:-c, c:-c].
+        """
+        lay = {"full": _lib.KERNEL_FULL, "window": _lib.KERNEL_WINDOW}[layout]
+        k = dev.to_device(kernel, self.dtype)
+        expect = self.kernel_shape(layout)
+        if tuple(k.shape) != expect and not (self.M == 1 and tuple(k.shape) == expect[1:]):
+            raise ValueError(f"kernel has shape {tuple(k.shape)}, expected {expect}")
+        _lib.call("jubik_linpatch_set_kernel", self._plan, dev.ptr(k), lay, dev.stream())
+        self._kernel_ref = k  # keep alive until the preparation has run
+        return self
+
+    # -------------------------------------------------------------- apply
+    def forward(self, x, scale=1.0, out=None):
+        """PSF(scale * x): (M, *lead, H, W) (module axis dropped when M = 1)."""
+        xt = dev.to_device(x, self.dtype)
+        want = self.shape if self.x_shared else (self.M,) + self.shape
+        if tuple(xt.shape) != want:
+            raise ValueError(f"input has shape {tuple(xt.shape)}, expected {want}")
+        oshape = ((self.M,) if self.M > 1 else ()) + self.shape
+        y = out if out is not None else torch.empty(oshape, dtype=self.dtype, device="cuda")
+        _lib.call("jubik_linpatch_forward", self._plan, dev.ptr(xt), dev.ptr(y),
+                  float(scale), dev.stream())
+        return y
+
+    def adjoint(self, gy, scale=1.0, out=None):
+        """scale * PSF^T(gy), summed over modules when the input is shared."""
+        gt = dev.to_device(gy, self.dtype)
+        ishape = ((self.M,) if self.M > 1 else ()) + self.shape
+        if tuple(gt.shape) != ishape:
+            raise ValueError(f"cotangent has shape {tuple(gt.shape)}, expected {ishape}")
+        oshape = self.shape if self.x_shared else (self.M,) + self.shape
+        gx = out if out is not None else torch.empty(oshape, dtype=self.dtype, device="cuda")
+        _lib.call("jubik_linpatch_adjoint", self._plan, dev.ptr(gt), dev.ptr(gx),
+                  float(scale), dev.stream())
+        return gx
+
+
+_PLANS = {}
+
+
+def _plan_for(shape, n, margin, M, kernel_per_energy, dtype):
+    key = (tuple(shape), n, margin, M, kernel_per_energy, dtype, torch.cuda.current_device())
+    op = _PLANS.get(key)
+    if op is None:
+        op = LinpatchOperator(shape, n, margin, n_modules=M,
+                              kernel_per_energy=kernel_per_energy, dtype=dtype)
+        _PLANS[key] = op
+    return op
+
+
+def _check_linpatch_args(x, domain, kernel, n, margin, modules):
+    if not is_domain(domain):
+        raise ValueError("domain has to be an instance of jubik.Domain")
+    shape = tuple(int(s) for s in domain.shape)
+    if tuple(x.shape[-2:]) != shape[-2:]:
+        raise ValueError(f"input of shape {tuple(x.shape)} does not match domain {shape}")
+    lead = tuple(x.shape[:-2])
+    kshape = tuple(kernel.shape)
+    if modules:
+        kshape = kshape[1:]
+    P = n * n
+    if not kshape or kshape[0] != P:
+        raise ValueError(f"kernel needs {P} patches on its first axis, got {tuple(kernel.shape)}")
+    if kshape[-2:] != shape[-2:]:
+        raise ValueError("kernel frames must have the shape of the domain")
+    klead = kshape[1:-2]
+    if klead == lead:
+        per_energy = True
+    elif klead == ():
+        per_energy = False
+    else:
+        raise ValueError(f"kernel shape {tuple(kernel.shape)} does not broadcast to input "
+                         f"{tuple(x.shape)}")
+    return lead, per_energy
+
+
+def _linpatch(x, domain, kernel, n, margin, modules):
+    n = int(n)
+    margin = int(margin)
+    lead, per_energy = _check_linpatch_args(x, domain, kernel, n, margin, modules)
+    M = int(kernel.shape[0]) if modules else 1
+    dtype = dev.float_dtype(x, kernel)
+    out_shape = ((M,) if modules else ()) + lead
+    if margin == 0:
+        # The reference crops [..., 0:-0, 0:-0]: an empty result.
+        empty = torch.empty(out_shape + (0, 0), dtype=dtype)
+        return empty if isinstance(x, torch.Tensor) else empty.numpy()
+    if not per_energy and lead:
+        g = _geometry(domain.shape, n, margin)
+        P = n * n
+        print("kernel_shape:", (P, g["L"], g["L"]))
+        print("signal_shape:", (P,) + lead + (g["L"], g["L"]))
+        print("Dimension Inconsistency. Broadcasting PSFs")
+    dev.require_device()
+    op = _plan_for(tuple(lead) + tuple(x.shape[-2:]), n, margin, M, per_energy, dtype)
+    op.set_kernel(kernel, "full")
+    y = op.forward(x)
+    if modules and M == 1:
+        y = y.unsqueeze(0)
+    return dev.like_input(y, x)
+
+
+def linpatch_convolve(x, domain, kernel, n_patches_per_axis, margin):
+    """Functional linear-patching convolution (jubik/convolve.py:66-145).
+
+    x (*lead, H, W); kernel (P, [*lead,] H, W) full-frame PSFs, one per
+    patch, P = n_patches_per_axis ** 2.  float32 inputs run in FP32,
+    anything else in FP64.
+    """
+    return _linpatch(x, domain, kernel, n_patches_per_axis, margin, modules=False)
+
+
+def linpatch_convolve_modules(x, domain, kernels, n_patches_per_axis, margin):
+    """``vmap(linpatch_convolve, in_axes=(None, None, 0, None, None))`` of the
+    response builders (chandra_response.py:127-130, erosita_response.py:209-212):
+    kernels (M, P, [*lead,] H, W) -> (M, *lead, H, W)."""
+    return _linpatch(x, domain, kernels, n_patches_per_axis, margin, modules=True)
+
+
+def integrate(x, domain, axes):
+    """dvol * sum(x) over ``axes`` (jubik/convolve.py:179-184).
+
+    The reference evaluates this with numpy on the host at set-up time
+    (PSF normalisation, chandra_response.py:123); it is not on the
+    per-evaluation path and stays a host reduction here.
+    """
+    total = np.sum(np.asarray(x), axis=tuple(axes))
+    dvol = float(math.prod(domain.distances[i] for i in axes))
+    return dvol * total

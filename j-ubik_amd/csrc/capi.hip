@@ -1,0 +1,510 @@
+// C ABI (include/jubik_hip.h): plans, argument validation, error text and
+// the fused gradient evaluation / metric product built from the kernels.
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/jubik_hip.h"
+#include "gp.hpp"
+#include "linpatch.hpp"
+#include "response.hpp"
+
+namespace jubik {
+
+static thread_local std::string g_error;
+
+void set_error(const std::string& msg) { g_error = msg; }
+int fail(int code, const std::string& msg) {
+  g_error = msg;
+  return code;
+}
+int check_hip(hipError_t e, const char* what) {
+  if (e == hipSuccess) return kOk;
+  g_error = std::string(what) + ": " + hipGetErrorString(e);
+  return e == hipErrorOutOfMemory ? kOutOfMemory : kRuntimeError;
+}
+
+namespace {
+
+// Keeps the plan's device current for the duration of a call.
+struct DeviceGuard {
+  int prev = -1;
+  bool ok = true;
+  explicit DeviceGuard(int dev) {
+    if (hipGetDevice(&prev) != hipSuccess) prev = -1;
+    if (prev != dev) ok = hipSetDevice(dev) == hipSuccess;
+  }
+  ~DeviceGuard() {
+    int cur = -1;
+    if (prev >= 0 && hipGetDevice(&cur) == hipSuccess && cur != prev) (void)hipSetDevice(prev);
+  }
+};
+
+size_t elem_size(int dtype) { return dtype == 0 ? sizeof(float) : sizeof(double); }
+
+int dev_alloc(void** p, size_t bytes, const char* what) {
+  *p = nullptr;
+  if (bytes == 0) bytes = 256;
+  hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess) {
+    *p = nullptr;
+    (void)hipGetLastError();
+    return fail(e == hipErrorOutOfMemory ? kOutOfMemory : kRuntimeError,
+                std::string("allocating ") + what + " (" + std::to_string(bytes) +
+                    " bytes): " + hipGetErrorString(e));
+  }
+  return kOk;
+}
+
+void dev_free(void*& p) {
+  if (p) (void)hipFree(p);
+  p = nullptr;
+}
+
+// Twiddles exp(-2 pi i e / L) in the plan's precision.
+int upload_twiddles(void** dst, int dtype, int L) {
+  const size_t es = elem_size(dtype);
+  std::vector<unsigned char> host(2 * es * L);
+  for (int e = 0; e < L; ++e) {
+    const long double ang = -2.0L * 3.141592653589793238462643383279502884L * e / L;
+    const double c = (double)cosl(ang), s = (double)sinl(ang);
+    if (dtype == 0) {
+      float* h = reinterpret_cast<float*>(host.data());
+      h[2 * e] = (float)c;
+      h[2 * e + 1] = (float)s;
+    } else {
+      double* h = reinterpret_cast<double*>(host.data());
+      h[2 * e] = c;
+      h[2 * e + 1] = s;
+    }
+  }
+  int st = dev_alloc(dst, host.size(), "twiddles");
+  if (st) return st;
+  JUBIK_HIP(hipMemcpy(*dst, host.data(), host.size(), hipMemcpyHostToDevice));
+  return kOk;
+}
+
+// Tent weights of jubik/convolve.py:18-25: numpy linspace(0, 1, d) (i * step,
+// last entry exactly 1) followed by its mirror image.
+int upload_tent(void** dst, int dtype, int d) {
+  std::vector<double> b(2 * d);
+  const double step = d > 1 ? 1.0 / (d - 1) : 0.0;
+  for (int i = 0; i < d; ++i) {
+    double v = (d > 1 && i == d - 1) ? 1.0 : i * step;
+    b[i] = v;
+    b[2 * d - 1 - i] = v;
+  }
+  const size_t es = elem_size(dtype);
+  std::vector<unsigned char> host(es * 2 * d);
+  for (int i = 0; i < 2 * d; ++i) {
+    if (dtype == 0) reinterpret_cast<float*>(host.data())[i] = (float)b[i];
+    else reinterpret_cast<double*>(host.data())[i] = b[i];
+  }
+  int st = dev_alloc(dst, host.size(), "tent weights");
+  if (st) return st;
+  JUBIK_HIP(hipMemcpy(*dst, host.data(), host.size(), hipMemcpyHostToDevice));
+  return kOk;
+}
+
+}  // namespace
+}  // namespace jubik
+
+using namespace jubik;
+
+struct jubik_linpatch {
+  int dtype = 0;
+  int device = 0;
+  LinGeom g{};
+  void* tw = nullptr;
+  void* tent = nullptr;
+  void* KT = nullptr;
+  void* R1 = nullptr;
+  void* R2 = nullptr;
+  size_t ws_bytes = 0, spec_bytes = 0;
+  bool ready = false;
+  LinBuffers bufs() const { return LinBuffers{tw, tent, KT, R1, R2}; }
+  void release() {
+    dev_free(tw);
+    dev_free(tent);
+    dev_free(KT);
+    dev_free(R1);
+    dev_free(R2);
+  }
+};
+
+struct jubik_gp {
+  int dtype = 0;
+  int device = 0;
+  GpGeom g{};
+  void* tw = nullptr;
+  void* work = nullptr;
+  void* u = nullptr;
+  GpBuffers bufs() const { return GpBuffers{tw, work}; }
+  void release() {
+    dev_free(tw);
+    dev_free(work);
+    dev_free(u);
+  }
+};
+
+struct jubik_model {
+  jubik_gp* gp = nullptr;
+  jubik_linpatch* lp = nullptr;
+  double pixel_area = 1.0;
+  const void* expo = nullptr;
+  const int32_t* counts = nullptr;
+  void* sky = nullptr;
+  void* dsky = nullptr;
+  void* resid = nullptr;
+  void* lam = nullptr;
+  void* partial = nullptr;
+  void* u = nullptr;
+  bool have_point = false;
+  void release() {
+    dev_free(sky);
+    dev_free(dsky);
+    dev_free(resid);
+    dev_free(lam);
+    dev_free(partial);
+    dev_free(u);
+  }
+};
+
+static hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+extern "C" {
+
+const char* jubik_last_error(void) { return g_error.c_str(); }
+int jubik_abi_version(void) { return 1; }
+
+// ------------------------------------------------------------------ linpatch
+int jubik_linpatch_create(jubik_linpatch** plan, int dtype, int64_t H, int64_t W, int64_t E,
+                          int64_t M, int64_t n, int64_t margin, int kernel_per_energy,
+                          int x_shared) {
+  if (!plan) return fail(kInvalidArgument, "plan pointer is NULL");
+  *plan = nullptr;
+  if (dtype != 0 && dtype != 1) return fail(kInvalidArgument, "dtype must be 0 or 1");
+  if (H < 2 || W < 2 || E < 1 || M < 1 || n < 1)
+    return fail(kInvalidArgument, "H, W >= 2 and E, M, n_patches_per_axis >= 1 required");
+  if (margin < 1) return fail(kInvalidArgument, "margin must be positive");
+  if (n > 64) return fail(kUnsupported, "at most 64 patches per axis are supported");
+  if (W > 4096) return fail(kUnsupported, "grids wider than 4096 pixels are not supported");
+  const int64_t dx = H / n, dy = W / n;
+  if (dx < 1) return fail(kInvalidArgument, "more patches per axis than pixels");
+  if (dx != dy)
+    return fail(kInvalidArgument,
+                "patch strides differ along the two axes (square windows are required)");
+  LinGeom g{};
+  g.H = (int)H;
+  g.W = (int)W;
+  g.n = (int)n;
+  g.d = (int)dx;
+  g.h = (int)(dx / 2);
+  g.w = (int)(2 * dx);
+  g.m = (int)margin;
+  g.L = g.w + 2 * g.m;
+  g.L2 = g.L / 2 + 1;
+  g.limx = g.H + 2 * g.h - g.w;
+  g.limy = g.W + 2 * g.h - g.w;
+  g.P = g.n * g.n;
+  g.E = (int)E;
+  g.M = (int)M;
+  g.J = g.M * g.E;
+  g.x_shared = x_shared ? 1 : 0;
+  g.Ek = kernel_per_energy ? g.E : 1;
+  if (g.limx < 0 || g.limy < 0)
+    return fail(kInvalidArgument, "patch window of " + std::to_string(g.w) +
+                                      " does not fit the padded grid");
+  if (!linpatch_length_supported(g.L))
+    return fail(kUnsupported, "FFT length " + std::to_string(g.L) + " (window " +
+                                  std::to_string(g.w) + " + 2 * margin " +
+                                  std::to_string(g.m) + ") is not compiled in");
+  auto* p = new jubik_linpatch();
+  p->dtype = dtype;
+  p->g = g;
+  if (hipGetDevice(&p->device) != hipSuccess) {
+    delete p;
+    return fail(kRuntimeError, "no HIP device available");
+  }
+  const size_t es = elem_size(dtype);
+  const size_t cs = 2 * es;
+  const size_t nspec = (size_t)g.M * g.P * g.Ek;
+  p->spec_bytes = nspec * g.L2 * g.L * cs;
+  p->ws_bytes = (size_t)g.J * g.P * g.L2 * g.L * cs;
+  int st = upload_twiddles(&p->tw, dtype, g.L);
+  if (!st) st = upload_tent(&p->tent, dtype, g.d);
+  if (!st) st = dev_alloc(&p->KT, p->spec_bytes, "kernel spectra");
+  if (!st) st = dev_alloc(&p->R1, std::max(p->ws_bytes, nspec * g.L2 * g.w * cs), "workspace R1");
+  if (!st) st = dev_alloc(&p->R2, p->ws_bytes, "workspace R2");
+  if (st) {
+    p->release();
+    delete p;
+    return st;
+  }
+  *plan = p;
+  return kOk;
+}
+
+int jubik_linpatch_destroy(jubik_linpatch* plan) {
+  if (!plan) return kOk;
+  DeviceGuard dg(plan->device);
+  plan->release();
+  delete plan;
+  return kOk;
+}
+
+int jubik_linpatch_info(const jubik_linpatch* plan, int64_t* out) {
+  if (!plan || !out) return fail(kInvalidArgument, "NULL argument");
+  const LinGeom& g = plan->g;
+  out[0] = g.d;
+  out[1] = g.w;
+  out[2] = g.L;
+  out[3] = g.P;
+  out[4] = g.h;
+  out[5] = g.J;
+  out[6] = (int64_t)(2 * plan->ws_bytes);
+  out[7] = (int64_t)plan->spec_bytes;
+  return kOk;
+}
+
+int jubik_linpatch_set_kernel(jubik_linpatch* plan, const void* kernel, int layout,
+                              void* stream) {
+  if (!plan || !kernel) return fail(kInvalidArgument, "NULL argument");
+  if (layout != 0 && layout != 1) return fail(kInvalidArgument, "layout must be 0 or 1");
+  const LinGeom& g = plan->g;
+  if (layout == 0) {
+    // convolve.py:107-111: the centred cut is empty when (H - 2dx) // 2 == 0
+    // and has the wrong size for odd frames; the reference fails there too.
+    if ((g.H - g.w) / 2 == 0 || (g.W - g.w) / 2 == 0)
+      return fail(kInvalidArgument, "kernel cut is empty (frame of twice the patch stride)");
+    if (g.H % 2 || g.W % 2)
+      return fail(kInvalidArgument, "full-frame kernels need an even grid");
+  }
+  DeviceGuard dg(plan->device);
+  int st = linpatch_prepare(plan->dtype, g, plan->bufs(), kernel, layout, as_stream(stream));
+  if (st == kOk) plan->ready = true;
+  return st;
+}
+
+int jubik_linpatch_forward(jubik_linpatch* plan, const void* x, void* y, double scale,
+                           void* stream) {
+  if (!plan || !x || !y) return fail(kInvalidArgument, "NULL argument");
+  if (!plan->ready) return fail(kNotReady, "no kernel set (call jubik_linpatch_set_kernel)");
+  DeviceGuard dg(plan->device);
+  return linpatch_forward(plan->dtype, plan->g, plan->bufs(), x, y, scale, as_stream(stream));
+}
+
+int jubik_linpatch_adjoint(jubik_linpatch* plan, const void* gy, void* gx, double scale,
+                           void* stream) {
+  if (!plan || !gy || !gx) return fail(kInvalidArgument, "NULL argument");
+  if (!plan->ready) return fail(kNotReady, "no kernel set (call jubik_linpatch_set_kernel)");
+  DeviceGuard dg(plan->device);
+  return linpatch_adjoint(plan->dtype, plan->g, plan->bufs(), gy, gx, scale, as_stream(stream));
+}
+
+// ------------------------------------------------------------------ response
+int jubik_exposure_apply(int dtype, const void* exposure, int64_t n_e, const void* x,
+                         int64_t n_x, void* out, int64_t n_out, void* stream) {
+  if (dtype != 0 && dtype != 1) return fail(kInvalidArgument, "dtype must be 0 or 1");
+  if (n_out < 0 || n_x <= 0 || n_e <= 0 || (n_out % n_x) != 0 || (n_out % n_e) != 0)
+    return fail(kInvalidArgument, "operand sizes must divide the output size");
+  if (n_out && (!exposure || !x || !out)) return fail(kInvalidArgument, "NULL argument");
+  return exposure_apply(dtype, exposure, n_e, x, n_x, out, n_out, as_stream(stream));
+}
+
+int jubik_readout_gather(int dtype, const void* x, const int64_t* index, int64_t nnz, void* out,
+                         void* stream) {
+  if (dtype != 0 && dtype != 1) return fail(kInvalidArgument, "dtype must be 0 or 1");
+  if (nnz < 0) return fail(kInvalidArgument, "negative size");
+  if (nnz && (!x || !index || !out)) return fail(kInvalidArgument, "NULL argument");
+  return readout_gather(dtype, x, index, nnz, out, as_stream(stream));
+}
+
+int jubik_readout_scatter(int dtype, const void* v, const int64_t* index, int64_t nnz, int64_t n,
+                          void* out, void* stream) {
+  if (dtype != 0 && dtype != 1) return fail(kInvalidArgument, "dtype must be 0 or 1");
+  if (nnz < 0 || n < 0 || nnz > n) return fail(kInvalidArgument, "invalid sizes");
+  if (n && !out) return fail(kInvalidArgument, "NULL argument");
+  if (nnz && (!v || !index)) return fail(kInvalidArgument, "NULL argument");
+  return readout_scatter(dtype, v, index, nnz, n, out, as_stream(stream));
+}
+
+int jubik_poisson_nll_grad(int dtype, const void* lam, const void* data, int64_t n, double* nll,
+                           void* grad, void* work, void* stream) {
+  if (dtype != 0 && dtype != 1) return fail(kInvalidArgument, "dtype must be 0 or 1");
+  if (n < 0) return fail(kInvalidArgument, "negative size");
+  if (!nll || !work || (n && (!lam || !data))) return fail(kInvalidArgument, "NULL argument");
+  return poisson_nll_grad(dtype, lam, data, n, nll, grad, (double*)work, as_stream(stream));
+}
+
+// ------------------------------------------------------------------ GP
+int jubik_gp_create(jubik_gp** plan, int dtype, int64_t N, int64_t H, int64_t W, int64_t E,
+                    int has_alpha) {
+  if (!plan) return fail(kInvalidArgument, "plan pointer is NULL");
+  *plan = nullptr;
+  if (dtype != 0 && dtype != 1) return fail(kInvalidArgument, "dtype must be 0 or 1");
+  if (E < 1 || H < 1 || W < 1 || H > N || W > N)
+    return fail(kInvalidArgument, "sky grid must fit in the GP grid");
+  if (!gp_length_supported((int)N))
+    return fail(kUnsupported, "GP grid size " + std::to_string(N) + " is not compiled in");
+  if (dtype == 1 && N > 1024) return fail(kUnsupported, "float64 GP grids are compiled up to 1024");
+  if (!has_alpha && E != 1)
+    return fail(kInvalidArgument, "several energies need the spectral-index field");
+  auto* p = new jubik_gp();
+  p->dtype = dtype;
+  p->g = GpGeom{(int)N, (int)H, (int)W, (int)E, has_alpha ? 1 : 0, 0.0, 0.0};
+  if (hipGetDevice(&p->device) != hipSuccess) {
+    delete p;
+    return fail(kRuntimeError, "no HIP device available");
+  }
+  const size_t cs = 2 * elem_size(dtype);
+  int st = upload_twiddles(&p->tw, dtype, (int)N);
+  if (!st) st = dev_alloc(&p->work, (size_t)N * N * cs, "GP workspace");
+  if (!st) st = dev_alloc(&p->u, (size_t)H * W * cs, "GP cotangent");
+  if (st) {
+    p->release();
+    delete p;
+    return st;
+  }
+  *plan = p;
+  return kOk;
+}
+
+int jubik_gp_destroy(jubik_gp* plan) {
+  if (!plan) return kOk;
+  DeviceGuard dg(plan->device);
+  plan->release();
+  delete plan;
+  return kOk;
+}
+
+int jubik_gp_forward(jubik_gp* plan, const void* amp_ref, const void* amp_alpha,
+                     const void* xi_ref, const void* xi_alpha, double zm, double mu,
+                     const void* dnu, void* sky, void* stream) {
+  if (!plan || !amp_ref || !xi_ref || !sky) return fail(kInvalidArgument, "NULL argument");
+  if (plan->g.has_alpha && (!amp_alpha || !xi_alpha || !dnu))
+    return fail(kInvalidArgument, "spectral-index inputs are NULL");
+  DeviceGuard dg(plan->device);
+  GpGeom g = plan->g;
+  g.zm = zm;
+  g.mu = mu;
+  return gp_forward(plan->dtype, g, plan->bufs(), amp_ref, amp_alpha, xi_ref, xi_alpha, dnu, sky,
+                    as_stream(stream));
+}
+
+int jubik_gp_vjp(jubik_gp* plan, const void* amp_ref, const void* amp_alpha, const void* dnu,
+                 const void* sky, const void* cot_sky, void* g_ref, void* g_alpha, void* stream) {
+  if (!plan || !amp_ref || !sky || !cot_sky || !g_ref)
+    return fail(kInvalidArgument, "NULL argument");
+  if (plan->g.has_alpha && (!amp_alpha || !dnu || !g_alpha))
+    return fail(kInvalidArgument, "spectral-index inputs are NULL");
+  DeviceGuard dg(plan->device);
+  int st = gp_fold(plan->dtype, plan->g, sky, cot_sky, dnu, plan->u, as_stream(stream));
+  if (st) return st;
+  return gp_adjoint_u(plan->dtype, plan->g, plan->bufs(), amp_ref, amp_alpha, plan->u, g_ref,
+                      g_alpha, as_stream(stream));
+}
+
+// ------------------------------------------------------------------ model
+int jubik_model_create(jubik_model** model, jubik_gp* gp, jubik_linpatch* lp, double pixel_area) {
+  if (!model || !gp || !lp) return fail(kInvalidArgument, "NULL argument");
+  *model = nullptr;
+  if (gp->dtype != lp->dtype) return fail(kInvalidArgument, "gp and linpatch dtypes differ");
+  if (gp->device != lp->device) return fail(kInvalidArgument, "gp and linpatch devices differ");
+  const LinGeom& g = lp->g;
+  if (gp->g.H != g.H || gp->g.W != g.W || gp->g.E != g.E)
+    return fail(kInvalidArgument, "gp sky grid and linpatch grid differ");
+  if (!g.x_shared) return fail(kInvalidArgument, "the model needs a linpatch plan with x_shared");
+  auto* m = new jubik_model();
+  m->gp = gp;
+  m->lp = lp;
+  m->pixel_area = pixel_area;
+  DeviceGuard dg(lp->device);
+  const size_t es = elem_size(lp->dtype);
+  const size_t img = (size_t)g.H * g.W;
+  int st = dev_alloc(&m->sky, img * g.E * es, "sky");
+  if (!st) st = dev_alloc(&m->dsky, img * g.E * es, "sky tangent");
+  if (!st) st = dev_alloc(&m->resid, img * g.J * es, "residual");
+  if (!st) st = dev_alloc(&m->lam, img * g.J * es, "lambda");
+  if (!st) st = dev_alloc(&m->partial, (size_t)g.J * g.H * sizeof(double), "NLL partials");
+  if (!st) st = dev_alloc(&m->u, img * 2 * es, "GP cotangent");
+  if (st) {
+    m->release();
+    delete m;
+    return st;
+  }
+  *model = m;
+  return kOk;
+}
+
+int jubik_model_destroy(jubik_model* model) {
+  if (!model) return kOk;
+  DeviceGuard dg(model->lp->device);
+  model->release();
+  delete model;
+  return kOk;
+}
+
+int jubik_model_set_data(jubik_model* model, const void* exposure, const int32_t* counts) {
+  if (!model || !exposure || !counts) return fail(kInvalidArgument, "NULL argument");
+  model->expo = exposure;
+  model->counts = counts;
+  model->have_point = false;
+  return kOk;
+}
+
+int jubik_model_grad(jubik_model* model, const void* amp_ref, const void* amp_alpha,
+                     const void* xi_ref, const void* xi_alpha, double zm, double mu,
+                     const void* dnu, double* nll, void* g_ref, void* g_alpha, void* stream) {
+  if (!model || !nll || !g_ref) return fail(kInvalidArgument, "NULL argument");
+  if (!model->expo) return fail(kNotReady, "no data set (call jubik_model_set_data)");
+  jubik_linpatch* lp = model->lp;
+  jubik_gp* gp = model->gp;
+  if (!lp->ready) return fail(kNotReady, "no kernel set (call jubik_linpatch_set_kernel)");
+  DeviceGuard dg(lp->device);
+  hipStream_t s = as_stream(stream);
+  int st = jubik_gp_forward(gp, amp_ref, amp_alpha, xi_ref, xi_alpha, zm, mu, dnu, model->sky,
+                            stream);
+  if (st) return st;
+  const LinGeom& g = lp->g;
+  st = linpatch_forward_poisson(lp->dtype, g, lp->bufs(), model->sky, model->pixel_area,
+                                model->expo, model->counts, model->resid, model->lam,
+                                (double*)model->partial, s);
+  if (st) return st;
+  st = reduce_partials((const double*)model->partial, (int64_t)g.J * g.H, nll, s);
+  if (st) return st;
+  st = linpatch_adjoint_gp(lp->dtype, g, lp->bufs(), model->resid, model->pixel_area, model->sky,
+                           dnu, model->u, s);
+  if (st) return st;
+  st = gp_adjoint_u(gp->dtype, gp->g, gp->bufs(), amp_ref, amp_alpha, model->u, g_ref, g_alpha,
+                    s);
+  if (st == kOk) model->have_point = true;
+  return st;
+}
+
+int jubik_model_metric(jubik_model* model, const void* amp_ref, const void* amp_alpha,
+                       const void* dnu, const void* v_ref, const void* v_alpha, void* out_ref,
+                       void* out_alpha, void* stream) {
+  if (!model || !v_ref || !out_ref) return fail(kInvalidArgument, "NULL argument");
+  if (!model->have_point)
+    return fail(kNotReady, "no base point (call jubik_model_grad first)");
+  jubik_linpatch* lp = model->lp;
+  jubik_gp* gp = model->gp;
+  DeviceGuard dg(lp->device);
+  hipStream_t s = as_stream(stream);
+  int st = gp_tangent(gp->dtype, gp->g, gp->bufs(), amp_ref, amp_alpha, v_ref, v_alpha, dnu,
+                      model->sky, model->dsky, s);
+  if (st) return st;
+  const LinGeom& g = lp->g;
+  st = linpatch_forward_metric(lp->dtype, g, lp->bufs(), model->dsky, model->pixel_area,
+                               model->expo, model->counts, model->lam, model->resid, s);
+  if (st) return st;
+  st = linpatch_adjoint_gp(lp->dtype, g, lp->bufs(), model->resid, model->pixel_area, model->sky,
+                           dnu, model->u, s);
+  if (st) return st;
+  return gp_adjoint_u(gp->dtype, gp->g, gp->bufs(), amp_ref, amp_alpha, model->u, out_ref,
+                      out_alpha, s);
+}
+
+}  // extern "C"

@@ -1,0 +1,87 @@
+// Shared definitions of the jubik HIP library: status codes, error text,
+// launch checking and the geometry of the linear-patching PSF operator.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+
+#include "fft.hpp"
+
+namespace jubik {
+
+enum Status : int {
+  kOk = 0,
+  kInvalidArgument = -1,
+  kUnsupported = -2,
+  kRuntimeError = -3,
+  kOutOfMemory = -4,
+  kNotReady = -5,
+};
+
+void set_error(const std::string& msg);
+int fail(int code, const std::string& msg);
+int check_hip(hipError_t e, const char* what);
+
+#define JUBIK_HIP(call)                                  \
+  do {                                                   \
+    hipError_t _e = (call);                              \
+    if (_e != hipSuccess) return check_hip(_e, #call);   \
+  } while (0)
+
+// Geometry of linpatch_convolve on one (H, W) grid with n patches per axis
+// (jubik/convolve.py:66-145).  Windows are w = 2d wide with stride d; the
+// slicing pads the grid by h = d / 2 zeros; every window is padded by the
+// margin m to the FFT length L = w + 2m.  Window starts are
+// min(i * d, lim), the clamping of jax.lax.dynamic_slice; the same starts
+// serve the slicing of the image and the overlap-add canvas.
+struct LinGeom {
+  int H, W;       // sky grid
+  int n;          // patches per axis
+  int d;          // patch stride (equal on both axes)
+  int h;          // zero padding of the slicing, d / 2
+  int w;          // window 2d
+  int m;          // margin
+  int L;          // FFT length w + 2m
+  int L2;         // L / 2 + 1 half-spectrum length
+  int limx, limy; // clamp of the window starts: H + 2h - w, W + 2h - w
+  int P;          // n * n
+  int E;          // energies per module
+  int M;          // modules
+  int J;          // jobs M * E
+  int x_shared;   // 1: the input image of job (mod, e) is x[e]
+  int Ek;         // kernel energies: E, or 1 when one kernel serves all
+};
+
+__device__ __forceinline__ int start_x(const LinGeom& g, int i) { return min(i * g.d, g.limx); }
+__device__ __forceinline__ int start_y(const LinGeom& g, int i) { return min(i * g.d, g.limy); }
+// Spectrum index of (job, patch): kernels are laid out (M, P, Ek).
+__device__ __forceinline__ int64_t spec_index(const LinGeom& g, int job, int p) {
+  const int mod = job / g.E, e = job - mod * g.E;
+  return ((int64_t)mod * g.P + p) * g.Ek + (g.Ek == 1 ? 0 : e);
+}
+
+// Tile sizes: vectors per work-group so a tile of V x L complex values
+// occupies at most ~48 KiB of LDS.
+template <typename T, int L>
+struct Tile {
+  static constexpr int bytes = 49152;
+  static constexpr int raw = bytes / (L * (int)sizeof(cplx<T>));
+  static constexpr int V = raw >= 32 ? 32 : raw >= 16 ? 16 : raw >= 8 ? 8 : raw >= 4 ? 4 : raw >= 2 ? 2 : 1;
+};
+
+// Block-wide sum of one double per thread (NT a multiple of 64).
+template <int NT>
+__device__ double block_sum(double v, double* red) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) red[wid] = v;
+  __syncthreads();
+  double s = 0.0;
+  if (threadIdx.x == 0)
+    for (int i = 0; i < NT / 64; ++i) s += red[i];
+  return s;  // valid in thread 0
+}
+
+}  // namespace jubik

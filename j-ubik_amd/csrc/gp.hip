@@ -1,0 +1,238 @@
+// Correlated-field GP sky on an N x N grid (fixed amplitude field):
+//
+//   log s_e = zm + HT(amp_ref * xi_ref) + (HT(amp_alpha * xi_alpha) + mu) * dnu_e
+//
+// (jubik/sky_model/multifrequency/spectral_product_mf_sky.py:171-219,281-298;
+// HT(u) = Re(FFT u) + Im(FFT u), nifty.re's hartley).  Both real transforms
+// share one complex 2-D FFT of c = amp_ref xi_ref + i amp_alpha xi_alpha:
+// with Z = FFT(c) and Z' = Z(-k),
+//   HT(a) = (Z.re + Z'.re + Z.im - Z'.im) / 2,
+//   HT(b) = (Z.im + Z'.im - Z.re + Z'.re) / 2.
+// Pass 1 transforms columns (a tile of adjacent columns per work-group);
+// pass 2 transforms the row pair (k, -k) in one work-group, so the pairing
+// needs no extra pass, and applies the epilogue (exp + crop to the sky grid,
+// or the amplitude multiply of the adjoint, or the tangent map).
+#include "gp.hpp"
+
+namespace jubik {
+
+constexpr int GNT = 256;
+
+// grid (N / V).  MODE 0: c = amp * xi (forward / tangent);  MODE 1: c = u
+// (complex (H, W) cotangent, zero outside the sky grid).
+template <typename T, int N, int MODE>
+__global__ __launch_bounds__(GNT) void gp_cols(GpGeom g, const T* __restrict__ amp_ref,
+                                               const T* __restrict__ amp_alpha,
+                                               const T* __restrict__ xi_ref,
+                                               const T* __restrict__ xi_alpha,
+                                               const cplx<T>* __restrict__ u,
+                                               const cplx<T>* __restrict__ twg,
+                                               cplx<T>* __restrict__ Tw) {
+  constexpr int V = Tile<T, N>::V;
+  __shared__ cplx<T> buf[V * N];
+  __shared__ cplx<T> tw[N];
+  const int tid = threadIdx.x;
+  const int c0 = blockIdx.x * V;
+  load_twiddles(tw, twg, N, tid, GNT);
+  for (int e = tid; e < V * N; e += GNT) {
+    const int x1 = e / V, v = e - (e / V) * V;
+    const int x2 = c0 + v;
+    cplx<T> val = {T(0), T(0)};
+    if constexpr (MODE == 0) {
+      const int64_t o = (int64_t)x1 * N + x2;
+      val.re = amp_ref[o] * xi_ref[o];
+      if (g.has_alpha) val.im = amp_alpha[o] * xi_alpha[o];
+    } else {
+      if (x1 < g.H && x2 < g.W) val = u[(int64_t)x1 * g.W + x2];
+    }
+    buf[v * N + x1] = val;
+  }
+  __syncthreads();
+  fft_lds<T, N, V, N, GNT, false>(buf, tw, tid);
+  for (int e = tid; e < V * N; e += GNT) {
+    const int x1 = e / V, v = e - (e / V) * V;
+    Tw[(int64_t)x1 * N + c0 + v] = buf[v * N + x1];
+  }
+}
+
+// grid (N / 2 + 1).  MODE 0: sky = exp(log s) cropped to (E, H, W);
+// MODE 1: g_ref = amp_ref HT(u_ref), g_alpha = amp_alpha HT(u_alpha) on N x N;
+// MODE 2: tangent ds_e = sky_e (HT(a) + HT(b) dnu_e) on (E, H, W).
+template <typename T, int N, int MODE>
+__global__ __launch_bounds__(GNT) void gp_rows(GpGeom g, const cplx<T>* __restrict__ Tw,
+                                               const cplx<T>* __restrict__ twg,
+                                               const T* __restrict__ dnu,
+                                               const T* __restrict__ amp_ref,
+                                               const T* __restrict__ amp_alpha,
+                                               const T* __restrict__ sky_in,
+                                               T* __restrict__ out_a, T* __restrict__ out_b) {
+  __shared__ cplx<T> buf[2 * N];
+  __shared__ cplx<T> tw[N];
+  const int tid = threadIdx.x;
+  const int r0 = blockIdx.x;
+  const int r1 = (N - r0) % N;
+  load_twiddles(tw, twg, N, tid, GNT);
+  for (int e = tid; e < 2 * N; e += GNT) {
+    const int v = e / N, k = e - (e / N) * N;
+    buf[e] = Tw[(int64_t)(v ? r1 : r0) * N + k];
+  }
+  __syncthreads();
+  fft_lds<T, N, 2, N, GNT, false>(buf, tw, tid);
+  const int nrows = (r0 == r1) ? 1 : 2;
+  for (int e = tid; e < nrows * N; e += GNT) {
+    const int v = e / N, k = e - (e / N) * N;
+    const int x = v ? r1 : r0;
+    const cplx<T> z = buf[v * N + k];
+    const cplx<T> zp = buf[(1 - v) * N + (k == 0 ? 0 : N - k)];
+    const T hta = T(0.5) * (z.re + zp.re + z.im - zp.im);
+    const T htb = T(0.5) * (z.im + zp.im - z.re + zp.re);
+    if constexpr (MODE == 1) {
+      const int64_t o = (int64_t)x * N + k;
+      out_a[o] = amp_ref[o] * hta;
+      if (g.has_alpha) out_b[o] = amp_alpha[o] * htb;
+    } else {
+      if (x >= g.H || k >= g.W) continue;
+      for (int en = 0; en < g.E; ++en) {
+        const int64_t o = ((int64_t)en * g.H + x) * g.W + k;
+        if constexpr (MODE == 0) {
+          T ls = T(g.zm) + hta;
+          if (g.has_alpha) ls += (htb + T(g.mu)) * dnu[en];
+          out_a[o] = exp(ls);
+        } else {
+          T dl = hta;
+          if (g.has_alpha) dl += htb * dnu[en];
+          out_a[o] = sky_in[o] * dl;
+        }
+      }
+    }
+  }
+}
+
+// u(H, W) = sum_e sky_e * cot_e * (1, dnu_e): cotangent of log s folded over
+// the energies (transpose of the energy broadcast).
+template <typename T>
+__global__ __launch_bounds__(GNT) void gp_fold(GpGeom g, const T* __restrict__ sky,
+                                               const T* __restrict__ cot,
+                                               const T* __restrict__ dnu,
+                                               cplx<T>* __restrict__ u) {
+  const int64_t n = (int64_t)g.H * g.W;
+  for (int64_t i = blockIdx.x * (int64_t)GNT + threadIdx.x; i < n; i += (int64_t)gridDim.x * GNT) {
+    T a = T(0), b = T(0);
+    for (int e = 0; e < g.E; ++e) {
+      const T v = sky[e * n + i] * cot[e * n + i];
+      a += v;
+      if (g.has_alpha) b += v * dnu[e];
+    }
+    u[i] = cplx<T>{a, b};
+  }
+}
+
+// ====================================================================== host
+#define JUBIK_GP_LENGTHS(X) X(16) X(32) X(64) X(128) X(256) X(512) X(768) X(1024) X(2048)
+
+bool gp_length_supported(int N) {
+  switch (N) {
+#define JUBIK_CASE(l) case l:
+    JUBIK_GP_LENGTHS(JUBIK_CASE)
+#undef JUBIK_CASE
+    return true;
+    default:
+      return false;
+  }
+}
+
+template <typename T, int N>
+struct GpOps {
+  static constexpr int V = Tile<T, N>::V;
+  using C = cplx<T>;
+  static int forward(const GpGeom& g, const GpBuffers& b, const void* amp_ref,
+                     const void* amp_alpha, const void* xi_ref, const void* xi_alpha,
+                     const void* dnu, void* sky, hipStream_t s) {
+    gp_cols<T, N, 0><<<N / V, GNT, 0, s>>>(g, (const T*)amp_ref, (const T*)amp_alpha,
+                                           (const T*)xi_ref, (const T*)xi_alpha, nullptr,
+                                           (const C*)b.tw, (C*)b.work);
+    gp_rows<T, N, 0><<<N / 2 + 1, GNT, 0, s>>>(g, (const C*)b.work, (const C*)b.tw,
+                                               (const T*)dnu, nullptr, nullptr, nullptr,
+                                               (T*)sky, nullptr);
+    return check_hip(hipGetLastError(), "gp_forward");
+  }
+  static int tangent(const GpGeom& g, const GpBuffers& b, const void* amp_ref,
+                     const void* amp_alpha, const void* v_ref, const void* v_alpha,
+                     const void* dnu, const void* sky, void* dsky, hipStream_t s) {
+    gp_cols<T, N, 0><<<N / V, GNT, 0, s>>>(g, (const T*)amp_ref, (const T*)amp_alpha,
+                                           (const T*)v_ref, (const T*)v_alpha, nullptr,
+                                           (const C*)b.tw, (C*)b.work);
+    gp_rows<T, N, 2><<<N / 2 + 1, GNT, 0, s>>>(g, (const C*)b.work, (const C*)b.tw,
+                                               (const T*)dnu, nullptr, nullptr, (const T*)sky,
+                                               (T*)dsky, nullptr);
+    return check_hip(hipGetLastError(), "gp_tangent");
+  }
+  static int adjoint_u(const GpGeom& g, const GpBuffers& b, const void* amp_ref,
+                       const void* amp_alpha, const void* u, void* g_ref, void* g_alpha,
+                       hipStream_t s) {
+    gp_cols<T, N, 1><<<N / V, GNT, 0, s>>>(g, nullptr, nullptr, nullptr, nullptr, (const C*)u,
+                                           (const C*)b.tw, (C*)b.work);
+    gp_rows<T, N, 1><<<N / 2 + 1, GNT, 0, s>>>(g, (const C*)b.work, (const C*)b.tw, nullptr,
+                                               (const T*)amp_ref, (const T*)amp_alpha, nullptr,
+                                               (T*)g_ref, (T*)g_alpha);
+    return check_hip(hipGetLastError(), "gp_adjoint");
+  }
+  static int fold(const GpGeom& g, const void* sky, const void* cot, const void* dnu, void* u,
+                  hipStream_t s) {
+    const int64_t n = (int64_t)g.H * g.W;
+    const int blocks = (int)std::min<int64_t>((n + GNT - 1) / GNT, 4096);
+    gp_fold<T><<<blocks, GNT, 0, s>>>(g, (const T*)sky, (const T*)cot, (const T*)dnu, (C*)u);
+    return check_hip(hipGetLastError(), "gp_fold");
+  }
+};
+
+template <typename T, typename F>
+static int gp_dispatch_len(int N, F&& f) {
+  if (sizeof(T) == 8 && N > 1024)
+    return fail(kUnsupported, "float64 GP grids are compiled up to 1024");
+  switch (N) {
+#define JUBIK_CASE(l) \
+  case l:             \
+    return f(GpOps<T, l>{});
+    JUBIK_GP_LENGTHS(JUBIK_CASE)
+#undef JUBIK_CASE
+    default:
+      return fail(kUnsupported, "GP grid size " + std::to_string(N) + " is not compiled in");
+  }
+}
+
+template <typename F>
+static int gp_dispatch(int dtype, int N, F&& f) {
+  if (dtype == 0) return gp_dispatch_len<float>(N, f);
+  if (dtype == 1) return gp_dispatch_len<double>(N, f);
+  return fail(kInvalidArgument, "dtype must be 0 (float32) or 1 (float64)");
+}
+
+int gp_forward(int dtype, const GpGeom& g, const GpBuffers& b, const void* amp_ref,
+               const void* amp_alpha, const void* xi_ref, const void* xi_alpha, const void* dnu,
+               void* sky, hipStream_t s) {
+  return gp_dispatch(dtype, g.N, [&](auto ops) {
+    return decltype(ops)::forward(g, b, amp_ref, amp_alpha, xi_ref, xi_alpha, dnu, sky, s);
+  });
+}
+int gp_tangent(int dtype, const GpGeom& g, const GpBuffers& b, const void* amp_ref,
+               const void* amp_alpha, const void* v_ref, const void* v_alpha, const void* dnu,
+               const void* sky, void* dsky, hipStream_t s) {
+  return gp_dispatch(dtype, g.N, [&](auto ops) {
+    return decltype(ops)::tangent(g, b, amp_ref, amp_alpha, v_ref, v_alpha, dnu, sky, dsky, s);
+  });
+}
+int gp_adjoint_u(int dtype, const GpGeom& g, const GpBuffers& b, const void* amp_ref,
+                 const void* amp_alpha, const void* u, void* g_ref, void* g_alpha,
+                 hipStream_t s) {
+  return gp_dispatch(dtype, g.N, [&](auto ops) {
+    return decltype(ops)::adjoint_u(g, b, amp_ref, amp_alpha, u, g_ref, g_alpha, s);
+  });
+}
+int gp_fold(int dtype, const GpGeom& g, const void* sky, const void* cot, const void* dnu,
+            void* u, hipStream_t s) {
+  return gp_dispatch(dtype, g.N,
+                     [&](auto ops) { return decltype(ops)::fold(g, sky, cot, dnu, u, s); });
+}
+
+}  // namespace jubik

@@ -1,0 +1,32 @@
+// Host-side launchers of the GP sky kernels.
+#pragma once
+#include "common.hpp"
+
+namespace jubik {
+
+struct GpGeom {
+  int N;          // GP grid N x N
+  int H, W;       // sky grid (crop of the GP grid at the origin)
+  int E;          // energies
+  int has_alpha;  // spectral-index field present
+  double zm, mu;  // zero mode and spectral-index mean
+};
+
+struct GpBuffers {
+  const void* tw;  // N twiddles
+  void* work;      // N x N complex
+};
+
+bool gp_length_supported(int N);
+int gp_forward(int dtype, const GpGeom& g, const GpBuffers& b, const void* amp_ref,
+               const void* amp_alpha, const void* xi_ref, const void* xi_alpha, const void* dnu,
+               void* sky, hipStream_t s);
+int gp_tangent(int dtype, const GpGeom& g, const GpBuffers& b, const void* amp_ref,
+               const void* amp_alpha, const void* v_ref, const void* v_alpha, const void* dnu,
+               const void* sky, void* dsky, hipStream_t s);
+int gp_adjoint_u(int dtype, const GpGeom& g, const GpBuffers& b, const void* amp_ref,
+                 const void* amp_alpha, const void* u, void* g_ref, void* g_alpha, hipStream_t s);
+int gp_fold(int dtype, const GpGeom& g, const void* sky, const void* cot, const void* dnu,
+            void* u, hipStream_t s);
+
+}  // namespace jubik

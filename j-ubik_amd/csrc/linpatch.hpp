@@ -1,0 +1,44 @@
+// Host-side launchers of the linear-patching PSF kernels.
+#pragma once
+#include "common.hpp"
+
+namespace jubik {
+
+// Device buffers owned by a linpatch plan (all typed by the plan's dtype).
+struct LinBuffers {
+  const void* tw;    // L twiddles exp(-2 pi i e / L)
+  const void* tent;  // w tent weights (jubik/convolve.py:18-25)
+  void* KT;          // prepared spectra, (M, P, Ek) x L2 x L, column-major
+  void* R1;          // row-pass workspace, J x P x L2 x L
+  void* R2;          // column-pass workspace, J x P x L x L2
+};
+
+bool linpatch_length_supported(int L);
+
+// dtype: 0 = float32, 1 = float64.  layout: 0 = full frame (H, W), 1 = cut
+// window (w, w).  Kernels are laid out (M, P, Ek, ...).
+int linpatch_prepare(int dtype, const LinGeom& g, const LinBuffers& b, const void* kernel,
+                     int layout, hipStream_t s);
+// y (J, H, W) = PSF(scale * x);  x is (E, H, W) when g.x_shared else (J, H, W).
+int linpatch_forward(int dtype, const LinGeom& g, const LinBuffers& b, const void* x, void* y,
+                     double scale, hipStream_t s);
+// Forward fused with exposure and the Poisson residual: lam = expo * PSF,
+// resid = expo * (1 - d / lam) where d >= 0 (else 0), NLL partials per row.
+int linpatch_forward_poisson(int dtype, const LinGeom& g, const LinBuffers& b, const void* x,
+                             double scale, const void* expo, const int* counts, void* resid,
+                             void* lam, double* partial, hipStream_t s);
+// Tangent forward for the MGVI metric: cot = expo * (expo * PSF(scale * dx)) /
+// lam where d >= 0 (else 0); lam is the forward output at the base point.
+int linpatch_forward_metric(int dtype, const LinGeom& g, const LinBuffers& b, const void* dx,
+                            double scale, const void* expo, const int* counts, const void* lam,
+                            void* cot, hipStream_t s);
+// gx = scale * PSF^T(gy); gy (J, H, W); gx (E, H, W) when g.x_shared (summed
+// over modules) else (J, H, W).
+int linpatch_adjoint(int dtype, const LinGeom& g, const LinBuffers& b, const void* gy, void* gx,
+                     double scale, hipStream_t s);
+// Adjoint chained into the GP: u = sum_jobs scale * sky[e] * PSF^T(gy)_job *
+// (1, dnu[e]) stored as complex (H, W).
+int linpatch_adjoint_gp(int dtype, const LinGeom& g, const LinBuffers& b, const void* gy,
+                        double scale, const void* sky, const void* dnu, void* u, hipStream_t s);
+
+}  // namespace jubik

@@ -1,0 +1,138 @@
+// Exposure, read-out and Poisson likelihood kernels
+// (jubik/response.py:12-112; the NLL formula of nifty.re's Poissonian as
+// documented at demos/x-ray-imaging.py:312-313).  Reductions run in FP64 in
+// a fixed order: per-block partial sums, then one block sums the partials.
+#include "response.hpp"
+
+namespace jubik {
+
+constexpr int RNT = 256;
+constexpr int RBLOCKS = 1024;  // grid of the reducing kernels (partials per call)
+
+template <typename T>
+__global__ __launch_bounds__(RNT) void k_exposure(const T* __restrict__ expo, int64_t n_e,
+                                                  const T* __restrict__ x, int64_t n_x,
+                                                  T* __restrict__ out, int64_t n_out) {
+  for (int64_t i = blockIdx.x * (int64_t)RNT + threadIdx.x; i < n_out;
+       i += (int64_t)gridDim.x * RNT)
+    out[i] = expo[n_e == n_out ? i : i % n_e] * x[n_x == n_out ? i : i % n_x];
+}
+
+template <typename T>
+__global__ __launch_bounds__(RNT) void k_gather(const T* __restrict__ x,
+                                                const int64_t* __restrict__ idx,
+                                                T* __restrict__ out, int64_t nnz) {
+  for (int64_t k = blockIdx.x * (int64_t)RNT + threadIdx.x; k < nnz;
+       k += (int64_t)gridDim.x * RNT)
+    out[k] = x[idx[k]];
+}
+
+template <typename T>
+__global__ __launch_bounds__(RNT) void k_fill(T* __restrict__ out, int64_t n) {
+  for (int64_t k = blockIdx.x * (int64_t)RNT + threadIdx.x; k < n; k += (int64_t)gridDim.x * RNT)
+    out[k] = T(0);
+}
+
+template <typename T>
+__global__ __launch_bounds__(RNT) void k_scatter(const T* __restrict__ v,
+                                                 const int64_t* __restrict__ idx,
+                                                 T* __restrict__ out, int64_t nnz) {
+  for (int64_t k = blockIdx.x * (int64_t)RNT + threadIdx.x; k < nnz;
+       k += (int64_t)gridDim.x * RNT)
+    out[idx[k]] = v[k];
+}
+
+// partial[b] = sum over this block's elements of (lam - d log lam);
+// grad = 1 - d / lam.
+template <typename T>
+__global__ __launch_bounds__(RNT) void k_poisson(const T* __restrict__ lam,
+                                                 const T* __restrict__ d, int64_t n,
+                                                 T* __restrict__ grad,
+                                                 double* __restrict__ partial) {
+  __shared__ double red[RNT / 64];
+  double acc = 0.0;
+  for (int64_t k = blockIdx.x * (int64_t)RNT + threadIdx.x; k < n;
+       k += (int64_t)gridDim.x * RNT) {
+    const T l = lam[k], dk = d[k];
+    acc += (double)l - (double)dk * log((double)l);
+    if (grad) grad[k] = T(1) - dk / l;
+  }
+  acc = block_sum<RNT>(acc, red);
+  if (threadIdx.x == 0) partial[blockIdx.x] = acc;
+}
+
+__global__ __launch_bounds__(RNT) void k_reduce(const double* __restrict__ partial, int64_t n,
+                                                double* __restrict__ out) {
+  __shared__ double red[RNT / 64];
+  double acc = 0.0;
+  for (int64_t k = threadIdx.x; k < n; k += RNT) acc += partial[k];
+  acc = block_sum<RNT>(acc, red);
+  if (threadIdx.x == 0) out[0] = acc;
+}
+
+static int grid_for(int64_t n) {
+  int64_t b = (n + RNT - 1) / RNT;
+  if (b < 1) b = 1;
+  if (b > 8192) b = 8192;
+  return (int)b;
+}
+
+template <typename T>
+static int exposure_t(const void* expo, int64_t n_e, const void* x, int64_t n_x, void* out,
+                      int64_t n_out, hipStream_t s) {
+  k_exposure<T><<<grid_for(n_out), RNT, 0, s>>>((const T*)expo, n_e, (const T*)x, n_x, (T*)out,
+                                                n_out);
+  return check_hip(hipGetLastError(), "exposure");
+}
+
+int exposure_apply(int dtype, const void* expo, int64_t n_e, const void* x, int64_t n_x,
+                   void* out, int64_t n_out, hipStream_t s) {
+  if (n_out <= 0) return kOk;
+  if (dtype == 0) return exposure_t<float>(expo, n_e, x, n_x, out, n_out, s);
+  return exposure_t<double>(expo, n_e, x, n_x, out, n_out, s);
+}
+
+int readout_gather(int dtype, const void* x, const int64_t* idx, int64_t nnz, void* out,
+                   hipStream_t s) {
+  if (nnz <= 0) return kOk;
+  if (dtype == 0)
+    k_gather<float><<<grid_for(nnz), RNT, 0, s>>>((const float*)x, idx, (float*)out, nnz);
+  else
+    k_gather<double><<<grid_for(nnz), RNT, 0, s>>>((const double*)x, idx, (double*)out, nnz);
+  return check_hip(hipGetLastError(), "readout_gather");
+}
+
+int readout_scatter(int dtype, const void* v, const int64_t* idx, int64_t nnz, int64_t n,
+                    void* out, hipStream_t s) {
+  if (n <= 0) return kOk;
+  if (dtype == 0) {
+    k_fill<float><<<grid_for(n), RNT, 0, s>>>((float*)out, n);
+    if (nnz > 0)
+      k_scatter<float><<<grid_for(nnz), RNT, 0, s>>>((const float*)v, idx, (float*)out, nnz);
+  } else {
+    k_fill<double><<<grid_for(n), RNT, 0, s>>>((double*)out, n);
+    if (nnz > 0)
+      k_scatter<double><<<grid_for(nnz), RNT, 0, s>>>((const double*)v, idx, (double*)out, nnz);
+  }
+  return check_hip(hipGetLastError(), "readout_scatter");
+}
+
+int poisson_nll_grad(int dtype, const void* lam, const void* d, int64_t n, double* nll,
+                     void* grad, double* work, hipStream_t s) {
+  const int blocks = (int)std::min<int64_t>(std::max<int64_t>((n + RNT - 1) / RNT, 1), RBLOCKS);
+  if (dtype == 0)
+    k_poisson<float><<<blocks, RNT, 0, s>>>((const float*)lam, (const float*)d, n, (float*)grad,
+                                            work);
+  else
+    k_poisson<double><<<blocks, RNT, 0, s>>>((const double*)lam, (const double*)d, n,
+                                             (double*)grad, work);
+  k_reduce<<<1, RNT, 0, s>>>(work, blocks, nll);
+  return check_hip(hipGetLastError(), "poisson_nll_grad");
+}
+
+int reduce_partials(const double* partial, int64_t n, double* out, hipStream_t s) {
+  k_reduce<<<1, RNT, 0, s>>>(partial, n, out);
+  return check_hip(hipGetLastError(), "reduce_partials");
+}
+
+}  // namespace jubik

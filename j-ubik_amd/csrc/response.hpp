@@ -1,0 +1,19 @@
+// Host-side launchers of the exposure, read-out and Poisson kernels.
+#pragma once
+#include "common.hpp"
+
+namespace jubik {
+
+constexpr int64_t kPoissonWorkDoubles = 1024;
+
+int exposure_apply(int dtype, const void* expo, int64_t n_e, const void* x, int64_t n_x,
+                   void* out, int64_t n_out, hipStream_t s);
+int readout_gather(int dtype, const void* x, const int64_t* idx, int64_t nnz, void* out,
+                   hipStream_t s);
+int readout_scatter(int dtype, const void* v, const int64_t* idx, int64_t nnz, int64_t n,
+                    void* out, hipStream_t s);
+int poisson_nll_grad(int dtype, const void* lam, const void* d, int64_t n, double* nll,
+                     void* grad, double* work, hipStream_t s);
+int reduce_partials(const double* partial, int64_t n, double* out, hipStream_t s);
+
+}  // namespace jubik

@@ -1,0 +1,110 @@
+"""Fused likelihood of the X-ray forward model: one gradient evaluation per call.
+
+lam_M = Exposure_M(linpatch_convolve(a * s(xi), K_M)) read out where the
+counts are valid, NLL = sum(lam - d log lam) (FP64): the likelihood built by
+jubik/instruments/erosita/erosita_likelihood.py:51-89 and
+jubik/instruments/chandra/chandra_likelihood.py:15-37 composed with the GP
+sky.  ``nll_and_grad`` runs forward, Poisson residual and the hand-written
+adjoint chain in libjubik_hip.so; ``metric`` gives the MGVI metric product
+J^T diag(1/lam) J v at the last gradient position.
+"""
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _device as dev
+from . import _lib
+from .convolve import LinpatchOperator
+from .sky import CorrelatedFieldSky
+
+
+def dense_counts(data, keep):
+    """Counts as an int32 (M, E, H, W) array with -1 where not read out.
+
+    ``data`` holds one 1-D C-order vector per module (the read-out wire
+    format of jubik/response.py:110); ``keep`` is the (M, E, H, W) bool mask.
+    """
+    keep = np.asarray(keep, dtype=bool)
+    out = np.full(keep.shape, -1, dtype=np.int32)
+    for i, v in enumerate(data):
+        v = np.asarray(v)
+        if v.size != int(keep[i].sum()):
+            raise ValueError(f"module {i}: {v.size} counts for {int(keep[i].sum())} pixels")
+        if v.size and (np.any(v < 0) or np.any(v != np.round(v))):
+            raise ValueError("counts must be non-negative integers")
+        out[i][keep[i]] = v.astype(np.int32)
+    return out
+
+
+class XrayModel:
+    """GP sky + per-module linear-patching PSF + exposure + Poisson data."""
+
+    def __init__(self, sky: CorrelatedFieldSky, psf: LinpatchOperator, pixel_area, exposure,
+                 counts):
+        if psf.dtype != sky.dtype:
+            raise ValueError("sky and PSF operators must share a dtype")
+        if not psf.x_shared:
+            raise ValueError("the PSF operator must read one sky for all modules")
+        self.sky, self.psf = sky, psf
+        self.dtype = sky.dtype
+        self.pixel_area = float(pixel_area)
+        J = psf.M * psf.E
+        shape = (psf.M, psf.E, psf.H, psf.W)
+        self.exposure = dev.to_device(exposure, self.dtype).reshape(shape).contiguous()
+        self.counts = torch.as_tensor(np.asarray(counts, dtype=np.int32) if not isinstance(
+            counts, torch.Tensor) else counts.to(torch.int32), device="cuda").reshape(
+                shape).contiguous()
+        self._model = ctypes.c_void_p()
+        _lib.call("jubik_model_create", ctypes.byref(self._model), sky._plan, psf._plan,
+                  self.pixel_area)
+        _lib.call("jubik_model_set_data", self._model, dev.ptr(self.exposure),
+                  dev.ptr(self.counts))
+        self._nll = torch.empty(1, dtype=torch.float64, device="cuda")
+        self.J = J
+
+    def __del__(self):
+        m = getattr(self, "_model", None)
+        if m is not None and m.value:
+            try:
+                _lib.load().jubik_model_destroy(m)
+            except Exception:
+                pass
+            self._model = None
+
+    def _grads(self):
+        N = self.sky.N
+        g_ref = torch.empty((N, N), dtype=self.dtype, device="cuda")
+        g_alpha = torch.empty_like(g_ref) if self.sky.has_alpha else None
+        return g_ref, g_alpha
+
+    def nll_and_grad_device(self, xi_ref, xi_alpha=None, out=None):
+        """Enqueue one gradient evaluation; returns device tensors
+        (nll float64 [1], g_ref, g_alpha) without synchronising."""
+        s = self.sky
+        xr = dev.to_device(xi_ref, self.dtype)
+        xa = dev.to_device(xi_alpha, self.dtype) if s.has_alpha else None
+        g_ref, g_alpha = out if out is not None else self._grads()
+        _lib.call("jubik_model_grad", self._model, dev.ptr(s.amp_ref), dev.ptr(s.amp_alpha),
+                  dev.ptr(xr), dev.ptr(xa), s.zm, s.mu, dev.ptr(s.dnu_dev), dev.ptr(self._nll),
+                  dev.ptr(g_ref), dev.ptr(g_alpha), dev.stream())
+        return self._nll, g_ref, g_alpha
+
+    def nll_and_grad(self, xi_ref, xi_alpha=None):
+        """(NLL as a Python float, grad_xi_ref, grad_xi_alpha)."""
+        nll, g_ref, g_alpha = self.nll_and_grad_device(xi_ref, xi_alpha)
+        val = float(nll.item())
+        return val, dev.like_input(g_ref, xi_ref), (
+            dev.like_input(g_alpha, xi_ref) if g_alpha is not None else None)
+
+    def metric(self, v_ref, v_alpha=None):
+        """J^T diag(1/lam) J v at the position of the last gradient call."""
+        s = self.sky
+        vr = dev.to_device(v_ref, self.dtype)
+        va = dev.to_device(v_alpha, self.dtype) if s.has_alpha else None
+        o_ref, o_alpha = self._grads()
+        _lib.call("jubik_model_metric", self._model, dev.ptr(s.amp_ref), dev.ptr(s.amp_alpha),
+                  dev.ptr(s.dnu_dev), dev.ptr(vr), dev.ptr(va), dev.ptr(o_ref),
+                  dev.ptr(o_alpha), dev.stream())
+        return dev.like_input(o_ref, v_ref), (
+            dev.like_input(o_alpha, v_ref) if o_alpha is not None else None)
