@@ -1,0 +1,319 @@
+"""Benchmark: VI gradient evaluations per second of the X-ray forward model.
+
+Workload (one GPU): sky 1024 x 1024 x 4 energies from the spectral GP
+(xi_ref, xi_alpha on a 1024^2 grid, power-law slope -4, fluctuations 1,
+offset -2), one module, linpatch_convolve with 8 x 8 patches and a 64-px
+margin (FFT length 384), per-patch Gaussian+Moffat PSFs (beta 2.5, core
+sigma 1-4 px), exposure ramp 5e2-2e3 s with 5 % of pixels masked, Poisson
+counts.  One step = one gradient evaluation (GP forward, PSF forward,
+exposure, Poisson NLL in FP64, PSF adjoint, GP adjoint) with inputs resident
+in HBM.  The mixed path computes in FP32 with FP64 reductions; beside it run
+the same fused kernels in FP64, the plain unfused FP64 GPU path (torch +
+hipFFT + autograd, the reference's op order) and the float64 CPU oracle on
+the host cores.  Prints one JSON line.
+"""
+import argparse
+import json
+import os
+import statistics
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "VI grad evals/sec (resp+adjoint+NLL) 1024²×4E; mixed vs FP64 @1e-5 rel err"
+HBM_PEAK = 8.0e12
+FP32_PEAK = 157.3e12
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--H", type=int, default=1024)
+    ap.add_argument("--E", type=int, default=4)
+    ap.add_argument("--n", type=int, default=8)
+    ap.add_argument("--margin", type=int, default=64)
+    ap.add_argument("--cpu-seconds", type=float, default=25.0,
+                    help="budget of the CPU-oracle sample")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--profile-only", action="store_true",
+                    help="only run the mixed gradient steps (for rocprofv3)")
+    return ap.parse_args()
+
+
+def build_inputs(H, E, n, m, seed=2):
+    from tests.problems import powerlaw_amp, psf_windows
+    rng = np.random.default_rng(seed)
+    amp_ref = powerlaw_amp(H)
+    amp_alpha = 0.5 * amp_ref
+    energies = np.array([1.0, 1.6, 2.5, 4.0][:E]) if E <= 4 else np.geomspace(1, 4, E)
+    dnu = np.log(energies / energies[0])
+    w = 2 * (H // n)
+    win = psf_windows(rng, n, E, w, sigma=(1.0, 4.0), moffat=2.5)
+    ramp = np.linspace(5e2, 2e3, H)[None, None, :]
+    expo = np.broadcast_to(ramp, (E, H, H)).copy()
+    expo[rng.random(expo.shape) < 0.05] = 0.0
+    keep = expo > 0
+    xi_true = rng.normal(size=(H, H))
+    xa_true = rng.normal(size=(H, H))
+    xi = xi_true + 0.1 * rng.normal(size=(H, H))
+    xa = xa_true + 0.1 * rng.normal(size=(H, H))
+    return dict(amp_ref=amp_ref, amp_alpha=amp_alpha, dnu=dnu, zm=-2.0, mu=-1.0, win=win,
+                expo=expo, keep=keep, xi_true=(xi_true, xa_true), xi=(xi, xa), rng=rng,
+                H=H, E=E, n=n, m=m, pixel_area=1.0)
+
+
+def make_model(ja, torch, inp, dtype, counts):
+    H, E = inp["H"], inp["E"]
+    sky = ja.CorrelatedFieldSky(H, (H, H), inp["amp_ref"], inp["amp_alpha"], zm=inp["zm"],
+                                mu=inp["mu"], dnu=inp["dnu"], dtype=dtype)
+    psf = ja.LinpatchOperator((E, H, H), inp["n"], inp["m"], dtype=dtype)
+    psf.set_kernel(inp["win"], layout="window")
+    return ja.XrayModel(sky, psf, inp["pixel_area"], inp["expo"][None], counts[None])
+
+
+def draw_counts(ja, torch, inp):
+    """Poisson counts from lam at the true position (FP64 GPU forward)."""
+    H, E = inp["H"], inp["E"]
+    sky = ja.CorrelatedFieldSky(H, (H, H), inp["amp_ref"], inp["amp_alpha"], zm=inp["zm"],
+                                mu=inp["mu"], dnu=inp["dnu"], dtype=torch.float64)
+    s = sky(*[torch.as_tensor(v, device="cuda") for v in inp["xi_true"]])
+    psf = ja.LinpatchOperator((E, H, H), inp["n"], inp["m"], dtype=torch.float64)
+    psf.set_kernel(inp["win"], layout="window")
+    lam = (psf.forward(s, scale=inp["pixel_area"]).cpu().numpy() * inp["expo"])
+    counts = np.where(inp["keep"], inp["rng"].poisson(np.maximum(lam, 0)), -1).astype(np.int32)
+    return counts
+
+
+def time_steps(torch, fn, steps, warmup):
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        fn()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / steps
+
+
+def kernel_bytes(H, E, n, m, M=1):
+    """Bytes each kernel must move per launch given the three-pass design
+    (inputs read once, outputs written once; FP32, complex = 8 B)."""
+    dx = H // n
+    w, L = 2 * dx, 2 * dx + 2 * m
+    L2 = L // 2 + 1
+    J, P = M * E, n * n
+    img = J * H * H * 4
+    c = 8
+    r1_fwd = J * P * w * L2 * c
+    r2_fwd = J * P * L * L2 * c
+    r1_adj = J * P * L * L2 * c
+    r2_adj = J * P * w * L2 * c
+    kt = M * P * E * L2 * L * c
+    gp = H * H * c
+    return {
+        "gp_cols_fwd": 4 * H * H * 4 + gp,
+        "gp_rows_fwd": gp + E * H * H * 4,
+        "lp_rows_fwd": E * H * H * 4 + r1_fwd,
+        "lp_cols_fwd": r1_fwd + kt + r2_fwd,
+        "lp_fwd_out_poisson": r2_fwd + 4 * img,
+        "reduce_partials": J * H * 8,
+        "lp_rows_adj": img + r1_adj,
+        "lp_cols_adj": r1_adj + kt + r2_adj,
+        "lp_adj_out_gp": r2_adj + E * H * H * 4 + gp,
+        "gp_cols_adj": gp + gp,
+        "gp_rows_adj": gp + 4 * H * H * 4,
+    }
+
+
+def eval_bytes(H, E, n, m, M=1):
+    """B_alg of one gradient evaluation: 4 B x [2 |xi| + M E P w^2 + 2 M E H W]."""
+    w = 2 * (H // n)
+    xi = 2 * H * H
+    return 4 * (2 * xi + M * E * n * n * w * w + 2 * M * E * H * H)
+
+
+def fft_flops(H, E, n, m, M=1):
+    """Nominal FLOPs: 2.5 N log2 N per real 2-D transform, 4 per patch-energy,
+    6 L (L/2+1) per spectrum multiply (x2), GP 2 x 2.5 N log2 N per field x 2."""
+    L = 2 * (H // n) + 2 * m
+    pe = M * E * n * n
+    t = 2.5 * L * L * np.log2(L * L)
+    gp = 2 * 2 * 2.5 * H * H * np.log2(H * H)
+    return pe * (4 * t + 2 * 6 * L * (L // 2 + 1)) + gp
+
+
+def cpu_baseline(inp, counts, budget):
+    from oracle import convolve as ocv
+    from oracle import model as om
+    H, E = inp["H"], inp["E"]
+    prob = om.Problem(shape=(E, H, H), distances=(1.0, 1.0, 1.0), n=inp["n"], margin=inp["m"],
+                      pixel_area=inp["pixel_area"], amp_ref=inp["amp_ref"],
+                      amp_alpha=inp["amp_alpha"], zm=inp["zm"], mu_alpha=inp["mu"],
+                      dnu=inp["dnu"])
+    keep = inp["keep"]
+    prob.modules = [om.Module(window=inp["win"], exposure=inp["expo"], keep=keep,
+                              data=counts[keep].astype(np.float64))]
+    times = []
+    t_start = time.perf_counter()
+    result = None
+    while True:
+        t0 = time.perf_counter()
+        result = om.nll_and_grad(prob, *inp["xi"])
+        times.append(time.perf_counter() - t0)
+        if time.perf_counter() - t_start > budget or len(times) >= 6:
+            break
+    timed = times[1:] if len(times) > 1 else times
+    med = statistics.median(timed)
+    return dict(value=1.0 / med, unit="grad evals/s", cores=ocv._WORKERS, kind="port",
+                sample=f"{len(timed)} full gradient evaluations of the same workload "
+                       f"(float64 numpy/scipy oracle, scipy.fft workers={ocv._WORKERS}), "
+                       f"median {med:.2f} s"), result
+
+
+def rel(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def nmax(a, b):
+    return float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
+              "(the hot path is sharded by module/energy/sample across GPUs by design, "
+              "but only the one-GPU figure is measured)", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    import __graft_entry__ as ge
+    ja = ge.load_package()
+    from jubik_amd import _lib
+
+    H, E, n, m = args.H, args.E, args.n, args.margin
+    inp = build_inputs(H, E, n, m)
+    counts = draw_counts(ja, torch, inp)
+    mixed = make_model(ja, torch, inp, torch.float32, counts)
+    xi32 = [torch.as_tensor(v, dtype=torch.float32, device="cuda") for v in inp["xi"]]
+    out32 = mixed._grads()
+    step = lambda: mixed.nll_and_grad_device(*xi32, out=out32)
+    if args.profile_only:
+        t = time_steps(torch, step, args.steps, args.warmup)
+        print(json.dumps({"profile_only": True, "ms_per_step": t * 1e3}))
+        return
+
+    t_mixed = time_steps(torch, step, args.steps, args.warmup)
+    nll32, g32r, g32a = mixed.nll_and_grad_device(*xi32)
+    torch.cuda.synchronize()
+    g32 = np.concatenate([g32r.double().cpu().numpy().ravel(), g32a.double().cpu().numpy().ravel()])
+    nll32 = float(nll32.item())
+
+    # per-kernel HIP-event timing on the launch stream (outside the timed region)
+    _lib.timing_enable(True)
+    nprof = max(10, min(args.steps, 30))
+    for _ in range(nprof):
+        step()
+    torch.cuda.synchronize()
+    rep = _lib.timing_report()
+    _lib.timing_enable(False)
+    kb = kernel_bytes(H, E, n, m)
+    kernels = {k: {"launches": c, "avg_us": 1e3 * tot / c} for k, (c, tot, lo, hi) in rep.items()}
+    dom = max(rep.items(), key=lambda kv: kv[1][1])[0]
+    dom_avg = rep[dom][1] / rep[dom][0] * 1e-3
+    achieved = kb.get(dom, 0) / dom_avg
+
+    # fused kernels in FP64
+    f64 = make_model(ja, torch, inp, torch.float64, counts)
+    xi64 = [torch.as_tensor(v, dtype=torch.float64, device="cuda") for v in inp["xi"]]
+    o64 = f64._grads()
+    t_f64 = time_steps(torch, lambda: f64.nll_and_grad_device(*xi64, out=o64),
+                       max(5, args.steps // 5), 2)
+    nll64, g64r, g64a = f64.nll_and_grad_device(*xi64)
+    torch.cuda.synchronize()
+    g64 = np.concatenate([g64r.cpu().numpy().ravel(), g64a.cpu().numpy().ravel()])
+    nll64 = float(nll64.item())
+    del f64
+
+    plain = None
+    if not args.no_plain:
+        from jubik_amd.plain_fp64 import PlainFP64Model
+        w = 2 * (H // n)
+        kfull = torch.zeros((n * n, E, H, H), dtype=torch.float64, device="cuda")
+        c = (H - w) // 2
+        kfull[..., c:c + w, c:c + w] = torch.as_tensor(inp["win"], device="cuda")
+        kfull = torch.fft.ifftshift(kfull, dim=(-2, -1)).contiguous()
+        res = {}
+        for prep in (True, False):
+            pm = PlainFP64Model(inp["amp_ref"], inp["amp_alpha"], inp["zm"], inp["mu"],
+                                inp["dnu"], kfull, n, m, inp["pixel_area"], inp["expo"],
+                                inp["keep"], counts[inp["keep"]].astype(np.float64),
+                                prepare_each_call=prep)
+            t = time_steps(torch, lambda: pm.nll_and_grad(*xi64), 3, 1)
+            val, gr, ga = pm.nll_and_grad(*xi64)
+            gp_ = np.concatenate([gr.cpu().numpy().ravel(), ga.cpu().numpy().ravel()])
+            res["per_call_kernel_prep" if prep else "prepared_kernel"] = dict(
+                ms_per_step=t * 1e3, evals_per_s=1.0 / t,
+                grad_rel_err_vs_fused_fp64=rel(gp_, g64))
+            del pm
+        del kfull
+        torch.cuda.empty_cache()
+        plain = res
+
+    cpu = None
+    grad_vs_oracle = None
+    if not args.no_cpu:
+        cpu, (nll_o, (gor, goa)) = cpu_baseline(inp, counts, args.cpu_seconds)
+        go = np.concatenate([gor.ravel(), goa.ravel()])
+        grad_vs_oracle = {"mixed_rel": rel(g32, go), "mixed_nmax": nmax(g32, go),
+                          "fused_fp64_rel": rel(g64, go), "nll_mixed_rel": abs(nll32 - nll_o) /
+                          abs(nll_o), "nll_fp64_rel": abs(nll64 - nll_o) / abs(nll_o)}
+
+    value = 1.0 / t_mixed
+    best_plain = max(plain.values(), key=lambda r: r["evals_per_s"]) if plain else None
+    line = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "grad evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": t_mixed * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"W*: {H}x{H}x{E}E spectral GP sky, 1 module, linpatch n={n} "
+                               f"margin={m} (L={2 * (H // n) + 2 * m}), Gaussian+Moffat PSFs, "
+                               "exposure 5e2-2e3 s, 5% mask, Poisson NLL (FP64 reduction)",
+                   "precision": "FP32 FFTs/PSF products/exp, FP64 NLL reductions"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK, "traffic": None, "kernel": dom,
+                     "kernel_avg_us": dom_avg * 1e6, "bytes_per_launch": kb.get(dom, 0)},
+        "eval_roofline": {"B_alg": eval_bytes(H, E, n, m),
+                          "achieved": eval_bytes(H, E, n, m) / t_mixed,
+                          "frac_hbm": eval_bytes(H, E, n, m) / t_mixed / HBM_PEAK,
+                          "fft_flops": fft_flops(H, E, n, m),
+                          "frac_fp32": fft_flops(H, E, n, m) / t_mixed / FP32_PEAK},
+        "cpu_baseline": cpu,
+        "fused_fp64": {"ms_per_step": t_f64 * 1e3, "evals_per_s": 1.0 / t_f64,
+                       "grad_rel_err_mixed_vs_fp64": rel(g32, g64),
+                       "grad_nmax_err_mixed_vs_fp64": nmax(g32, g64)},
+        "plain_fp64": plain,
+        "speedup_vs_plain_fp64": (value / best_plain["evals_per_s"]) if best_plain else None,
+        "grad_err_vs_cpu_oracle": grad_vs_oracle,
+        "kernels": kernels,
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

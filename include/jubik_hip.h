@@ -45,6 +45,13 @@ extern "C" {
 const char* jubik_last_error(void);
 int jubik_abi_version(void);
 
+/* Kernel timing (no counterpart in the reference, which has no profiling):
+ * while enabled, every kernel launch is bracketed by HIP events on its
+ * stream; jubik_timing_report synchronises them and writes one line per
+ * kernel, "name count total_ms min_ms max_ms", then forgets them. */
+int jubik_timing_enable(int on);
+int jubik_timing_report(char* buf, int64_t len);
+
 /* ---------------------------------------------------------------- linpatch
  * Replaces jubik/convolve.py:66-145 linpatch_convolve(x, domain, kernel,
  * n_patches_per_axis, margin), including the module vmap of its callers

@@ -21,6 +21,8 @@ _dblp = C.POINTER(C.c_double)
 # name -> (argtypes); every function returns int except jubik_last_error.
 SIGNATURES = {
     "jubik_abi_version": [],
+    "jubik_timing_enable": [_int],
+    "jubik_timing_report": [C.c_char_p, _i64],
     "jubik_linpatch_create": [C.POINTER(_vp), _int, _i64, _i64, _i64, _i64, _i64, _i64, _int,
                               _int],
     "jubik_linpatch_destroy": [_vp],
@@ -82,3 +84,18 @@ def check(status):
 
 def call(name, *args):
     check(getattr(load(), name)(*args))
+
+
+def timing_enable(on=True):
+    call("jubik_timing_enable", int(bool(on)))
+
+
+def timing_report():
+    """{kernel: (count, total_ms, min_ms, max_ms)} since the last report."""
+    buf = C.create_string_buffer(1 << 16)
+    call("jubik_timing_report", buf, len(buf))
+    out = {}
+    for line in buf.value.decode().splitlines():
+        name, cnt, tot, lo, hi = line.split()
+        out[name] = (int(cnt), float(tot), float(lo), float(hi))
+    return out

@@ -1,7 +1,10 @@
 // C ABI (include/jubik_hip.h): plans, argument validation, error text and
 // the fused gradient evaluation / metric product built from the kernels.
+#include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -23,6 +26,22 @@ int check_hip(hipError_t e, const char* what) {
   if (e == hipSuccess) return kOk;
   g_error = std::string(what) + ": " + hipGetErrorString(e);
   return e == hipErrorOutOfMemory ? kOutOfMemory : kRuntimeError;
+}
+
+namespace {
+struct TimingRec {
+  const char* name;
+  hipEvent_t a, b;
+};
+std::atomic<bool> g_timing{false};
+std::mutex g_timing_mu;
+std::vector<TimingRec> g_timing_recs;
+}  // namespace
+
+bool timing_enabled() { return g_timing.load(std::memory_order_relaxed); }
+void timing_record(const char* name, hipEvent_t a, hipEvent_t b) {
+  std::lock_guard<std::mutex> lk(g_timing_mu);
+  g_timing_recs.push_back({name, a, b});
 }
 
 namespace {
@@ -177,6 +196,57 @@ extern "C" {
 
 const char* jubik_last_error(void) { return g_error.c_str(); }
 int jubik_abi_version(void) { return 1; }
+
+int jubik_timing_enable(int on) {
+  g_timing.store(on != 0);
+  return kOk;
+}
+
+int jubik_timing_report(char* buf, int64_t len) {
+  if (!buf || len <= 0) return fail(kInvalidArgument, "NULL or empty buffer");
+  std::vector<TimingRec> recs;
+  {
+    std::lock_guard<std::mutex> lk(g_timing_mu);
+    recs.swap(g_timing_recs);
+  }
+  struct Agg {
+    std::string name;
+    int64_t count = 0;
+    double total = 0, lo = 1e300, hi = 0;
+  };
+  std::vector<Agg> aggs;
+  int st = kOk;
+  for (auto& r : recs) {
+    float ms = 0.f;
+    if (st == kOk) {
+      hipError_t e = hipEventSynchronize(r.b);
+      if (e == hipSuccess) e = hipEventElapsedTime(&ms, r.a, r.b);
+      if (e != hipSuccess) st = check_hip(e, "timing report");
+    }
+    (void)hipEventDestroy(r.a);
+    (void)hipEventDestroy(r.b);
+    if (st != kOk) continue;
+    Agg* a = nullptr;
+    for (auto& x : aggs)
+      if (x.name == r.name) a = &x;
+    if (!a) {
+      aggs.push_back(Agg{r.name});
+      a = &aggs.back();
+    }
+    a->count += 1;
+    a->total += ms;
+    a->lo = std::min(a->lo, (double)ms);
+    a->hi = std::max(a->hi, (double)ms);
+  }
+  if (st != kOk) return st;
+  std::string out;
+  for (auto& a : aggs)
+    out += a.name + " " + std::to_string(a.count) + " " + std::to_string(a.total) + " " +
+           std::to_string(a.lo) + " " + std::to_string(a.hi) + "\n";
+  if ((int64_t)out.size() + 1 > len) return fail(kInvalidArgument, "report buffer too small");
+  std::memcpy(buf, out.c_str(), out.size() + 1);
+  return kOk;
+}
 
 // ------------------------------------------------------------------ linpatch
 int jubik_linpatch_create(jubik_linpatch** plan, int dtype, int64_t H, int64_t W, int64_t E,

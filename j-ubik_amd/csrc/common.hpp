@@ -23,6 +23,28 @@ void set_error(const std::string& msg);
 int fail(int code, const std::string& msg);
 int check_hip(hipError_t e, const char* what);
 
+// Optional per-launch timing (jubik_timing_enable): HIP events recorded on
+// the launch stream around each kernel, aggregated by name on report.
+bool timing_enabled();
+void timing_record(const char* name, hipEvent_t start, hipEvent_t stop);
+struct TimedLaunch {
+  const char* name;
+  hipStream_t stream;
+  hipEvent_t a = nullptr, b = nullptr;
+  TimedLaunch(const char* n, hipStream_t s) : name(n), stream(s) {
+    if (timing_enabled() && hipEventCreate(&a) == hipSuccess && hipEventCreate(&b) == hipSuccess)
+      (void)hipEventRecord(a, s);
+    else
+      a = b = nullptr;
+  }
+  ~TimedLaunch() {
+    if (a && b) {
+      (void)hipEventRecord(b, stream);
+      timing_record(name, a, b);
+    }
+  }
+};
+
 #define JUBIK_HIP(call)                                  \
   do {                                                   \
     hipError_t _e = (call);                              \

@@ -148,40 +148,61 @@ struct GpOps {
   static int forward(const GpGeom& g, const GpBuffers& b, const void* amp_ref,
                      const void* amp_alpha, const void* xi_ref, const void* xi_alpha,
                      const void* dnu, void* sky, hipStream_t s) {
-    gp_cols<T, N, 0><<<N / V, GNT, 0, s>>>(g, (const T*)amp_ref, (const T*)amp_alpha,
-                                           (const T*)xi_ref, (const T*)xi_alpha, nullptr,
-                                           (const C*)b.tw, (C*)b.work);
-    gp_rows<T, N, 0><<<N / 2 + 1, GNT, 0, s>>>(g, (const C*)b.work, (const C*)b.tw,
-                                               (const T*)dnu, nullptr, nullptr, nullptr,
-                                               (T*)sky, nullptr);
+    {
+      TimedLaunch _t("gp_cols_fwd", s);
+      gp_cols<T, N, 0><<<N / V, GNT, 0, s>>>(g, (const T*)amp_ref, (const T*)amp_alpha,
+                                             (const T*)xi_ref, (const T*)xi_alpha, nullptr,
+                                             (const C*)b.tw, (C*)b.work);
+    }
+    {
+      TimedLaunch _t("gp_rows_fwd", s);
+      gp_rows<T, N, 0><<<N / 2 + 1, GNT, 0, s>>>(g, (const C*)b.work, (const C*)b.tw,
+                                                 (const T*)dnu, nullptr, nullptr, nullptr,
+                                                 (T*)sky, nullptr);
+    }
     return check_hip(hipGetLastError(), "gp_forward");
   }
   static int tangent(const GpGeom& g, const GpBuffers& b, const void* amp_ref,
                      const void* amp_alpha, const void* v_ref, const void* v_alpha,
                      const void* dnu, const void* sky, void* dsky, hipStream_t s) {
-    gp_cols<T, N, 0><<<N / V, GNT, 0, s>>>(g, (const T*)amp_ref, (const T*)amp_alpha,
-                                           (const T*)v_ref, (const T*)v_alpha, nullptr,
-                                           (const C*)b.tw, (C*)b.work);
-    gp_rows<T, N, 2><<<N / 2 + 1, GNT, 0, s>>>(g, (const C*)b.work, (const C*)b.tw,
-                                               (const T*)dnu, nullptr, nullptr, (const T*)sky,
-                                               (T*)dsky, nullptr);
+    {
+      TimedLaunch _t("gp_cols_fwd", s);
+      gp_cols<T, N, 0><<<N / V, GNT, 0, s>>>(g, (const T*)amp_ref, (const T*)amp_alpha,
+                                             (const T*)v_ref, (const T*)v_alpha, nullptr,
+                                             (const C*)b.tw, (C*)b.work);
+    }
+    {
+      TimedLaunch _t("gp_rows_tangent", s);
+      gp_rows<T, N, 2><<<N / 2 + 1, GNT, 0, s>>>(g, (const C*)b.work, (const C*)b.tw,
+                                                 (const T*)dnu, nullptr, nullptr, (const T*)sky,
+                                                 (T*)dsky, nullptr);
+    }
     return check_hip(hipGetLastError(), "gp_tangent");
   }
   static int adjoint_u(const GpGeom& g, const GpBuffers& b, const void* amp_ref,
                        const void* amp_alpha, const void* u, void* g_ref, void* g_alpha,
                        hipStream_t s) {
-    gp_cols<T, N, 1><<<N / V, GNT, 0, s>>>(g, nullptr, nullptr, nullptr, nullptr, (const C*)u,
-                                           (const C*)b.tw, (C*)b.work);
-    gp_rows<T, N, 1><<<N / 2 + 1, GNT, 0, s>>>(g, (const C*)b.work, (const C*)b.tw, nullptr,
-                                               (const T*)amp_ref, (const T*)amp_alpha, nullptr,
-                                               (T*)g_ref, (T*)g_alpha);
+    {
+      TimedLaunch _t("gp_cols_adj", s);
+      gp_cols<T, N, 1><<<N / V, GNT, 0, s>>>(g, nullptr, nullptr, nullptr, nullptr, (const C*)u,
+                                             (const C*)b.tw, (C*)b.work);
+    }
+    {
+      TimedLaunch _t("gp_rows_adj", s);
+      gp_rows<T, N, 1><<<N / 2 + 1, GNT, 0, s>>>(g, (const C*)b.work, (const C*)b.tw, nullptr,
+                                                 (const T*)amp_ref, (const T*)amp_alpha, nullptr,
+                                                 (T*)g_ref, (T*)g_alpha);
+    }
     return check_hip(hipGetLastError(), "gp_adjoint");
   }
   static int fold(const GpGeom& g, const void* sky, const void* cot, const void* dnu, void* u,
                   hipStream_t s) {
     const int64_t n = (int64_t)g.H * g.W;
     const int blocks = (int)std::min<int64_t>((n + GNT - 1) / GNT, 4096);
-    gp_fold<T><<<blocks, GNT, 0, s>>>(g, (const T*)sky, (const T*)cot, (const T*)dnu, (C*)u);
+    {
+      TimedLaunch _t("gp_fold", s);
+      gp_fold<T><<<blocks, GNT, 0, s>>>(g, (const T*)sky, (const T*)cot, (const T*)dnu, (C*)u);
+    }
     return check_hip(hipGetLastError(), "gp_fold");
   }
 };

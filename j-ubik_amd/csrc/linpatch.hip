@@ -507,41 +507,65 @@ struct LinOps {
                      hipStream_t s) {
     dim3 gr((g.w / 2 + V - 1) / V, g.P, g.M * g.Ek);
     if (layout == 0)
-      lp_rows<T, L, ROW_KERNEL_FULL><<<gr, NT, 0, s>>>(g, (const T*)kernel, (const T*)b.tent,
-                                                       (const C*)b.tw, (C*)b.R1, T(1), g.w);
+      {
+        TimedLaunch _t("lp_rows_kernel", s);
+        lp_rows<T, L, ROW_KERNEL_FULL><<<gr, NT, 0, s>>>(g, (const T*)kernel, (const T*)b.tent,
+                                                         (const C*)b.tw, (C*)b.R1, T(1), g.w);
+      }
     else
-      lp_rows<T, L, ROW_KERNEL_WIN><<<gr, NT, 0, s>>>(g, (const T*)kernel, (const T*)b.tent,
-                                                      (const C*)b.tw, (C*)b.R1, T(1), g.w);
+      {
+        TimedLaunch _t("lp_rows_kernel", s);
+        lp_rows<T, L, ROW_KERNEL_WIN><<<gr, NT, 0, s>>>(g, (const T*)kernel, (const T*)b.tent,
+                                                        (const C*)b.tw, (C*)b.R1, T(1), g.w);
+      }
     dim3 gc((L2 + V - 1) / V, g.P, g.M * g.Ek);
-    lp_cols<T, L, COL_KERNEL><<<gc, NT, 0, s>>>(g, (const C*)b.R1, nullptr, (const C*)b.tw,
-                                                (C*)b.KT);
+    {
+      TimedLaunch _t("lp_cols_kernel", s);
+      lp_cols<T, L, COL_KERNEL><<<gc, NT, 0, s>>>(g, (const C*)b.R1, nullptr, (const C*)b.tw,
+                                                  (C*)b.KT);
+    }
     return launch_status("linpatch_prepare");
   }
 
   static void fwd_front(const LinGeom& g, const LinBuffers& b, const void* x, double scale,
                         hipStream_t s) {
     dim3 gr((g.w / 2 + V - 1) / V, g.P, g.J);
-    lp_rows<T, L, ROW_SIGNAL><<<gr, NT, 0, s>>>(g, (const T*)x, (const T*)b.tent, (const C*)b.tw,
-                                                (C*)b.R1, T(scale), g.w);
+    {
+      TimedLaunch _t("lp_rows_fwd", s);
+      lp_rows<T, L, ROW_SIGNAL><<<gr, NT, 0, s>>>(g, (const T*)x, (const T*)b.tent, (const C*)b.tw,
+                                                  (C*)b.R1, T(scale), g.w);
+    }
     dim3 gc((L2 + V - 1) / V, g.P, g.J);
-    lp_cols<T, L, COL_FWD><<<gc, NT, 0, s>>>(g, (const C*)b.R1, (const C*)b.KT, (const C*)b.tw,
-                                             (C*)b.R2);
+    {
+      TimedLaunch _t("lp_cols_fwd", s);
+      lp_cols<T, L, COL_FWD><<<gc, NT, 0, s>>>(g, (const C*)b.R1, (const C*)b.KT, (const C*)b.tw,
+                                               (C*)b.R2);
+    }
   }
 
   static void adj_front(const LinGeom& g, const LinBuffers& b, const void* gy, hipStream_t s) {
     dim3 gr((L / 2 + V - 1) / V, g.P, g.J);
-    lp_rows<T, L, ROW_COTANGENT><<<gr, NT, 0, s>>>(g, (const T*)gy, (const T*)b.tent,
-                                                   (const C*)b.tw, (C*)b.R1, T(1), L);
+    {
+      TimedLaunch _t("lp_rows_adj", s);
+      lp_rows<T, L, ROW_COTANGENT><<<gr, NT, 0, s>>>(g, (const T*)gy, (const T*)b.tent,
+                                                     (const C*)b.tw, (C*)b.R1, T(1), L);
+    }
     dim3 gc((L2 + V - 1) / V, g.P, g.J);
-    lp_cols<T, L, COL_ADJ><<<gc, NT, 0, s>>>(g, (const C*)b.R1, (const C*)b.KT, (const C*)b.tw,
-                                             (C*)b.R2);
+    {
+      TimedLaunch _t("lp_cols_adj", s);
+      lp_cols<T, L, COL_ADJ><<<gc, NT, 0, s>>>(g, (const C*)b.R1, (const C*)b.KT, (const C*)b.tw,
+                                               (C*)b.R2);
+    }
   }
 
   static int forward(const LinGeom& g, const LinBuffers& b, const void* x, void* y, double scale,
                      hipStream_t s) {
     fwd_front(g, b, x, scale, s);
-    lp_fwd_out<T, L, 0><<<dim3(g.H, g.J), NT, 0, s>>>(g, (const C*)b.R2, (const C*)b.tw, (T*)y,
-                                                      nullptr, nullptr, nullptr, nullptr);
+    {
+      TimedLaunch _t("lp_fwd_out", s);
+      lp_fwd_out<T, L, 0><<<dim3(g.H, g.J), NT, 0, s>>>(g, (const C*)b.R2, (const C*)b.tw, (T*)y,
+                                                        nullptr, nullptr, nullptr, nullptr);
+    }
     return launch_status("linpatch_forward");
   }
 
@@ -549,8 +573,11 @@ struct LinOps {
                              const void* expo, const int* counts, void* resid, void* lam,
                              double* partial, hipStream_t s) {
     fwd_front(g, b, x, scale, s);
-    lp_fwd_out<T, L, 1><<<dim3(g.H, g.J), NT, 0, s>>>(g, (const C*)b.R2, (const C*)b.tw, (T*)lam,
-                                                      (const T*)expo, counts, (T*)resid, partial);
+    {
+      TimedLaunch _t("lp_fwd_out_poisson", s);
+      lp_fwd_out<T, L, 1><<<dim3(g.H, g.J), NT, 0, s>>>(g, (const C*)b.R2, (const C*)b.tw, (T*)lam,
+                                                        (const T*)expo, counts, (T*)resid, partial);
+    }
     return launch_status("linpatch_forward_poisson");
   }
 
@@ -558,9 +585,12 @@ struct LinOps {
                             const void* expo, const int* counts, const void* lam, void* cot,
                             hipStream_t s) {
     fwd_front(g, b, dx, scale, s);
-    lp_fwd_out<T, L, 2><<<dim3(g.H, g.J), NT, 0, s>>>(g, (const C*)b.R2, (const C*)b.tw,
-                                                      (T*)lam, (const T*)expo, counts, (T*)cot,
-                                                      nullptr);
+    {
+      TimedLaunch _t("lp_fwd_out_metric", s);
+      lp_fwd_out<T, L, 2><<<dim3(g.H, g.J), NT, 0, s>>>(g, (const C*)b.R2, (const C*)b.tw,
+                                                        (T*)lam, (const T*)expo, counts, (T*)cot,
+                                                        nullptr);
+    }
     return launch_status("linpatch_forward_metric");
   }
 
@@ -568,18 +598,24 @@ struct LinOps {
                      double scale, hipStream_t s) {
     adj_front(g, b, gy, s);
     const int nimg = g.x_shared ? g.E : g.J;
-    lp_adj_out<T, L, 0><<<dim3(g.H, nimg), NT, 0, s>>>(g, (const C*)b.R2, (const C*)b.tw,
-                                                       (const T*)b.tent, (T*)gx, T(scale),
-                                                       nullptr, nullptr, nullptr);
+    {
+      TimedLaunch _t("lp_adj_out", s);
+      lp_adj_out<T, L, 0><<<dim3(g.H, nimg), NT, 0, s>>>(g, (const C*)b.R2, (const C*)b.tw,
+                                                         (const T*)b.tent, (T*)gx, T(scale),
+                                                         nullptr, nullptr, nullptr);
+    }
     return launch_status("linpatch_adjoint");
   }
 
   static int adjoint_gp(const LinGeom& g, const LinBuffers& b, const void* gy, double scale,
                         const void* sky, const void* dnu, void* u, hipStream_t s) {
     adj_front(g, b, gy, s);
-    lp_adj_out<T, L, 1><<<dim3(g.H, 1), NT, 0, s>>>(g, (const C*)b.R2, (const C*)b.tw,
-                                                    (const T*)b.tent, nullptr, T(scale),
-                                                    (const T*)sky, (const T*)dnu, (C*)u);
+    {
+      TimedLaunch _t("lp_adj_out_gp", s);
+      lp_adj_out<T, L, 1><<<dim3(g.H, 1), NT, 0, s>>>(g, (const C*)b.R2, (const C*)b.tw,
+                                                      (const T*)b.tent, nullptr, T(scale),
+                                                      (const T*)sky, (const T*)dnu, (C*)u);
+    }
     return launch_status("linpatch_adjoint_gp");
   }
 };

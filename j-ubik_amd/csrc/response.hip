@@ -131,6 +131,7 @@ int poisson_nll_grad(int dtype, const void* lam, const void* d, int64_t n, doubl
 }
 
 int reduce_partials(const double* partial, int64_t n, double* out, hipStream_t s) {
+  TimedLaunch t("reduce_partials", s);
   k_reduce<<<1, RNT, 0, s>>>(partial, n, out);
   return check_hip(hipGetLastError(), "reduce_partials");
 }
