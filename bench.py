@@ -45,6 +45,9 @@ def parse():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--profile-only", action="store_true",
                     help="only run the mixed gradient steps (for rocprofv3)")
+    ap.add_argument("--calib-copy-mib", type=int, default=0,
+                    help="with --profile-only: first copy a buffer of this many MiB "
+                         "(a known-size reference for the HBM byte counters)")
     return ap.parse_args()
 
 
@@ -206,6 +209,12 @@ def main():
     out32 = mixed._grads()
     step = lambda: mixed.nll_and_grad_device(*xi32, out=out32)
     if args.profile_only:
+        if args.calib_copy_mib:
+            src = torch.ones(args.calib_copy_mib << 18, dtype=torch.float32, device="cuda")
+            dst = torch.empty_like(src)
+            dst.copy_(src)
+            torch.cuda.synchronize()
+            del src, dst
         t = time_steps(torch, step, args.steps, args.warmup)
         print(json.dumps({"profile_only": True, "ms_per_step": t * 1e3}))
         return

@@ -423,7 +423,7 @@ int jubik_gp_create(jubik_gp** plan, int dtype, int64_t N, int64_t H, int64_t W,
     return fail(kInvalidArgument, "several energies need the spectral-index field");
   auto* p = new jubik_gp();
   p->dtype = dtype;
-  p->g = GpGeom{(int)N, (int)H, (int)W, (int)E, has_alpha ? 1 : 0, 0.0, 0.0};
+  p->g = GpGeom{(int)N, (int)H, (int)W, (int)E, has_alpha ? 1 : 0, 0.0, 0.0, 1};
   if (hipGetDevice(&p->device) != hipSuccess) {
     delete p;
     return fail(kRuntimeError, "no HIP device available");
@@ -498,7 +498,7 @@ int jubik_model_create(jubik_model** model, jubik_gp* gp, jubik_linpatch* lp, do
   if (!st) st = dev_alloc(&m->resid, img * g.J * es, "residual");
   if (!st) st = dev_alloc(&m->lam, img * g.J * es, "lambda");
   if (!st) st = dev_alloc(&m->partial, (size_t)g.J * g.H * sizeof(double), "NLL partials");
-  if (!st) st = dev_alloc(&m->u, img * 2 * es, "GP cotangent");
+  if (!st) st = dev_alloc(&m->u, img * g.E * 2 * es, "GP cotangent planes");
   if (st) {
     m->release();
     delete m;
@@ -547,8 +547,9 @@ int jubik_model_grad(jubik_model* model, const void* amp_ref, const void* amp_al
   st = linpatch_adjoint_gp(lp->dtype, g, lp->bufs(), model->resid, model->pixel_area, model->sky,
                            dnu, model->u, s);
   if (st) return st;
-  st = gp_adjoint_u(gp->dtype, gp->g, gp->bufs(), amp_ref, amp_alpha, model->u, g_ref, g_alpha,
-                    s);
+  GpGeom gg = gp->g;
+  gg.uplanes = g.E;
+  st = gp_adjoint_u(gp->dtype, gg, gp->bufs(), amp_ref, amp_alpha, model->u, g_ref, g_alpha, s);
   if (st == kOk) model->have_point = true;
   return st;
 }
@@ -573,7 +574,9 @@ int jubik_model_metric(jubik_model* model, const void* amp_ref, const void* amp_
   st = linpatch_adjoint_gp(lp->dtype, g, lp->bufs(), model->resid, model->pixel_area, model->sky,
                            dnu, model->u, s);
   if (st) return st;
-  return gp_adjoint_u(gp->dtype, gp->g, gp->bufs(), amp_ref, amp_alpha, model->u, out_ref,
+  GpGeom gg = gp->g;
+  gg.uplanes = g.E;
+  return gp_adjoint_u(gp->dtype, gg, gp->bufs(), amp_ref, amp_alpha, model->u, out_ref,
                       out_alpha, s);
 }
 

@@ -83,13 +83,20 @@ __device__ __forceinline__ int64_t spec_index(const LinGeom& g, int job, int p) 
   return ((int64_t)mod * g.P + p) * g.Ek + (g.Ek == 1 ? 0 : e);
 }
 
-// Tile sizes: vectors per work-group so a tile of V x L complex values
-// occupies at most ~48 KiB of LDS.
+// Tile sizes: vectors per work-group so a tile of V vectors of length L at
+// the odd LDS stride occupies at most 52 KiB (16 float vectors at L = 384).
 template <typename T, int L>
 struct Tile {
-  static constexpr int bytes = 49152;
-  static constexpr int raw = bytes / (L * (int)sizeof(cplx<T>));
+  static constexpr int bytes = 53248;
+  static constexpr int raw = bytes / ((L | 1) * (int)sizeof(cplx<T>));
   static constexpr int V = raw >= 32 ? 32 : raw >= 16 ? 16 : raw >= 8 ? 8 : raw >= 4 ? 4 : raw >= 2 ? 2 : 1;
+};
+
+// Row passes use half the column tile: their global traffic is contiguous
+// rows whatever V is, and the smaller LDS footprint doubles residency.
+template <typename T, int L>
+struct RowTile {
+  static constexpr int V = Tile<T, L>::V >= 2 ? Tile<T, L>::V / 2 : 1;
 };
 
 // Block-wide sum of one double per thread (NT a multiple of 64).

@@ -1,16 +1,22 @@
 // Batched LDS-resident mixed-radix FFT building blocks for gfx950.
 //
-// A work-group of NT threads transforms V complex vectors of length L that
-// sit in LDS (vector v at buf + v * VS).  Each radix stage is a Stockham
-// step: every thread loads R points (stride L/R) of a butterfly into
-// registers, applies the twiddles W_{Ns R}^{t k}, runs the R-point DFT and,
-// after a barrier, stores the results at their auto-sorted positions, so
-// the output is in natural order and no bit reversal pass is needed.  L is
-// a compile-time constant; its radix sequence is chosen greedily
-// (8, then 4, 2, 3, 5, 7), so every 7-smooth length is supported.
+// A transform of length L is a short sequence of Stockham stages with large
+// radices (16, 24, 32, ...), so L = 384 takes two stages.  In a stage every
+// task (vector v, butterfly j) holds R points x[j + t L/R] in registers,
+// multiplies them by W_{Ns R}^{t k} (k = j mod Ns), runs an R-point DFT in
+// registers (itself a compile-time Cooley-Tukey split with twiddles folded
+// into instruction literals) and stores the results at their auto-sorted
+// positions (j - k) R + k + t Ns: the output is in natural order.
 //
-// The twiddle table ``tw`` (LDS) holds exp(-2 pi i e / L), e = 0..L-1.
-// Inverse transforms are unnormalised.
+// Tasks are mapped vector-fastest (vec = idx mod V), vectors sit in LDS at an
+// odd stride VS: lanes of a wave address different vectors at fixed position
+// offsets, which keeps every stage free of LDS bank conflicts.  Stages expose
+// gather / compute / scatter separately so kernels can read the first stage
+// straight from global memory, keep the last stage in registers (spectrum
+// multiply, inverse first stage) and write results straight back.
+//
+// The LDS twiddle table ``tw`` holds exp(-2 pi i e / L), e < L.  Inverse
+// transforms conjugate every twiddle and are unnormalised.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -47,32 +53,75 @@ __device__ __forceinline__ cplx<T> mul_mi(cplx<T> a) {
   return INV ? cplx<T>{-a.im, a.re} : cplx<T>{a.im, -a.re};
 }
 
-// ---------------------------------------------------------------- radix plan
-__host__ __device__ constexpr int radix_at(int L, int i) {
-  int r = 1;
-  for (int s = 0; s <= i; ++s) {
-    if (L % 8 == 0) r = 8;
-    else if (L % 4 == 0) r = 4;
-    else if (L % 2 == 0) r = 2;
-    else if (L % 3 == 0) r = 3;
-    else if (L % 5 == 0) r = 5;
-    else if (L % 7 == 0) r = 7;
-    else return (L == 1) ? 1 : -1;
-    L /= r;
-  }
-  return r;
-}
-__host__ __device__ constexpr int num_stages(int L) {
-  int s = 0;
-  while (L > 1) {
-    int r = radix_at(L, 0);
-    if (r < 0) return -1;
-    L /= r;
-    ++s;
+// ------------------------------------------------- compile-time unit roots
+// exp(-2 pi i e / N) from Taylor polynomials on [0, pi/4] after an exact
+// octant reduction of e / N; evaluated by the compiler, so every constant
+// twiddle of a register DFT becomes an instruction literal.
+struct dpair {
+  double re, im;
+};
+constexpr double kQuarterPi = 0.78539816339744830961566084581987572;
+constexpr double poly_sin(double x) {
+  double x2 = x * x, t = x, s = x;
+  for (int i = 1; i < 13; ++i) {
+    t *= -x2 / ((2.0 * i) * (2.0 * i + 1.0));
+    s += t;
   }
   return s;
 }
-__host__ __device__ constexpr bool fft_length_ok(int L) { return L >= 2 && num_stages(L) > 0; }
+constexpr double poly_cos(double x) {
+  double x2 = x * x, t = 1.0, s = 1.0;
+  for (int i = 1; i < 13; ++i) {
+    t *= -x2 / ((2.0 * i - 1.0) * (2.0 * i));
+    s += t;
+  }
+  return s;
+}
+constexpr dpair unit_root(long e, long n) {
+  const long r = ((e % n) + n) % n;
+  const long o = (8 * r) / n;
+  const long rr = 8 * r - o * n;
+  const double phi = kQuarterPi * (double)rr / (double)n;
+  const double psi = kQuarterPi * (double)(n - rr) / (double)n;
+  double c = 0, s = 0;  // cos and sin of 2 pi r / n
+  switch (o) {
+    case 0: c = poly_cos(phi); s = poly_sin(phi); break;
+    case 1: c = poly_sin(psi); s = poly_cos(psi); break;
+    case 2: c = -poly_sin(phi); s = poly_cos(phi); break;
+    case 3: c = -poly_cos(psi); s = poly_sin(psi); break;
+    case 4: c = -poly_cos(phi); s = -poly_sin(phi); break;
+    case 5: c = -poly_sin(psi); s = -poly_cos(psi); break;
+    case 6: c = poly_sin(phi); s = -poly_cos(phi); break;
+    default: c = poly_cos(psi); s = -poly_sin(psi); break;
+  }
+  return {c, -s};
+}
+template <int N>
+struct RootTable {
+  double re[N];
+  double im[N];
+  constexpr RootTable() : re{}, im{} {
+    for (int e = 0; e < N; ++e) {
+      const dpair w = unit_root(e, N);
+      re[e] = w.re;
+      im[e] = w.im;
+    }
+  }
+};
+template <int N>
+inline constexpr RootTable<N> kRoots{};
+
+// x * W_N^e (conjugated for the inverse); e is a constant after unrolling.
+template <typename T, int N, bool INV>
+__device__ __forceinline__ cplx<T> twiddle_c(cplx<T> x, int e) {
+  if (e == 0) return x;
+  if (4 * e == N) return mul_mi<T, INV>(x);
+  if (2 * e == N) return {-x.re, -x.im};
+  if (4 * e == 3 * N) return mul_mi<T, !INV>(x);
+  const T wr = T(kRoots<N>.re[e]);
+  const T wi = INV ? T(-kRoots<N>.im[e]) : T(kRoots<N>.im[e]);
+  return {x.re * wr - x.im * wi, x.re * wi + x.im * wr};
+}
 
 // ---------------------------------------------------------------- small DFTs
 template <typename T, int R, bool INV>
@@ -107,7 +156,6 @@ struct Dft<T, 8, INV> {
     cplx<T> o[4] = {v[1], v[3], v[5], v[7]};
     Dft<T, 4, INV>::run(e);
     Dft<T, 4, INV>::run(o);
-    // o[q] *= W8^q (conjugated for the inverse)
     cplx<T> o1 = INV ? cplx<T>{h * (o[1].re - o[1].im), h * (o[1].re + o[1].im)}
                      : cplx<T>{h * (o[1].re + o[1].im), h * (o[1].im - o[1].re)};
     cplx<T> o2 = mul_mi<T, INV>(o[2]);
@@ -197,69 +245,218 @@ struct Dft<T, 7, INV> {
   }
 };
 
-// ---------------------------------------------------------------- one stage
-template <typename T, int L, int R, int Ns, int V, int VS, int NT, bool INV>
-__device__ __forceinline__ void fft_stage(cplx<T>* buf, const cplx<T>* tw, int tid) {
-  constexpr int NB = L / R;
-  constexpr int TOT = V * NB;
-  constexpr int BPT = (TOT + NT - 1) / NT;
-  constexpr int TWS = L / (Ns * R);
-  cplx<T> v[BPT][R];
-#pragma unroll
-  for (int b = 0; b < BPT; ++b) {
-    const int idx = tid + b * NT;
-    if (TOT % NT == 0 || idx < TOT) {
-      const int vec = idx / NB;
-      const int j = idx - vec * NB;
-      const cplx<T>* src = buf + vec * VS + j;
-#pragma unroll
-      for (int t = 0; t < R; ++t) v[b][t] = src[t * NB];
-      if constexpr (Ns > 1) {
-        const int k = j % Ns;
-#pragma unroll
-        for (int t = 1; t < R; ++t) {
-          cplx<T> w = tw[t * k * TWS];
-          v[b][t] = INV ? cmulc(v[b][t], w) : cmul(v[b][t], w);
-        }
-      }
-      Dft<T, R, INV>::run(v[b]);
-    }
-  }
-  __syncthreads();
-#pragma unroll
-  for (int b = 0; b < BPT; ++b) {
-    const int idx = tid + b * NT;
-    if (TOT % NT == 0 || idx < TOT) {
-      const int vec = idx / NB;
-      const int j = idx - vec * NB;
-      const int k = j % Ns;
-      cplx<T>* dst = buf + vec * VS + (j - k) * R + k;
-#pragma unroll
-      for (int t = 0; t < R; ++t) dst[t * Ns] = v[b][t];
-    }
-  }
-  __syncthreads();
+// Register DFT of any 7-smooth size: N = A * B with the inner B-point DFTs
+// on the A strided subsequences, constant twiddles W_N^{i k}, and the outer
+// A-point DFTs (output index k + B q).
+__host__ __device__ constexpr bool dft_is_base(int N) {
+  return N == 2 || N == 3 || N == 4 || N == 5 || N == 7 || N == 8;
+}
+__host__ __device__ constexpr int dft_split(int N) {
+  return N % 4 == 0 ? 4 : N % 2 == 0 ? 2 : N % 3 == 0 ? 3 : N % 5 == 0 ? 5 : 7;
 }
 
-template <typename T, int L, int Lrem, int Ns, int V, int VS, int NT, bool INV>
-struct FftStages {
-  static __device__ __forceinline__ void run(cplx<T>* buf, const cplx<T>* tw, int tid) {
-    constexpr int R = radix_at(Lrem, 0);
-    fft_stage<T, L, R, Ns, V, VS, NT, INV>(buf, tw, tid);
-    FftStages<T, L, Lrem / R, Ns * R, V, VS, NT, INV>::run(buf, tw, tid);
-  }
-};
-template <typename T, int L, int Ns, int V, int VS, int NT, bool INV>
-struct FftStages<T, L, 1, Ns, V, VS, NT, INV> {
-  static __device__ __forceinline__ void run(cplx<T>*, const cplx<T>*, int) {}
+template <typename T, int N, bool INV, bool BASE = dft_is_base(N)>
+struct RegDft;
+
+template <typename T, int N, bool INV>
+struct RegDft<T, N, INV, true> {
+  static __device__ __forceinline__ void run(cplx<T>* v) { Dft<T, N, INV>::run(v); }
 };
 
-// Transform V vectors (stride VS) in LDS.  Caller must __syncthreads()
+template <typename T, int N, bool INV>
+struct RegDft<T, N, INV, false> {
+  static constexpr int A = dft_split(N);
+  static constexpr int B = N / A;
+  static __device__ __forceinline__ void run(cplx<T>* v) {
+    cplx<T> u[A][B];
+#pragma unroll
+    for (int i = 0; i < A; ++i)
+#pragma unroll
+      for (int j = 0; j < B; ++j) u[i][j] = v[i + A * j];
+#pragma unroll
+    for (int i = 0; i < A; ++i) RegDft<T, B, INV>::run(u[i]);
+#pragma unroll
+    for (int i = 1; i < A; ++i)
+#pragma unroll
+      for (int k = 1; k < B; ++k) u[i][k] = twiddle_c<T, N, INV>(u[i][k], i * k);
+#pragma unroll
+    for (int k = 0; k < B; ++k) {
+      cplx<T> w[A];
+#pragma unroll
+      for (int i = 0; i < A; ++i) w[i] = u[i][k];
+      RegDft<T, A, INV>::run(w);
+#pragma unroll
+      for (int q = 0; q < A; ++q) v[k + B * q] = w[q];
+    }
+  }
+};
+
+// ---------------------------------------------------------------- plans
+// Forward radix sequence per length; inverse transforms use it reversed, so
+// the last forward stage and the first inverse stage share their register
+// layout (tasks (vec, j) holding positions j + t L/R).
+__host__ __device__ constexpr int plan_radix(int L, int s) {
+  int r0 = 0, r1 = 0, r2 = 0;
+  switch (L) {
+    case 16: r0 = 16; break;
+    case 24: r0 = 24; break;
+    case 32: r0 = 32; break;
+    case 48: r0 = 16; r1 = 3; break;
+    case 64: r0 = 8; r1 = 8; break;
+    case 96: r0 = 16; r1 = 6; break;
+    case 128: r0 = 16; r1 = 8; break;
+    case 192: r0 = 16; r1 = 12; break;
+    case 224: r0 = 16; r1 = 14; break;
+    case 256: r0 = 16; r1 = 16; break;
+    case 324: r0 = 18; r1 = 18; break;
+    case 384: r0 = 16; r1 = 24; break;
+    case 512: r0 = 16; r1 = 32; break;
+    case 768: r0 = 16; r1 = 16; r2 = 3; break;
+    case 1024: r0 = 32; r1 = 32; break;
+    case 2048: r0 = 16; r1 = 16; r2 = 8; break;
+    default: return -1;
+  }
+  return s == 0 ? r0 : s == 1 ? r1 : s == 2 ? r2 : 0;
+}
+__host__ __device__ constexpr int plan_stages(int L) {
+  int n = 0;
+  while (n < 3 && plan_radix(L, n) > 0) ++n;
+  return n;
+}
+__host__ __device__ constexpr bool fft_length_ok(int L) { return plan_radix(L, 0) > 0; }
+// Radix of stage s in execution order (reversed for the inverse).
+__host__ __device__ constexpr int stage_radix(int L, bool inv, int s) {
+  const int n = plan_stages(L);
+  if (s >= n) return 0;
+  return plan_radix(L, inv ? n - 1 - s : s);
+}
+__host__ __device__ constexpr int stage_ns(int L, bool inv, int s) {
+  int ns = 1;
+  for (int i = 0; i < s; ++i) ns *= stage_radix(L, inv, i);
+  return ns;
+}
+// Odd vector stride (in complex elements) for V vectors of length L.
+__host__ __device__ constexpr int odd_stride(int L) { return L | 1; }
+
+// ---------------------------------------------------------------- stages
+template <typename T, int L, bool INV, int S, int V, int VS, int NT>
+struct Stage {
+  static constexpr int R = stage_radix(L, INV, S);
+  static constexpr int Ns = stage_ns(L, INV, S);
+  static constexpr int NB = L / R;
+  static constexpr int TOT = V * NB;
+  static constexpr int BPT = (TOT + NT - 1) / NT;
+  static constexpr int TWS = L / (Ns * R);
+  static_assert((V & (V - 1)) == 0, "V must be a power of two");
+  using Regs = cplx<T>[BPT][R];
+
+  static __device__ __forceinline__ bool task(int tid, int b, int& vec, int& j) {
+    const int idx = tid + b * NT;
+    vec = idx & (V - 1);
+    j = idx / V;
+    return (TOT % NT == 0) || idx < TOT;
+  }
+  static __device__ __forceinline__ void gather_lds(const cplx<T>* buf, Regs& v, int tid) {
+#pragma unroll
+    for (int b = 0; b < BPT; ++b) {
+      int vec, j;
+      if (task(tid, b, vec, j)) {
+        const cplx<T>* src = buf + vec * VS + j;
+#pragma unroll
+        for (int t = 0; t < R; ++t) v[b][t] = src[t * NB];
+      }
+    }
+  }
+  // v[b][t] = src(vec, j + t * NB)
+  template <class Src>
+  static __device__ __forceinline__ void gather(Regs& v, int tid, Src src) {
+#pragma unroll
+    for (int b = 0; b < BPT; ++b) {
+      int vec, j;
+      if (task(tid, b, vec, j)) {
+#pragma unroll
+        for (int t = 0; t < R; ++t) v[b][t] = src(vec, j + t * NB);
+      }
+    }
+  }
+  static __device__ __forceinline__ void compute(Regs& v, const cplx<T>* tw, int tid) {
+#pragma unroll
+    for (int b = 0; b < BPT; ++b) {
+      int vec, j;
+      if (task(tid, b, vec, j)) {
+        if constexpr (Ns > 1) {
+          const int k = j % Ns;
+#pragma unroll
+          for (int t = 1; t < R; ++t) {
+            const cplx<T> w = tw[t * k * TWS];
+            v[b][t] = INV ? cmulc(v[b][t], w) : cmul(v[b][t], w);
+          }
+        }
+        RegDft<T, R, INV>::run(v[b]);
+      }
+    }
+  }
+  static __device__ __forceinline__ void scatter_lds(cplx<T>* buf, const Regs& v, int tid) {
+#pragma unroll
+    for (int b = 0; b < BPT; ++b) {
+      int vec, j;
+      if (task(tid, b, vec, j)) {
+        const int k = j % Ns;
+        cplx<T>* dst = buf + vec * VS + (j - k) * R + k;
+#pragma unroll
+        for (int t = 0; t < R; ++t) dst[t * Ns] = v[b][t];
+      }
+    }
+  }
+  // Last stage only (Ns * R == L): dst(vec, j + t * NB, value).
+  template <class Dst>
+  static __device__ __forceinline__ void emit(const Regs& v, int tid, Dst dst) {
+    static_assert(Ns * R == L, "emit is for the last stage");
+#pragma unroll
+    for (int b = 0; b < BPT; ++b) {
+      int vec, j;
+      if (task(tid, b, vec, j)) {
+#pragma unroll
+        for (int t = 0; t < R; ++t) dst(vec, j + t * NB, v[b][t]);
+      }
+    }
+  }
+  // Apply f(vec, pos, value&) to every value held (pos = j + t NB for the
+  // last stage, i.e. natural output positions).
+  template <class F>
+  static __device__ __forceinline__ void for_each(Regs& v, int tid, F f) {
+#pragma unroll
+    for (int b = 0; b < BPT; ++b) {
+      int vec, j;
+      if (task(tid, b, vec, j)) {
+#pragma unroll
+        for (int t = 0; t < R; ++t) f(vec, j + t * NB, v[b][t]);
+      }
+    }
+  }
+};
+
+// Stages S..end, LDS -> LDS (a barrier after every scatter).
+template <typename T, int L, bool INV, int S, int V, int VS, int NT>
+__device__ __forceinline__ void fft_stages_lds(cplx<T>* buf, const cplx<T>* tw, int tid) {
+  if constexpr (S < plan_stages(L)) {
+    using St = Stage<T, L, INV, S, V, VS, NT>;
+    typename St::Regs v;
+    St::gather_lds(buf, v, tid);
+    St::compute(v, tw, tid);
+    __syncthreads();
+    St::scatter_lds(buf, v, tid);
+    __syncthreads();
+    fft_stages_lds<T, L, INV, S + 1, V, VS, NT>(buf, tw, tid);
+  }
+}
+
+// Whole transform of V vectors in LDS (stride VS).  Caller synchronises
 // after filling ``buf`` and ``tw``; the routine ends with a barrier.
 template <typename T, int L, int V, int VS, int NT, bool INV>
 __device__ __forceinline__ void fft_lds(cplx<T>* buf, const cplx<T>* tw, int tid) {
-  static_assert(fft_length_ok(L), "FFT length must be 7-smooth");
-  FftStages<T, L, L, 1, V, VS, NT, INV>::run(buf, tw, tid);
+  static_assert(fft_length_ok(L), "FFT length has no plan");
+  fft_stages_lds<T, L, INV, 0, V, VS, NT>(buf, tw, tid);
 }
 
 template <typename T>

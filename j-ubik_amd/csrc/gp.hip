@@ -18,8 +18,9 @@ namespace jubik {
 
 constexpr int GNT = 256;
 
-// grid (N / V).  MODE 0: c = amp * xi (forward / tangent);  MODE 1: c = u
-// (complex (H, W) cotangent, zero outside the sky grid).
+// grid (N / V).  MODE 0: c = amp * xi (forward / tangent);  MODE 1: c = the
+// sum of the uplanes complex (H, W) cotangent planes, zero outside the sky
+// grid.
 template <typename T, int N, int MODE>
 __global__ __launch_bounds__(GNT) void gp_cols(GpGeom g, const T* __restrict__ amp_ref,
                                                const T* __restrict__ amp_alpha,
@@ -43,7 +44,9 @@ __global__ __launch_bounds__(GNT) void gp_cols(GpGeom g, const T* __restrict__ a
       val.re = amp_ref[o] * xi_ref[o];
       if (g.has_alpha) val.im = amp_alpha[o] * xi_alpha[o];
     } else {
-      if (x1 < g.H && x2 < g.W) val = u[(int64_t)x1 * g.W + x2];
+      if (x1 < g.H && x2 < g.W)
+        for (int pl = 0; pl < g.uplanes; ++pl)
+          val = cadd(val, u[((int64_t)pl * g.H + x1) * g.W + x2]);
     }
     buf[v * N + x1] = val;
   }

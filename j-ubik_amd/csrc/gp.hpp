@@ -10,6 +10,7 @@ struct GpGeom {
   int E;          // energies
   int has_alpha;  // spectral-index field present
   double zm, mu;  // zero mode and spectral-index mean
+  int uplanes;    // adjoint input: number of (H, W) complex planes summed on load
 };
 
 struct GpBuffers {
