@@ -5,10 +5,12 @@ visible, every compute call raises.  Status codes map to the exceptions the
 reference raises (ValueError for invalid arguments).
 """
 import ctypes as C
+import os
 import threading
 from pathlib import Path
 
-LIB_PATH = Path(__file__).resolve().parent / "libjubik_hip.so"
+LIB_PATH = Path(os.environ.get("JUBIK_HIP_LIB") or
+                Path(__file__).resolve().parent / "libjubik_hip.so")
 
 OK, EINVAL, EUNSUPPORTED, EHIP, ENOMEM, ENOTREADY = 0, -1, -2, -3, -4, -5
 FLOAT32, FLOAT64 = 0, 1

@@ -300,12 +300,13 @@ int jubik_linpatch_create(jubik_linpatch** plan, int dtype, int64_t H, int64_t W
   const size_t es = elem_size(dtype);
   const size_t cs = 2 * es;
   const size_t nspec = (size_t)g.M * g.P * g.Ek;
-  p->spec_bytes = nspec * g.L2 * g.L * cs;
-  p->ws_bytes = (size_t)g.J * g.P * g.L2 * g.L * cs;
+  const size_t l2p = (size_t)half_stride(g.L);
+  p->spec_bytes = nspec * l2p * g.L * cs;
+  p->ws_bytes = (size_t)g.J * g.P * l2p * g.L * cs;
   int st = upload_twiddles(&p->tw, dtype, g.L);
   if (!st) st = upload_tent(&p->tent, dtype, g.d);
   if (!st) st = dev_alloc(&p->KT, p->spec_bytes, "kernel spectra");
-  if (!st) st = dev_alloc(&p->R1, std::max(p->ws_bytes, nspec * g.L2 * g.w * cs), "workspace R1");
+  if (!st) st = dev_alloc(&p->R1, std::max(p->ws_bytes, nspec * l2p * g.w * cs), "workspace R1");
   if (!st) st = dev_alloc(&p->R2, p->ws_bytes, "workspace R2");
   if (st) {
     p->release();

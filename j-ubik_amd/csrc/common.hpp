@@ -57,6 +57,11 @@ struct TimedLaunch {
 // margin m to the FFT length L = w + 2m.  Window starts are
 // min(i * d, lim), the clamping of jax.lax.dynamic_slice; the same starts
 // serve the slicing of the image and the overlap-add canvas.
+// Row stride of the half-spectrum arrays (R1, R2, kernel spectra): L/2 + 1
+// complex values rounded up to 16 (128 bytes in FP32), so the 16-column
+// tiles of the column pass are single aligned cache lines.
+__host__ __device__ constexpr int half_stride(int L) { return (L / 2 + 1 + 15) / 16 * 16; }
+
 struct LinGeom {
   int H, W;       // sky grid
   int n;          // patches per axis
@@ -85,18 +90,23 @@ __device__ __forceinline__ int64_t spec_index(const LinGeom& g, int job, int p) 
 
 // Tile sizes: vectors per work-group so a tile of V vectors of length L at
 // the odd LDS stride occupies at most 52 KiB (16 float vectors at L = 384).
+#ifndef JUBIK_COL_TILE_BYTES
+#define JUBIK_COL_TILE_BYTES 53248
+#endif
+__host__ __device__ constexpr int tile_vectors(int bytes, int L, int elem) {
+  const int raw = bytes / ((L | 1) * elem);
+  return raw >= 32 ? 32 : raw >= 16 ? 16 : raw >= 8 ? 8 : raw >= 4 ? 4 : raw >= 2 ? 2 : 1;
+}
 template <typename T, int L>
 struct Tile {
-  static constexpr int bytes = 53248;
-  static constexpr int raw = bytes / ((L | 1) * (int)sizeof(cplx<T>));
-  static constexpr int V = raw >= 32 ? 32 : raw >= 16 ? 16 : raw >= 8 ? 8 : raw >= 4 ? 4 : raw >= 2 ? 2 : 1;
+  static constexpr int V = tile_vectors(JUBIK_COL_TILE_BYTES, L, (int)sizeof(cplx<T>));
 };
 
-// Row passes use half the column tile: their global traffic is contiguous
-// rows whatever V is, and the smaller LDS footprint doubles residency.
+// Row passes use half that LDS budget: their global traffic is contiguous
+// rows whatever V is, and the smaller footprint doubles residency.
 template <typename T, int L>
 struct RowTile {
-  static constexpr int V = Tile<T, L>::V >= 2 ? Tile<T, L>::V / 2 : 1;
+  static constexpr int V = tile_vectors(26624, L, (int)sizeof(cplx<T>));
 };
 
 // Block-wide sum of one double per thread (NT a multiple of 64).

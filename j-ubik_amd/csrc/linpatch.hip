@@ -1,574 +1,25 @@
-// Spatially variant PSF by linear patching (jubik/convolve.py:66-145) and
-// its adjoint, as three fused passes per direction:
-//
-//   rows : gather two real window rows per complex vector (tent weights and
-//          zero margins applied on the fly), FFT along the row, split the two
-//          half spectra and store them row-major (R1[.][r][k], k < L/2 + 1).
-//   cols : read V adjacent spectrum columns straight into registers, FFT
-//          along the columns, multiply by the prepared kernel spectrum in
-//          registers (conjugated for the adjoint), inverse FFT starting from
-//          the same registers, store row-major (R2[.][f][k]).
-//   out  : per output image row, rebuild the full spectra of two patches per
-//          complex vector, inverse FFT, and sum the patch rows that cover the
-//          row in a fixed order (gather-form overlap-add, no atomics), with a
-//          fused epilogue (plain store, or exposure + Poisson residual + FP64
-//          NLL partials, or the chain into the GP adjoint).
-//
-// Kernel spectra are prepared by the same rows/cols passes from the cut
-// window, with the ifftshift applied as (-1)^(k1+k2) and the 1/(sum L^2)
-// normalisation folded in (convolve.py:104-131 and 162-176; dvol cancels);
-// they are stored KT[spec][f][k].
-#include <type_traits>
-
+// Linear-patching PSF entry points: dispatch on the element type to the
+// float (linpatch_f32.hip) and double (linpatch_f64.hip) instantiations of
+// the kernels in linpatch_impl.hpp.
 #include "linpatch.hpp"
 
 namespace jubik {
 
-enum RowMode { ROW_SIGNAL = 0, ROW_COTANGENT = 1, ROW_KERNEL_WIN = 2, ROW_KERNEL_FULL = 3 };
-enum ColMode { COL_FWD = 0, COL_ADJ = 1, COL_KERNEL = 2 };
-
-constexpr int NT = 256;
-constexpr int MAXN = 64;
-
-// ------------------------------------------------------------------ rows
-// grid (ceil(nrows/2 / V), P, Z); Z = jobs (signal modes) or M*Ek (kernel).
-template <typename T, int L, int MODE>
-__global__ __launch_bounds__(NT) void lp_rows(LinGeom g, const T* __restrict__ src,
-                                              const T* __restrict__ tent,
-                                              const cplx<T>* __restrict__ twg,
-                                              cplx<T>* __restrict__ R1, T scale, int nrows) {
-  constexpr int V = RowTile<T, L>::V;
-  constexpr int VS = odd_stride(L);
-  constexpr int L2 = L / 2 + 1;
-  __shared__ cplx<T> buf[V * VS];
-  __shared__ cplx<T> tw[L];
-  const int tid = threadIdx.x;
-  const int p = blockIdx.y;
-  const int z = blockIdx.z;
-  const int q0 = blockIdx.x * V;
-  const int ix = p % g.n, iy = p / g.n;
-  const int sx = start_x(g, ix), sy = start_y(g, iy);
-
-  int64_t r1base;
-  const T* img;
-  if constexpr (MODE == ROW_SIGNAL || MODE == ROW_COTANGENT) {
-    const int job = z;
-    r1base = (int64_t)job * g.P + p;
-    int im = job;
-    if (MODE == ROW_SIGNAL && g.x_shared) im = job % g.E;
-    img = src + (int64_t)im * g.H * g.W;
-  } else {
-    const int mod = z / g.Ek, ek = z - mod * g.Ek;
-    const int64_t spec = ((int64_t)mod * g.P + p) * g.Ek + ek;
-    r1base = spec;
-    img = src + spec * (MODE == ROW_KERNEL_WIN ? (int64_t)g.w * g.w : (int64_t)g.H * g.W);
-  }
-
-  load_twiddles(tw, twg, L, tid, NT);
-  constexpr int EPT = (V * L + NT - 1) / NT;
-#pragma unroll
-  for (int u = 0; u < EPT; ++u) {
-    const int e = tid + u * NT;
-    if (V * L % NT != 0 && e >= V * L) break;
-    const int v = e / L, f = e - (e / L) * L;
-    const int r0 = 2 * (q0 + v);
-    T a = T(0), b = T(0);
-    if (r0 < nrows) {
-      if constexpr (MODE == ROW_SIGNAL) {
-        const int c = f - g.m;
-        if (c >= 0 && c < g.w) {
-          const int j = sy + c - g.h;
-          const int i0 = sx + r0 - g.h;
-          if (j >= 0 && j < g.W) {
-            const T wc = tent[c] * scale;
-            if (i0 >= 0 && i0 < g.H) a = tent[r0] * wc * img[i0 * g.W + j];
-            if (i0 + 1 >= 0 && i0 + 1 < g.H) b = tent[r0 + 1] * wc * img[(i0 + 1) * g.W + j];
-          }
-        }
-      } else if constexpr (MODE == ROW_COTANGENT) {
-        const int j = sy + f - g.h - g.m;
-        const int i0 = sx + r0 - g.h - g.m;
-        if (j >= 0 && j < g.W) {
-          if (i0 >= 0 && i0 < g.H) a = img[i0 * g.W + j];
-          if (i0 + 1 >= 0 && i0 + 1 < g.H) b = img[(i0 + 1) * g.W + j];
-        }
-      } else if constexpr (MODE == ROW_KERNEL_WIN) {
-        const int c = f - g.m;
-        if (c >= 0 && c < g.w) {
-          a = img[r0 * g.w + c];
-          b = img[(r0 + 1) * g.w + c];
-        }
-      } else {  // ROW_KERNEL_FULL: cut[r][c] = K[(r - d) mod H][(c - d) mod W]
-        const int c = f - g.m;
-        if (c >= 0 && c < g.w) {
-          const int j = (c - g.d + g.W) % g.W;
-          const int ia = (r0 - g.d + g.H) % g.H;
-          const int ib = (r0 + 1 - g.d + g.H) % g.H;
-          a = img[ia * g.W + j];
-          b = img[ib * g.W + j];
-        }
-      }
-    }
-    buf[v * VS + f] = cplx<T>{a, b};
-  }
-  __syncthreads();
-  fft_lds<T, L, V, VS, NT, false>(buf, tw, tid);
-
-  cplx<T>* out = R1 + r1base * L2 * nrows;
-  for (int e = tid; e < V * L2; e += NT) {
-    const int v = e / L2, k = e - (e / L2) * L2;
-    const int r0 = 2 * (q0 + v);
-    if (r0 >= nrows) continue;
-    const cplx<T> zk = buf[v * VS + k];
-    const cplx<T> zn = buf[v * VS + (k == 0 ? 0 : L - k)];
-    const cplx<T> A = {T(0.5) * (zk.re + zn.re), T(0.5) * (zk.im - zn.im)};
-    const cplx<T> B = {T(0.5) * (zk.im + zn.im), T(0.5) * (zn.re - zk.re)};
-    out[r0 * L2 + k] = A;
-    out[(r0 + 1) * L2 + k] = B;
-  }
-}
-
-// ------------------------------------------------------------------ cols
-// Forward stages, then (signal modes) the spectrum multiply and the inverse
-// stages; the last forward and first inverse stage share registers.
-template <typename T, int L, bool INV, int S, int V, int VS, int NT_>
-__device__ __forceinline__ void stages_mid(cplx<T>* buf, const cplx<T>* tw, int tid) {
-  // stages S .. plan_stages(L) - 2 (all but the last), LDS -> LDS
-  if constexpr (S + 1 < plan_stages(L)) {
-    using St = Stage<T, L, INV, S, V, VS, NT_>;
-    typename St::Regs v;
-    St::gather_lds(buf, v, tid);
-    St::compute(v, tw, tid);
-    __syncthreads();
-    St::scatter_lds(buf, v, tid);
-    __syncthreads();
-    stages_mid<T, L, INV, S + 1, V, VS, NT_>(buf, tw, tid);
-  }
-}
-
-// grid (ceil(L2 / V), P, Z).
-template <typename T, int L, int MODE>
-__global__ __launch_bounds__(NT) void lp_cols(LinGeom g, const cplx<T>* __restrict__ R1,
-                                              const cplx<T>* __restrict__ KT,
-                                              const cplx<T>* __restrict__ twg,
-                                              cplx<T>* __restrict__ out) {
-  constexpr int V = Tile<T, L>::V;
-  constexpr int VS = odd_stride(L);
-  constexpr int L2 = L / 2 + 1;
-  constexpr int NS = plan_stages(L);
-  __shared__ cplx<T> buf[V * VS];
-  __shared__ cplx<T> tw[L];
-  __shared__ double red[NT / 64];
-  __shared__ double total_s;
-  const int tid = threadIdx.x;
-  const int p = blockIdx.y;
-  const int z = blockIdx.z;
-  const int k0 = blockIdx.x * V;
-
-  int64_t idx, spec;
-  if constexpr (MODE == COL_KERNEL) {
-    const int mod = z / g.Ek, ek = z - mod * g.Ek;
-    spec = ((int64_t)mod * g.P + p) * g.Ek + ek;
-    idx = spec;
-  } else {
-    idx = (int64_t)z * g.P + p;
-    spec = spec_index(g, z, p);
-  }
-  const int nin = (MODE == COL_ADJ) ? L : g.w;
-  const cplx<T>* in = R1 + idx * L2 * nin;
-  const int m = g.m, w = g.w;
-
-  load_twiddles(tw, twg, L, tid, NT);
-  T factor = T(1);
-  if constexpr (MODE == COL_KERNEL) {
-    double s = 0.0;
-    for (int rr = tid; rr < w; rr += NT) s += (double)in[rr * L2].re;
-    s = block_sum<NT>(s, red);
-    if (tid == 0) total_s = s;
-  }
-  __syncthreads();
-  if constexpr (MODE == COL_KERNEL) factor = T(1.0 / (total_s * (double)L * (double)L));
-
-  // forward stage 0 straight from global memory
-  using F0 = Stage<T, L, false, 0, V, VS, NT>;
-  using FL = Stage<T, L, false, NS - 1, V, VS, NT>;
-  typename FL::Regs last;
-  auto load_col = [&](int vec, int pos) -> cplx<T> {
-    const int k = k0 + vec;
-    if (k >= L2) return cplx<T>{T(0), T(0)};
-    if constexpr (MODE == COL_ADJ) {
-      return in[pos * L2 + k];
-    } else {
-      const int rr = pos - m;
-      return (rr >= 0 && rr < w) ? in[rr * L2 + k] : cplx<T>{T(0), T(0)};
-    }
-  };
-  if constexpr (NS == 1) {
-    F0::gather(last, tid, load_col);
-    F0::compute(last, tw, tid);
-  } else {
-    typename F0::Regs v0;
-    F0::gather(v0, tid, load_col);
-    F0::compute(v0, tw, tid);
-    F0::scatter_lds(buf, v0, tid);
-    __syncthreads();
-    stages_mid<T, L, false, 1, V, VS, NT>(buf, tw, tid);
-    FL::gather_lds(buf, last, tid);
-    FL::compute(last, tw, tid);
-  }
-
-  if constexpr (MODE == COL_KERNEL) {
-    cplx<T>* kt = out + spec * L2 * L;
-    FL::emit(last, tid, [&](int vec, int pos, cplx<T> val) {
-      const int k = k0 + vec;
-      if (k < L2) {
-        const T sgn = ((k + pos) & 1) ? -factor : factor;
-        kt[pos * L2 + k] = cscale(val, sgn);
-      }
-    });
-    return;
-  } else {
-    const cplx<T>* ks = KT + spec * L2 * L;
-    FL::for_each(last, tid, [&](int vec, int pos, cplx<T>& val) {
-      const int k = k0 + vec;
-      if (k < L2) {
-        const cplx<T> kv = ks[pos * L2 + k];
-        val = (MODE == COL_ADJ) ? cmulc(val, kv) : cmul(val, kv);
-      }
-    });
-    using I0 = Stage<T, L, true, 0, V, VS, NT>;
-    using IL = Stage<T, L, true, NS - 1, V, VS, NT>;
-    static_assert(I0::R == FL::R && I0::BPT == FL::BPT, "shared register layout");
-    I0::compute(last, tw, tid);
-    cplx<T>* o = out + idx * L2 * (MODE == COL_ADJ ? w : L);
-    auto store = [&](int vec, int pos, cplx<T> val) {
-      const int k = k0 + vec;
-      if (k >= L2) return;
-      if constexpr (MODE == COL_ADJ) {
-        const int r = pos - m;
-        if (r >= 0 && r < w) o[r * L2 + k] = val;
-      } else {
-        o[pos * L2 + k] = val;
-      }
-    };
-    if constexpr (NS == 1) {
-      I0::emit(last, tid, store);
-    } else {
-      __syncthreads();
-      I0::scatter_lds(buf, last, tid);
-      __syncthreads();
-      stages_mid<T, L, true, 1, V, VS, NT>(buf, tw, tid);
-      typename IL::Regs v;
-      IL::gather_lds(buf, v, tid);
-      IL::compute(v, tw, tid);
-      IL::emit(v, tid, store);
-    }
-  }
-}
-
-// ------------------------------------------------------------------ overlap
-// The overlap kernels own one output image row per work-group.  Patch rows
-// of one patch column (same iy, hence the same column offset) enter the
-// row with plain (forward) or tent-weighted (adjoint) sums, which commute
-// with the inverse row FFT: their half-spectrum rows are summed first, so
-// each work-group runs one inverse FFT per pair of patch columns (patch
-// columns iy = 2q, 2q+1 share one complex vector as real and imaginary
-// part).  Output columns are owned by threads (j = tid + t * NT) and summed
-// in registers in a fixed order, so results are deterministic.
-// Output columns per thread (JPT): W <= JPT * NT; instantiated for 4 and 16.
-
-template <typename T, int L>
-struct OutTile {
-  static constexpr int V = 8;  // iy pairs per pass: n <= 16 in one pass
-  static constexpr int EPT = (V * L + NT - 1) / NT;
-};
-
-// First patch column whose window [s, s + len) can contain canvas column C.
-__device__ __forceinline__ int first_cover(int C, int len, int d) {
-  return C >= len ? (C - len) / d + 1 : 0;
-}
-
-// grid (H, J).
-template <typename T, int L, int EPI, int JPT>
-__global__ __launch_bounds__(NT) void lp_fwd_out(LinGeom g, const cplx<T>* __restrict__ R2,
-                                                 const cplx<T>* __restrict__ twg,
-                                                 T* __restrict__ y, const T* __restrict__ expo,
-                                                 const int* __restrict__ counts,
-                                                 T* __restrict__ resid,
-                                                 double* __restrict__ partial) {
-  constexpr int V = OutTile<T, L>::V;
-  constexpr int EPT = OutTile<T, L>::EPT;
-  constexpr int VS = odd_stride(L);
-  constexpr int L2 = L / 2 + 1;
-  __shared__ cplx<T> buf[V * VS];
-  __shared__ cplx<T> tw[L];
-  __shared__ int lst_ix[MAXN], lst_f[MAXN];
-  __shared__ int nc_s;
-  __shared__ double red[NT / 64];
-  const int tid = threadIdx.x;
-  const int i = blockIdx.x;
-  const int job = blockIdx.y;
-  const int R = i + g.h + g.m;
-
-  load_twiddles(tw, twg, L, tid, NT);
-  if (tid == 0) {
-    int nc = 0;
-    for (int ix = 0; ix < g.n; ++ix) {
-      const int s = start_x(g, ix);
-      if (R >= s && R < s + L) {
-        lst_ix[nc] = ix;
-        lst_f[nc] = R - s;
-        ++nc;
-      }
-    }
-    nc_s = nc;
-  }
-  T acc[JPT];
-#pragma unroll
-  for (int t = 0; t < JPT; ++t) acc[t] = T(0);
-  __syncthreads();
-  const int nc = nc_s;
-  const int npair = (g.n + 1) / 2;
-  const cplx<T>* base = R2 + (int64_t)job * g.P * L * L2;
-
-  for (int q0 = 0; q0 < npair; q0 += V) {
-    // buf[v][k] = sum over the covering patch rows of A + iB (thread-owned
-    // elements: no barrier between the contributions)
-    for (int ci = 0; ci < nc; ++ci) {
-      const int ix = lst_ix[ci], f = lst_f[ci];
-#pragma unroll
-      for (int u = 0; u < EPT; ++u) {
-        const int e = tid + u * NT;
-        if (V * L % NT != 0 && e >= V * L) break;
-        const int v = e / L, k = e - (e / L) * L;
-        const int q = q0 + v;
-        cplx<T> z = {T(0), T(0)};
-        if (q < npair) {
-          const bool cj = k > L / 2;
-          const int kk = cj ? L - k : k;
-          const cplx<T> a = base[((2 * q * g.n + ix) * L + f) * L2 + kk];
-          cplx<T> b = {T(0), T(0)};
-          if (2 * q + 1 < g.n) b = base[(((2 * q + 1) * g.n + ix) * L + f) * L2 + kk];
-          const T ai = cj ? -a.im : a.im, bi = cj ? -b.im : b.im;
-          z = cplx<T>{a.re - bi, ai + b.re};
-        }
-        cplx<T>& slot = buf[v * VS + k];
-        slot = ci == 0 ? z : cadd(slot, z);
-      }
-    }
-    __syncthreads();
-    fft_lds<T, L, V, VS, NT, true>(buf, tw, tid);
-#pragma unroll
-    for (int t = 0; t < JPT; ++t) {
-      const int j = tid + t * NT;
-      if (j < g.W) {
-        const int C = j + g.h + g.m;
-        T s = T(0);
-        for (int iy = first_cover(C, L, g.d); iy < g.n; ++iy) {
-          const int sy = start_y(g, iy);
-          if (sy > C) break;
-          const int c = C - sy;
-          const int v = (iy >> 1) - q0;
-          if (c < L && v >= 0 && v < V) {
-            const cplx<T> z = buf[v * VS + c];
-            s += (iy & 1) ? z.im : z.re;
-          }
-        }
-        acc[t] += s;
-      }
-    }
-    __syncthreads();
-  }
-
-  const int64_t row = ((int64_t)job * g.H + i) * g.W;
-  if constexpr (EPI == 0) {
-#pragma unroll
-    for (int t = 0; t < JPT; ++t) {
-      const int j = tid + t * NT;
-      if (j < g.W) y[row + j] = acc[t];
-    }
-  } else if constexpr (EPI == 2) {
-    // Metric: cotangent expo * (d lam / lam) on read-out pixels, y = lam.
-#pragma unroll
-    for (int t = 0; t < JPT; ++t) {
-      const int j = tid + t * NT;
-      if (j < g.W) {
-        const T ex = expo[row + j];
-        resid[row + j] = counts[row + j] >= 0 ? ex * (ex * acc[t]) / y[row + j] : T(0);
-      }
-    }
-  } else {
-    double nll = 0.0;
-#pragma unroll
-    for (int t = 0; t < JPT; ++t) {
-      const int j = tid + t * NT;
-      if (j < g.W) {
-        const T ex = expo[row + j];
-        const T lam = ex * acc[t];
-        const int d = counts[row + j];
-        T r = T(0);
-        if (d >= 0) {
-          const double dl = (double)lam;
-          nll += dl - (double)d * log(dl);
-          r = ex * (T(1) - T(d) / lam);
-        }
-        resid[row + j] = r;
-        if (y) y[row + j] = lam;
-      }
-    }
-    nll = block_sum<NT>(nll, red);
-    if (tid == 0) partial[(int64_t)job * g.H + i] = nll;
-  }
-}
-
-// EPI 0: grid (H, nimg); gx[img] = scale * sum over the jobs of img.
-// EPI 1: grid (H, E);  u[e] = sum_mod scale * sky[e] * xbar_(mod,e) * (1, dnu[e]),
-//        one complex (u_ref, u_alpha) plane per energy (the GP adjoint sums
-//        the planes while loading them).
-template <typename T, int L, int EPI, int JPT>
-__global__ __launch_bounds__(NT) void lp_adj_out(LinGeom g, const cplx<T>* __restrict__ R2,
-                                                 const cplx<T>* __restrict__ twg,
-                                                 const T* __restrict__ tent,
-                                                 T* __restrict__ gx, T scale,
-                                                 const T* __restrict__ sky,
-                                                 const T* __restrict__ dnu,
-                                                 cplx<T>* __restrict__ u) {
-  constexpr int V = OutTile<T, L>::V;
-  constexpr int EPT = OutTile<T, L>::EPT;
-  constexpr int VS = odd_stride(L);
-  constexpr int L2 = L / 2 + 1;
-  __shared__ cplx<T> buf[V * VS];
-  __shared__ cplx<T> tw[L];
-  __shared__ int lst_ix[MAXN], lst_r[MAXN];
-  __shared__ T lst_w[MAXN];
-  __shared__ int nc_s;
-  const int tid = threadIdx.x;
-  const int i = blockIdx.x;
-  const int Rp = i + g.h;
-
-  load_twiddles(tw, twg, L, tid, NT);
-  if (tid == 0) {
-    int nc = 0;
-    for (int ix = 0; ix < g.n; ++ix) {
-      const int s = start_x(g, ix);
-      if (Rp >= s && Rp < s + g.w) {
-        lst_ix[nc] = ix;
-        lst_r[nc] = Rp - s;
-        lst_w[nc] = tent[Rp - s];
-        ++nc;
-      }
-    }
-    nc_s = nc;
-  }
-  T acc[JPT], acc2[JPT], tmp[JPT];
-#pragma unroll
-  for (int t = 0; t < JPT; ++t) acc[t] = acc2[t] = T(0);
-  __syncthreads();
-  const int nc = nc_s;
-  const int npair = (g.n + 1) / 2;
-
-  // EPI 0 with a shared input and EPI 1: the jobs (mod, e) of energy e =
-  // blockIdx.y, summed over modules; EPI 0 otherwise: job blockIdx.y.
-  int jb = blockIdx.y, je = g.J, jstep = g.E;
-  if (EPI == 0 && !g.x_shared) {
-    je = blockIdx.y + 1;
-    jstep = 1;
-  }
-
-  for (int job = jb; job < je; job += jstep) {
-    const cplx<T>* base = R2 + (int64_t)job * g.P * g.w * L2;
-#pragma unroll
-    for (int t = 0; t < JPT; ++t) tmp[t] = T(0);
-    for (int q0 = 0; q0 < npair; q0 += V) {
-      for (int ci = 0; ci < nc; ++ci) {
-        const int ix = lst_ix[ci], r = lst_r[ci];
-        const T wr = lst_w[ci];
-#pragma unroll
-        for (int uu = 0; uu < EPT; ++uu) {
-          const int e = tid + uu * NT;
-          if (V * L % NT != 0 && e >= V * L) break;
-          const int v = e / L, k = e - (e / L) * L;
-          const int q = q0 + v;
-          cplx<T> z = {T(0), T(0)};
-          if (q < npair) {
-            const bool cj = k > L / 2;
-            const int kk = cj ? L - k : k;
-            const cplx<T> a = base[((2 * q * g.n + ix) * g.w + r) * L2 + kk];
-            cplx<T> b = {T(0), T(0)};
-            if (2 * q + 1 < g.n)
-              b = base[(((2 * q + 1) * g.n + ix) * g.w + r) * L2 + kk];
-            const T ai = cj ? -a.im : a.im, bi = cj ? -b.im : b.im;
-            z = cplx<T>{wr * (a.re - bi), wr * (ai + b.re)};
-          }
-          cplx<T>& slot = buf[v * VS + k];
-          slot = ci == 0 ? z : cadd(slot, z);
-        }
-      }
-      __syncthreads();
-      fft_lds<T, L, V, VS, NT, true>(buf, tw, tid);
-#pragma unroll
-      for (int t = 0; t < JPT; ++t) {
-        const int j = tid + t * NT;
-        if (j < g.W) {
-          const int Cp = j + g.h;
-          T s = T(0);
-          for (int iy = first_cover(Cp, g.w, g.d); iy < g.n; ++iy) {
-            const int sy = start_y(g, iy);
-            if (sy > Cp) break;
-            const int c = Cp - sy;
-            const int v = (iy >> 1) - q0;
-            if (c < g.w && v >= 0 && v < V) {
-              const cplx<T> z = buf[v * VS + g.m + c];
-              s += tent[c] * ((iy & 1) ? z.im : z.re);
-            }
-          }
-          tmp[t] += s;
-        }
-      }
-      __syncthreads();
-    }
-    if (EPI == 0) {
-#pragma unroll
-      for (int t = 0; t < JPT; ++t) acc[t] += tmp[t];
-    } else {
-      const int e = job % g.E;
-      const T dn = dnu ? dnu[e] : T(0);
-      const T* srow = sky + ((int64_t)e * g.H + i) * g.W;
-#pragma unroll
-      for (int t = 0; t < JPT; ++t) {
-        const int j = tid + t * NT;
-        if (j < g.W) {
-          const T val = scale * srow[j] * tmp[t];
-          acc[t] += val;
-          acc2[t] += dn * val;
-        }
-      }
-    }
-  }
-  if (EPI == 0) {
-    const int64_t row = ((int64_t)blockIdx.y * g.H + i) * g.W;
-#pragma unroll
-    for (int t = 0; t < JPT; ++t) {
-      const int j = tid + t * NT;
-      if (j < g.W) gx[row + j] = scale * acc[t];
-    }
-  } else {
-    const int64_t row = ((int64_t)blockIdx.y * g.H + i) * g.W;
-#pragma unroll
-    for (int t = 0; t < JPT; ++t) {
-      const int j = tid + t * NT;
-      if (j < g.W) u[row + j] = cplx<T>{acc[t], acc2[t]};
-    }
-  }
-}
-
-}  // namespace jubik
-
-// ====================================================================== host
-namespace jubik {
-
-#define JUBIK_LP_LENGTHS(X) \
-  X(16) X(24) X(32) X(48) X(64) X(96) X(128) X(192) X(224) X(256) X(324) X(384)
+template <typename T>
+int lp_prepare_t(const LinGeom&, const LinBuffers&, const void*, int, hipStream_t);
+template <typename T>
+int lp_forward_t(const LinGeom&, const LinBuffers&, const void*, void*, double, hipStream_t);
+template <typename T>
+int lp_forward_poisson_t(const LinGeom&, const LinBuffers&, const void*, double, const void*,
+                         const int*, void*, void*, double*, hipStream_t);
+template <typename T>
+int lp_forward_metric_t(const LinGeom&, const LinBuffers&, const void*, double, const void*,
+                        const int*, const void*, void*, hipStream_t);
+template <typename T>
+int lp_adjoint_t(const LinGeom&, const LinBuffers&, const void*, void*, double, hipStream_t);
+template <typename T>
+int lp_adjoint_gp_t(const LinGeom&, const LinBuffers&, const void*, double, const void*,
+                    const void*, void*, hipStream_t);
 
 bool linpatch_length_supported(int L) {
   switch (L) {
@@ -581,196 +32,45 @@ bool linpatch_length_supported(int L) {
   }
 }
 
-static int launch_status(const char* what) {
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return check_hip(e, what);
-  return kOk;
-}
-
-template <typename T, int L>
-struct LinOps {
-  static constexpr int V = Tile<T, L>::V;
-  static constexpr int VR = RowTile<T, L>::V;
-  static constexpr int L2 = L / 2 + 1;
-  using C = cplx<T>;
-
-  static int prepare(const LinGeom& g, const LinBuffers& b, const void* kernel, int layout,
-                     hipStream_t s) {
-    dim3 gr((g.w / 2 + VR - 1) / VR, g.P, g.M * g.Ek);
-    if (layout == 0)
-      {
-        TimedLaunch _t("lp_rows_kernel", s);
-        lp_rows<T, L, ROW_KERNEL_FULL><<<gr, NT, 0, s>>>(g, (const T*)kernel, (const T*)b.tent,
-                                                         (const C*)b.tw, (C*)b.R1, T(1), g.w);
-      }
-    else
-      {
-        TimedLaunch _t("lp_rows_kernel", s);
-        lp_rows<T, L, ROW_KERNEL_WIN><<<gr, NT, 0, s>>>(g, (const T*)kernel, (const T*)b.tent,
-                                                        (const C*)b.tw, (C*)b.R1, T(1), g.w);
-      }
-    dim3 gc((L2 + V - 1) / V, g.P, g.M * g.Ek);
-    {
-      TimedLaunch _t("lp_cols_kernel", s);
-      lp_cols<T, L, COL_KERNEL><<<gc, NT, 0, s>>>(g, (const C*)b.R1, nullptr, (const C*)b.tw,
-                                                  (C*)b.KT);
-    }
-    return launch_status("linpatch_prepare");
-  }
-
-  static void fwd_front(const LinGeom& g, const LinBuffers& b, const void* x, double scale,
-                        hipStream_t s) {
-    dim3 gr((g.w / 2 + VR - 1) / VR, g.P, g.J);
-    {
-      TimedLaunch _t("lp_rows_fwd", s);
-      lp_rows<T, L, ROW_SIGNAL><<<gr, NT, 0, s>>>(g, (const T*)x, (const T*)b.tent, (const C*)b.tw,
-                                                  (C*)b.R1, T(scale), g.w);
-    }
-    dim3 gc((L2 + V - 1) / V, g.P, g.J);
-    {
-      TimedLaunch _t("lp_cols_fwd", s);
-      lp_cols<T, L, COL_FWD><<<gc, NT, 0, s>>>(g, (const C*)b.R1, (const C*)b.KT, (const C*)b.tw,
-                                               (C*)b.R2);
-    }
-  }
-
-  static void adj_front(const LinGeom& g, const LinBuffers& b, const void* gy, hipStream_t s) {
-    dim3 gr((L / 2 + VR - 1) / VR, g.P, g.J);
-    {
-      TimedLaunch _t("lp_rows_adj", s);
-      lp_rows<T, L, ROW_COTANGENT><<<gr, NT, 0, s>>>(g, (const T*)gy, (const T*)b.tent,
-                                                     (const C*)b.tw, (C*)b.R1, T(1), L);
-    }
-    dim3 gc((L2 + V - 1) / V, g.P, g.J);
-    {
-      TimedLaunch _t("lp_cols_adj", s);
-      lp_cols<T, L, COL_ADJ><<<gc, NT, 0, s>>>(g, (const C*)b.R1, (const C*)b.KT, (const C*)b.tw,
-                                               (C*)b.R2);
-    }
-  }
-
-  // Output columns per thread: 4 covers W <= 1024, 16 covers W <= 4096.
-  template <class F>
-  static void with_jpt(const LinGeom& g, F&& f) {
-    if (g.W <= 4 * NT)
-      f(std::integral_constant<int, 4>{});
-    else
-      f(std::integral_constant<int, 16>{});
-  }
-
-  template <int EPI>
-  static void fwd_out(const LinGeom& g, const LinBuffers& b, T* y, const T* expo,
-                      const int* counts, T* resid, double* partial, hipStream_t s,
-                      const char* name) {
-    TimedLaunch _t(name, s);
-    with_jpt(g, [&](auto jpt) {
-      lp_fwd_out<T, L, EPI, decltype(jpt)::value><<<dim3(g.H, g.J), NT, 0, s>>>(
-          g, (const C*)b.R2, (const C*)b.tw, y, expo, counts, resid, partial);
-    });
-  }
-
-  template <int EPI>
-  static void adj_out(const LinGeom& g, const LinBuffers& b, int nimg, T* gx, double scale,
-                      const T* sky, const T* dnu, C* u, hipStream_t s, const char* name) {
-    TimedLaunch _t(name, s);
-    with_jpt(g, [&](auto jpt) {
-      lp_adj_out<T, L, EPI, decltype(jpt)::value><<<dim3(g.H, nimg), NT, 0, s>>>(
-          g, (const C*)b.R2, (const C*)b.tw, (const T*)b.tent, gx, T(scale), sky, dnu, u);
-    });
-  }
-
-  static int forward(const LinGeom& g, const LinBuffers& b, const void* x, void* y, double scale,
-                     hipStream_t s) {
-    fwd_front(g, b, x, scale, s);
-    fwd_out<0>(g, b, (T*)y, nullptr, nullptr, nullptr, nullptr, s, "lp_fwd_out");
-    return launch_status("linpatch_forward");
-  }
-
-  static int forward_poisson(const LinGeom& g, const LinBuffers& b, const void* x, double scale,
-                             const void* expo, const int* counts, void* resid, void* lam,
-                             double* partial, hipStream_t s) {
-    fwd_front(g, b, x, scale, s);
-    fwd_out<1>(g, b, (T*)lam, (const T*)expo, counts, (T*)resid, partial, s,
-               "lp_fwd_out_poisson");
-    return launch_status("linpatch_forward_poisson");
-  }
-
-  static int forward_metric(const LinGeom& g, const LinBuffers& b, const void* dx, double scale,
-                            const void* expo, const int* counts, const void* lam, void* cot,
-                            hipStream_t s) {
-    fwd_front(g, b, dx, scale, s);
-    fwd_out<2>(g, b, (T*)lam, (const T*)expo, counts, (T*)cot, nullptr, s, "lp_fwd_out_metric");
-    return launch_status("linpatch_forward_metric");
-  }
-
-  static int adjoint(const LinGeom& g, const LinBuffers& b, const void* gy, void* gx,
-                     double scale, hipStream_t s) {
-    adj_front(g, b, gy, s);
-    adj_out<0>(g, b, g.x_shared ? g.E : g.J, (T*)gx, scale, nullptr, nullptr, nullptr, s,
-               "lp_adj_out");
-    return launch_status("linpatch_adjoint");
-  }
-
-  static int adjoint_gp(const LinGeom& g, const LinBuffers& b, const void* gy, double scale,
-                        const void* sky, const void* dnu, void* u, hipStream_t s) {
-    adj_front(g, b, gy, s);
-    adj_out<1>(g, b, g.E, nullptr, scale, (const T*)sky, (const T*)dnu, (C*)u, s,
-               "lp_adj_out_gp");
-    return launch_status("linpatch_adjoint_gp");
-  }
-};
-
-template <template <typename, int> class Ops, typename T, typename F>
-static int dispatch_len(int L, F&& f) {
-  switch (L) {
-#define JUBIK_CASE(l) \
-  case l:             \
-    return f(Ops<T, l>{});
-    JUBIK_LP_LENGTHS(JUBIK_CASE)
-#undef JUBIK_CASE
-    default:
-      return fail(kUnsupported, "FFT length " + std::to_string(L) + " is not compiled in");
-  }
-}
-
-template <typename F>
-static int dispatch(int dtype, int L, F&& f) {
-  if (dtype == 0) return dispatch_len<LinOps, float>(L, f);
-  if (dtype == 1) return dispatch_len<LinOps, double>(L, f);
-  return fail(kInvalidArgument, "dtype must be 0 (float32) or 1 (float64)");
-}
-
 int linpatch_prepare(int dtype, const LinGeom& g, const LinBuffers& b, const void* kernel,
                      int layout, hipStream_t s) {
-  return dispatch(dtype, g.L, [&](auto ops) { return decltype(ops)::prepare(g, b, kernel, layout, s); });
+  if (dtype == 0) return lp_prepare_t<float>(g, b, kernel, layout, s);
+  if (dtype == 1) return lp_prepare_t<double>(g, b, kernel, layout, s);
+  return fail(kInvalidArgument, "dtype must be 0 (float32) or 1 (float64)");
 }
 int linpatch_forward(int dtype, const LinGeom& g, const LinBuffers& b, const void* x, void* y,
                      double scale, hipStream_t s) {
-  return dispatch(dtype, g.L, [&](auto ops) { return decltype(ops)::forward(g, b, x, y, scale, s); });
+  if (dtype == 0) return lp_forward_t<float>(g, b, x, y, scale, s);
+  if (dtype == 1) return lp_forward_t<double>(g, b, x, y, scale, s);
+  return fail(kInvalidArgument, "dtype must be 0 (float32) or 1 (float64)");
 }
 int linpatch_forward_poisson(int dtype, const LinGeom& g, const LinBuffers& b, const void* x,
                              double scale, const void* expo, const int* counts, void* resid,
                              void* lam, double* partial, hipStream_t s) {
-  return dispatch(dtype, g.L, [&](auto ops) {
-    return decltype(ops)::forward_poisson(g, b, x, scale, expo, counts, resid, lam, partial, s);
-  });
+  if (dtype == 0)
+    return lp_forward_poisson_t<float>(g, b, x, scale, expo, counts, resid, lam, partial, s);
+  if (dtype == 1)
+    return lp_forward_poisson_t<double>(g, b, x, scale, expo, counts, resid, lam, partial, s);
+  return fail(kInvalidArgument, "dtype must be 0 (float32) or 1 (float64)");
 }
 int linpatch_forward_metric(int dtype, const LinGeom& g, const LinBuffers& b, const void* dx,
                             double scale, const void* expo, const int* counts, const void* lam,
                             void* cot, hipStream_t s) {
-  return dispatch(dtype, g.L, [&](auto ops) {
-    return decltype(ops)::forward_metric(g, b, dx, scale, expo, counts, lam, cot, s);
-  });
+  if (dtype == 0) return lp_forward_metric_t<float>(g, b, dx, scale, expo, counts, lam, cot, s);
+  if (dtype == 1) return lp_forward_metric_t<double>(g, b, dx, scale, expo, counts, lam, cot, s);
+  return fail(kInvalidArgument, "dtype must be 0 (float32) or 1 (float64)");
 }
 int linpatch_adjoint(int dtype, const LinGeom& g, const LinBuffers& b, const void* gy, void* gx,
                      double scale, hipStream_t s) {
-  return dispatch(dtype, g.L, [&](auto ops) { return decltype(ops)::adjoint(g, b, gy, gx, scale, s); });
+  if (dtype == 0) return lp_adjoint_t<float>(g, b, gy, gx, scale, s);
+  if (dtype == 1) return lp_adjoint_t<double>(g, b, gy, gx, scale, s);
+  return fail(kInvalidArgument, "dtype must be 0 (float32) or 1 (float64)");
 }
 int linpatch_adjoint_gp(int dtype, const LinGeom& g, const LinBuffers& b, const void* gy,
                         double scale, const void* sky, const void* dnu, void* u, hipStream_t s) {
-  return dispatch(dtype, g.L, [&](auto ops) {
-    return decltype(ops)::adjoint_gp(g, b, gy, scale, sky, dnu, u, s);
-  });
+  if (dtype == 0) return lp_adjoint_gp_t<float>(g, b, gy, scale, sky, dnu, u, s);
+  if (dtype == 1) return lp_adjoint_gp_t<double>(g, b, gy, scale, sky, dnu, u, s);
+  return fail(kInvalidArgument, "dtype must be 0 (float32) or 1 (float64)");
 }
 
 }  // namespace jubik

@@ -13,6 +13,10 @@ struct LinBuffers {
   void* R2;          // column-pass workspace, J x P x L x L2
 };
 
+// FFT lengths L = 2 (H / n) + 2 margin compiled into the library.
+#define JUBIK_LP_LENGTHS(X) \
+  X(16) X(24) X(32) X(48) X(64) X(96) X(128) X(192) X(224) X(256) X(324) X(384)
+
 bool linpatch_length_supported(int L);
 
 // dtype: 0 = float32, 1 = float64.  layout: 0 = full frame (H, W), 1 = cut
