@@ -10,7 +10,7 @@
 //
 // Tasks are mapped vector-fastest (vec = idx mod V), vectors sit in LDS at an
 // odd stride VS: lanes of a wave address different vectors at fixed position
-// offsets, which keeps every stage free of LDS bank conflicts.  Stages expose
+// offsets, which keeps the stages free of LDS bank conflicts when V >= 16.  Stages expose
 // gather / compute / scatter separately so kernels can read the first stage
 // straight from global memory, keep the last stage in registers (spectrum
 // multiply, inverse first stage) and write results straight back.
@@ -361,9 +361,9 @@ struct Stage {
     for (int b = 0; b < BPT; ++b) {
       int vec, j;
       if (task(tid, b, vec, j)) {
-        const cplx<T>* src = buf + vec * VS + j;
+        const cplx<T>* src = buf + vec * VS;
 #pragma unroll
-        for (int t = 0; t < R; ++t) v[b][t] = src[t * NB];
+        for (int t = 0; t < R; ++t) v[b][t] = src[j + t * NB];
       }
     }
   }
@@ -402,9 +402,10 @@ struct Stage {
       int vec, j;
       if (task(tid, b, vec, j)) {
         const int k = j % Ns;
-        cplx<T>* dst = buf + vec * VS + (j - k) * R + k;
+        cplx<T>* dst = buf + vec * VS;
+        const int p0 = (j - k) * R + k;
 #pragma unroll
-        for (int t = 0; t < R; ++t) dst[t * Ns] = v[b][t];
+        for (int t = 0; t < R; ++t) dst[p0 + t * Ns] = v[b][t];
       }
     }
   }

@@ -30,7 +30,8 @@ __global__ __launch_bounds__(GNT) void gp_cols(GpGeom g, const T* __restrict__ a
                                                const cplx<T>* __restrict__ twg,
                                                cplx<T>* __restrict__ Tw) {
   constexpr int V = Tile<T, N>::V;
-  __shared__ cplx<T> buf[V * N];
+  constexpr int VS = odd_stride(N);
+  __shared__ cplx<T> buf[V * VS];
   __shared__ cplx<T> tw[N];
   const int tid = threadIdx.x;
   const int c0 = blockIdx.x * V;
@@ -48,13 +49,13 @@ __global__ __launch_bounds__(GNT) void gp_cols(GpGeom g, const T* __restrict__ a
         for (int pl = 0; pl < g.uplanes; ++pl)
           val = cadd(val, u[((int64_t)pl * g.H + x1) * g.W + x2]);
     }
-    buf[v * N + x1] = val;
+    buf[v * VS + x1] = val;
   }
   __syncthreads();
-  fft_lds<T, N, V, N, GNT, false>(buf, tw, tid);
+  fft_lds<T, N, V, VS, GNT, false>(buf, tw, tid);
   for (int e = tid; e < V * N; e += GNT) {
     const int x1 = e / V, v = e - (e / V) * V;
-    Tw[(int64_t)x1 * N + c0 + v] = buf[v * N + x1];
+    Tw[(int64_t)x1 * N + c0 + v] = buf[v * VS + x1];
   }
 }
 
@@ -69,7 +70,8 @@ __global__ __launch_bounds__(GNT) void gp_rows(GpGeom g, const cplx<T>* __restri
                                                const T* __restrict__ amp_alpha,
                                                const T* __restrict__ sky_in,
                                                T* __restrict__ out_a, T* __restrict__ out_b) {
-  __shared__ cplx<T> buf[2 * N];
+  constexpr int VS = odd_stride(N);
+  __shared__ cplx<T> buf[2 * VS];
   __shared__ cplx<T> tw[N];
   const int tid = threadIdx.x;
   const int r0 = blockIdx.x;
@@ -77,16 +79,16 @@ __global__ __launch_bounds__(GNT) void gp_rows(GpGeom g, const cplx<T>* __restri
   load_twiddles(tw, twg, N, tid, GNT);
   for (int e = tid; e < 2 * N; e += GNT) {
     const int v = e / N, k = e - (e / N) * N;
-    buf[e] = Tw[(int64_t)(v ? r1 : r0) * N + k];
+    buf[v * VS + k] = Tw[(int64_t)(v ? r1 : r0) * N + k];
   }
   __syncthreads();
-  fft_lds<T, N, 2, N, GNT, false>(buf, tw, tid);
+  fft_lds<T, N, 2, VS, GNT, false>(buf, tw, tid);
   const int nrows = (r0 == r1) ? 1 : 2;
   for (int e = tid; e < nrows * N; e += GNT) {
     const int v = e / N, k = e - (e / N) * N;
     const int x = v ? r1 : r0;
-    const cplx<T> z = buf[v * N + k];
-    const cplx<T> zp = buf[(1 - v) * N + (k == 0 ? 0 : N - k)];
+    const cplx<T> z = buf[v * VS + k];
+    const cplx<T> zp = buf[(1 - v) * VS + (k == 0 ? 0 : N - k)];
     const T hta = T(0.5) * (z.re + zp.re + z.im - zp.im);
     const T htb = T(0.5) * (z.im + zp.im - z.re + zp.re);
     if constexpr (MODE == 1) {

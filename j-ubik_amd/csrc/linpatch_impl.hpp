@@ -67,6 +67,11 @@ __global__ __launch_bounds__(NT) void lp_rows(LinGeom g, const T* __restrict__ s
   }
 
   load_twiddles(tw, twg, L, tid, NT);
+  __shared__ T tent_s[L];  // w <= L tent weights
+  if constexpr (MODE == ROW_SIGNAL) {
+    for (int t = tid; t < g.w; t += NT) tent_s[t] = tent[t];
+    __syncthreads();
+  }
   constexpr int EPT = (V * L + NT - 1) / NT;
 #pragma unroll
   for (int u = 0; u < EPT; ++u) {
@@ -82,9 +87,9 @@ __global__ __launch_bounds__(NT) void lp_rows(LinGeom g, const T* __restrict__ s
           const int j = sy + c - g.h;
           const int i0 = sx + r0 - g.h;
           if (j >= 0 && j < g.W) {
-            const T wc = tent[c] * scale;
-            if (i0 >= 0 && i0 < g.H) a = tent[r0] * wc * img[i0 * g.W + j];
-            if (i0 + 1 >= 0 && i0 + 1 < g.H) b = tent[r0 + 1] * wc * img[(i0 + 1) * g.W + j];
+            const T wc = tent_s[c] * scale;
+            if (i0 >= 0 && i0 < g.H) a = tent_s[r0] * wc * img[i0 * g.W + j];
+            if (i0 + 1 >= 0 && i0 + 1 < g.H) b = tent_s[r0 + 1] * wc * img[(i0 + 1) * g.W + j];
           }
         }
       } else if constexpr (MODE == ROW_COTANGENT) {
@@ -473,12 +478,14 @@ __global__ __launch_bounds__(NT) void lp_adj_out(LinGeom g, const cplx<T>* __res
   __shared__ cplx<T> tw[L];
   __shared__ int lst_ix[MAXN], lst_r[MAXN];
   __shared__ T lst_w[MAXN];
+  __shared__ T tent_s[L];  // w <= L tent weights
   __shared__ int nc_s;
   const int tid = threadIdx.x;
   const int i = blockIdx.x;
   const int Rp = i + g.h;
 
   load_twiddles(tw, twg, L, tid, NT);
+  for (int t = tid; t < g.w; t += NT) tent_s[t] = tent[t];
   if (tid == 0) {
     int nc = 0;
     for (int ix = 0; ix < g.n; ++ix) {
@@ -565,7 +572,7 @@ __global__ __launch_bounds__(NT) void lp_adj_out(LinGeom g, const cplx<T>* __res
             const int v = (iy >> 1) - q0;
             if (c < g.w && v >= 0 && v < V) {
               const cplx<T> z = buf[v * VS + g.m + c];
-              s += tent[c] * ((iy & 1) ? z.im : z.re);
+              s += tent_s[c] * ((iy & 1) ? z.im : z.re);
             }
           }
           tmp[t] += s;

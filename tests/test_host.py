@@ -1,0 +1,134 @@
+"""Host-side logic without a GPU: the C ABI library loads and exports every
+symbol include/jubik_hip.h declares, arguments are validated before any
+device work (with the reference's exception types and messages), and the
+compute path refuses to run without a HIP device instead of falling back."""
+import ctypes
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+ROOT = Path(__file__).resolve().parent.parent
+HEADER = ROOT / "include" / "jubik_hip.h"
+
+
+def _declared():
+    text = HEADER.read_text()
+    return sorted(set(re.findall(r"\b(jubik_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_header_declares_the_entry_points():
+    names = _declared()
+    for must in ("jubik_linpatch_create", "jubik_linpatch_forward", "jubik_linpatch_adjoint",
+                 "jubik_poisson_nll_grad", "jubik_gp_forward", "jubik_gp_vjp",
+                 "jubik_model_grad", "jubik_model_metric", "jubik_last_error"):
+        assert must in names
+
+
+def test_library_exports_every_declared_symbol(ja):
+    from jubik_amd import _lib
+    lib = _lib.load()
+    for name in _declared():
+        assert hasattr(lib, name), name
+    assert set(_lib.SIGNATURES) | {"jubik_last_error"} >= set(_declared())
+    assert lib.jubik_abi_version() == 1
+
+
+def test_invalid_arguments_fail_before_device_work(ja):
+    from jubik_amd import _lib
+    lib = _lib.load()
+    plan = ctypes.c_void_p()
+    # margin 0
+    st = lib.jubik_linpatch_create(ctypes.byref(plan), 0, 64, 64, 1, 1, 4, 0, 1, 1)
+    assert st == _lib.EINVAL and b"margin" in lib.jubik_last_error()
+    # bad dtype
+    st = lib.jubik_linpatch_create(ctypes.byref(plan), 7, 64, 64, 1, 1, 4, 8, 1, 1)
+    assert st == _lib.EINVAL
+    # FFT length not compiled in: H=64, n=4, margin 9 -> L = 50
+    st = lib.jubik_linpatch_create(ctypes.byref(plan), 0, 64, 64, 1, 1, 4, 9, 1, 1)
+    assert st == _lib.EUNSUPPORTED and b"FFT length 50" in lib.jubik_last_error()
+    # unequal strides
+    st = lib.jubik_linpatch_create(ctypes.byref(plan), 0, 64, 96, 1, 1, 4, 8, 1, 1)
+    assert st == _lib.EINVAL
+    with pytest.raises(ValueError):
+        _lib.check(_lib.EINVAL)
+    with pytest.raises(NotImplementedError):
+        _lib.check(_lib.EUNSUPPORTED)
+    # NULL plan pointers
+    assert lib.jubik_linpatch_forward(None, None, None, 1.0, None) == _lib.EINVAL
+    assert lib.jubik_model_grad(None, None, None, None, None, 0.0, 0.0, None, None, None, None,
+                                None) == _lib.EINVAL
+
+
+def test_python_api_errors_match_reference(ja):
+    x = np.ones((1, 16, 16))
+    with pytest.raises(ValueError, match="domain has to be an instance of jubik.Domain"):
+        ja.linpatch_convolve(x, ((1, 16, 16), (1, 1, 1)), np.ones((16, 1, 16, 16)), 4, 2)
+    with pytest.raises(ValueError, match="exposure_cut should be non-negative or None!"):
+        ja.build_exposure_function(np.ones((1, 4, 4)), -1)
+    with pytest.raises(ValueError, match="threshold should be positive!"):
+        ja.build_readout_function(np.ones((1, 1, 4, 4)), -1)
+    with pytest.raises(ValueError, match="length of keys should match"):
+        ja.build_readout_function(np.ones((2, 1, 4, 4)), None, ("a",))
+    with pytest.raises(ValueError, match="flags should have shape"):
+        ja.build_readout_function(np.ones((1, 4, 4)), None, ("a",))(np.ones((1, 1, 4, 4)))
+    with pytest.raises(ValueError, match="input should have shape"):
+        ja.build_readout_function(np.ones((1, 1, 4, 4)), None)(np.ones((1, 4, 4)))
+
+
+def test_builders_mutate_callers_arrays_like_reference(ja):
+    e = np.array([[[[1.0, 5.0], [30.0, 2.0]]]])
+    ja.build_exposure_function(e, 3.0)
+    np.testing.assert_array_equal(e, [[[[0.0, 5.0], [30.0, 0.0]]]])
+    f = np.array([[[[1.0, 5.0], [30.0, 2.0]]]])
+    ro = ja.build_readout_function(f, 3.0)
+    np.testing.assert_array_equal(f, [[[[0.0, 5.0], [30.0, 0.0]]]])
+    assert ro.keys == ("masked input",)
+
+
+def test_domain_duck_typing(ja):
+    from typing import NamedTuple
+
+    from jubik_amd.data import is_domain
+
+    class Domain(NamedTuple):
+        shape: tuple
+        distances: tuple
+
+    assert is_domain(ja.Domain((1,), (1.0,)))
+    assert is_domain(Domain((1,), (1.0,)))
+    assert not is_domain(((1,), (1.0,)))
+
+
+def test_dense_counts_wire_format(ja):
+    keep = np.zeros((1, 1, 2, 3), dtype=bool)
+    keep[0, 0, 0, 1] = keep[0, 0, 1, 2] = True
+    out = ja.dense_counts([np.array([4, 7])], keep)
+    np.testing.assert_array_equal(out, [[[[-1, 4, -1], [-1, -1, 7]]]])
+    with pytest.raises(ValueError):
+        ja.dense_counts([np.array([4])], keep)
+    with pytest.raises(ValueError):
+        ja.dense_counts([np.array([4, -1])], keep)
+
+
+def test_margin_zero_is_empty_without_device(ja):
+    x = np.ones((1, 16, 16))
+    out = ja.linpatch_convolve(x, ja.Domain((1, 16, 16), (1, 1, 1)), np.ones((16, 1, 16, 16)),
+                               4, 0)
+    assert out.shape == (1, 0, 0)
+
+
+def test_compute_path_requires_a_device(ja):
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a HIP device is visible")
+    x = np.ones((1, 16, 16))
+    with pytest.raises(RuntimeError, match="no HIP\\s+device"):
+        ja.linpatch_convolve(x, ja.Domain((1, 16, 16), (1, 1, 1)), np.ones((16, 1, 16, 16)), 4, 2)
+    with pytest.raises(RuntimeError):
+        ja.build_exposure_function(np.ones((1, 4, 4)))(np.ones((4, 4)))
+
+
+def test_integrate_matches_reference_fixture(ja):
+    assert ja.integrate(np.ones((3, 4)), ja.Domain((3, 4), (0.5, 0.25)), (0, 1)) == 1.5
