@@ -136,6 +136,22 @@ def kernel_bytes(H, E, n, m, M=1):
     }
 
 
+TRAFFIC_FILE = ROOT / "profiles" / "grad_eval_1024x4E_L384_hbm_traffic.txt"
+
+
+def measured_traffic(kernel, H, E, n, m):
+    """HBM bytes per dispatch of ``kernel`` from the committed rocprofv3 PMC
+    summary (FETCH_SIZE and WRITE_SIZE, scaled by a known-size copy), for the
+    default workload only; None otherwise."""
+    if (H, E, n, m) != (1024, 4, 8, 64) or not TRAFFIC_FILE.exists():
+        return None
+    for line in TRAFFIC_FILE.read_text().splitlines():
+        parts = line.split()
+        if parts and not line.startswith("#") and parts[0] == kernel:
+            return float(parts[-1])
+    return None
+
+
 def eval_bytes(H, E, n, m, M=1):
     """B_alg of one gradient evaluation: 4 B x [2 |xi| + M E P w^2 + 2 M E H W]."""
     w = 2 * (H // n)
@@ -305,7 +321,8 @@ def main():
                                "exposure 5e2-2e3 s, 5% mask, Poisson NLL (FP64 reduction)",
                    "precision": "FP32 FFTs/PSF products/exp, FP64 NLL reductions"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK, "traffic": None, "kernel": dom,
+                     "frac": achieved / HBM_PEAK,
+                     "traffic": measured_traffic(dom, H, E, n, m), "kernel": dom,
                      "kernel_avg_us": dom_avg * 1e6, "bytes_per_launch": kb.get(dom, 0)},
         "eval_roofline": {"B_alg": eval_bytes(H, E, n, m),
                           "achieved": eval_bytes(H, E, n, m) / t_mixed,
