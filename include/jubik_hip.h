@@ -83,6 +83,23 @@ int jubik_linpatch_forward(jubik_linpatch* plan, const void* x, void* y, double 
 int jubik_linpatch_adjoint(jubik_linpatch* plan, const void* gy, void* gx, double scale,
                            void* stream);
 
+/* ---------------------------------------------------------------- primitives
+ * slice_patches of jubik/convolve.py:28-63 on x (B, H, W): out (n^2, B,
+ * 2dx+2m, 2dy+2m), dx = (H - 2 margin) / n, patch p = iy * n + ix, window
+ * starts clamped as jax.lax.dynamic_slice does.  The adjoint is its
+ * transpose (the jax.linear_transpose used at jubik/convolve.py:138-143). */
+int jubik_slice_patches(int dtype, const void* x, int64_t B, int64_t H, int64_t W, int64_t n,
+                        int64_t margin, void* out, void* stream);
+int jubik_slice_patches_adjoint(int dtype, const void* patches, int64_t B, int64_t H, int64_t W,
+                                int64_t n, int64_t margin, void* x, void* stream);
+/* convolve of jubik/convolve.py:148-176 over the last two axes of real
+ * N x N arrays: out[b] = dvol * Re(IFFT2(FFT2(x[b / (nby / nbx)]) *
+ * FFT2(y[b]))).  ``work`` holds jubik_convolve_work_bytes(dtype, nby, N)
+ * bytes of device memory. */
+int64_t jubik_convolve_work_bytes(int dtype, int64_t nby, int64_t N);
+int jubik_convolve(int dtype, const void* x, const void* y, int64_t nbx, int64_t nby, int64_t N,
+                   double dvol, void* work, void* out, void* stream);
+
 /* ---------------------------------------------------------------- response
  * Exposure closure of jubik/response.py:12-45 (``exposures * x`` with
  * numpy broadcasting of operands whose shapes are suffixes of the output:

@@ -7,15 +7,16 @@ array work runs in libjubik_hip.so (hand-written HIP kernels behind a C ABI,
 include/jubik_hip.h); there is no CPU fallback.
 """
 from .data import Domain, Vector
-from .convolve import (LinpatchOperator, integrate, linpatch_convolve,
-                       linpatch_convolve_modules)
+from .convolve import (LinpatchOperator, convolve, integrate, linpatch_convolve,
+                       linpatch_convolve_modules, slice_patches, slice_patches_transpose)
 from .response import build_exposure_function, build_readout_function
 from .likelihood import Poissonian
 from .sky import CorrelatedFieldSky, powerlaw_amplitude
 from .model import XrayModel, dense_counts
 
 __all__ = [
-    "Domain", "Vector", "LinpatchOperator", "integrate", "linpatch_convolve",
+    "Domain", "Vector", "LinpatchOperator", "convolve", "integrate", "linpatch_convolve",
+    "slice_patches", "slice_patches_transpose",
     "linpatch_convolve_modules", "build_exposure_function", "build_readout_function",
     "Poissonian", "CorrelatedFieldSky", "powerlaw_amplitude", "XrayModel", "dense_counts",
 ]

@@ -199,6 +199,91 @@ def linpatch_convolve_modules(x, domain, kernels, n_patches_per_axis, margin):
     return _linpatch(x, domain, kernels, n_patches_per_axis, margin, modules=True)
 
 
+def slice_patches(x, shape, n_patches_per_axis, additional_margin):
+    """Overlapping windows of x, jubik/convolve.py:28-63, on the GPU.
+
+    Output (n^2, *x.shape[:-2], 2dx + 2m, 2dy + 2m) with dx = (H - 2m) // n;
+    patch p = iy * n + ix starts at (ix * dx, iy * dy) of x zero-padded by
+    dx // 2 (starts clamped as jax.lax.dynamic_slice does).
+    """
+    n, m = int(n_patches_per_axis), int(additional_margin)
+    H, W = int(shape[-2]), int(shape[-1])
+    if tuple(x.shape[-2:]) != (H, W):
+        raise ValueError(f"input of shape {tuple(x.shape)} does not match shape {tuple(shape)}")
+    dev.require_device()
+    dtype = dev.float_dtype(x)
+    xt = dev.to_device(x, dtype)
+    lead = tuple(xt.shape[:-2])
+    B = int(np.prod(lead)) if lead else 1
+    dx, dy = int((H - 2 * m) / n), int((W - 2 * m) / n)
+    out = torch.empty((n * n,) + lead + (2 * dx + 2 * m, 2 * dy + 2 * m), dtype=dtype,
+                      device="cuda")
+    _lib.call("jubik_slice_patches", dev.code(dtype), dev.ptr(xt), B, H, W, n, m, dev.ptr(out),
+              dev.stream())
+    return dev.like_input(out, x)
+
+
+def slice_patches_transpose(patches, shape, n_patches_per_axis, additional_margin):
+    """Transpose of slice_patches (what jax.linear_transpose gives,
+    jubik/convolve.py:138-143): sums the windows back onto (*lead, H, W)."""
+    n, m = int(n_patches_per_axis), int(additional_margin)
+    H, W = int(shape[-2]), int(shape[-1])
+    dev.require_device()
+    dtype = dev.float_dtype(patches)
+    pt = dev.to_device(patches, dtype)
+    lead = tuple(pt.shape[1:-2])
+    B = int(np.prod(lead)) if lead else 1
+    out = torch.empty(lead + (H, W), dtype=dtype, device="cuda")
+    _lib.call("jubik_slice_patches_adjoint", dev.code(dtype), dev.ptr(pt), B, H, W, n, m,
+              dev.ptr(out), dev.stream())
+    return dev.like_input(out, patches)
+
+
+def convolve(x, y, domain, axes):
+    """FFT convolution over the last two axes, jubik/convolve.py:148-176.
+
+    ``x`` is the kernel, ``y`` the signal; returns dvol * Re(ifft(fft(x)
+    fft(y))).  When y has one more axis than x, x is broadcast over it and
+    the reference's notice is printed.  Square transforms of the compiled
+    sizes (4 ... 1024) run on the GPU; others raise NotImplementedError.
+    """
+    nd_x, nd_y = len(x.shape), len(y.shape)
+    ax = tuple(a % nd_y for a in axes)
+    if ax != (nd_y - 2, nd_y - 1):
+        raise NotImplementedError("convolve runs over the last two axes only")
+    dvol = float(math.prod(domain.distances[a] for a in axes))
+    if nd_y > nd_x:
+        print("kernel_shape:", tuple(x.shape))
+        print("signal_shape:", tuple(y.shape))
+        print("Dimension Inconsistency. Broadcasting PSFs")
+    N = int(y.shape[-1])
+    if int(y.shape[-2]) != N or tuple(x.shape[-2:]) != (N, N):
+        raise NotImplementedError("convolve needs square arrays of equal size")
+    dev.require_device()
+    dtype = dev.float_dtype(x, y)
+    xt, yt = dev.to_device(x, dtype), dev.to_device(y, dtype)
+    if nd_y > nd_x:
+        if tuple(yt.shape[:-3]) != tuple(xt.shape[:-2]):
+            xt = xt.unsqueeze(-3).expand(yt.shape[:-3] + (1, N, N)).contiguous()
+            nbx = int(np.prod(yt.shape[:-3])) if yt.dim() > 3 else 1
+        else:
+            nbx = int(np.prod(xt.shape[:-2])) if xt.dim() > 2 else 1
+    else:
+        shape = torch.broadcast_shapes(tuple(xt.shape), tuple(yt.shape))
+        if tuple(xt.shape) != tuple(shape):
+            xt = xt.expand(shape).contiguous()
+        if tuple(yt.shape) != tuple(shape):
+            yt = yt.expand(shape).contiguous()
+        nbx = int(np.prod(shape[:-2])) if len(shape) > 2 else 1
+    nby = int(np.prod(yt.shape[:-2])) if yt.dim() > 2 else 1
+    out = torch.empty(tuple(yt.shape), dtype=dtype, device="cuda")
+    work = torch.empty(int(_lib.load().jubik_convolve_work_bytes(dev.code(dtype), nby, N)),
+                       dtype=torch.uint8, device="cuda")
+    _lib.call("jubik_convolve", dev.code(dtype), dev.ptr(xt), dev.ptr(yt), nbx, nby, N, dvol,
+              dev.ptr(work), dev.ptr(out), dev.stream())
+    return dev.like_input(out, y)
+
+
 def integrate(x, domain, axes):
     """dvol * sum(x) over ``axes`` (jubik/convolve.py:179-184).
 

@@ -11,6 +11,7 @@
 #include "../../include/jubik_hip.h"
 #include "gp.hpp"
 #include "linpatch.hpp"
+#include "primitives.hpp"
 #include "response.hpp"
 
 namespace jubik {
@@ -372,6 +373,51 @@ int jubik_linpatch_adjoint(jubik_linpatch* plan, const void* gy, void* gx, doubl
   if (!plan->ready) return fail(kNotReady, "no kernel set (call jubik_linpatch_set_kernel)");
   DeviceGuard dg(plan->device);
   return linpatch_adjoint(plan->dtype, plan->g, plan->bufs(), gy, gx, scale, as_stream(stream));
+}
+
+// ------------------------------------------------------------------ primitives
+int jubik_slice_patches(int dtype, const void* x, int64_t B, int64_t H, int64_t W, int64_t n,
+                        int64_t margin, void* out, void* stream) {
+  if (dtype != 0 && dtype != 1) return fail(kInvalidArgument, "dtype must be 0 or 1");
+  if (!x || !out) return fail(kInvalidArgument, "NULL argument");
+  return slice_patches(dtype, x, B, H, W, n, margin, out, as_stream(stream));
+}
+
+int jubik_slice_patches_adjoint(int dtype, const void* patches, int64_t B, int64_t H, int64_t W,
+                                int64_t n, int64_t margin, void* x, void* stream) {
+  if (dtype != 0 && dtype != 1) return fail(kInvalidArgument, "dtype must be 0 or 1");
+  if (!patches || !x) return fail(kInvalidArgument, "NULL argument");
+  return slice_patches_adjoint(dtype, patches, B, H, W, n, margin, x, as_stream(stream));
+}
+
+int64_t jubik_convolve_work_bytes(int dtype, int64_t nby, int64_t N) {
+  return (nby * N * N + N) * 2 * (int64_t)elem_size(dtype);
+}
+
+int jubik_convolve(int dtype, const void* x, const void* y, int64_t nbx, int64_t nby, int64_t N,
+                   double dvol, void* work, void* out, void* stream) {
+  if (dtype != 0 && dtype != 1) return fail(kInvalidArgument, "dtype must be 0 or 1");
+  if (!x || !y || !work || !out) return fail(kInvalidArgument, "NULL argument");
+  if (!convolve_length_supported((int)N))
+    return fail(kUnsupported, "convolve size " + std::to_string(N) + " is not compiled in");
+  // twiddles go behind the complex work planes
+  const size_t es = elem_size(dtype);
+  static thread_local std::vector<unsigned char> host;
+  host.resize(2 * es * N);
+  for (int64_t e = 0; e < N; ++e) {
+    const long double ang = -2.0L * 3.141592653589793238462643383279502884L * e / N;
+    const double c = (double)cosl(ang), sn = (double)sinl(ang);
+    if (dtype == 0) {
+      reinterpret_cast<float*>(host.data())[2 * e] = (float)c;
+      reinterpret_cast<float*>(host.data())[2 * e + 1] = (float)sn;
+    } else {
+      reinterpret_cast<double*>(host.data())[2 * e] = c;
+      reinterpret_cast<double*>(host.data())[2 * e + 1] = sn;
+    }
+  }
+  char* tw = reinterpret_cast<char*>(work) + (size_t)nby * N * N * 2 * es;
+  JUBIK_HIP(hipMemcpyAsync(tw, host.data(), host.size(), hipMemcpyHostToDevice, as_stream(stream)));
+  return convolve2d(dtype, x, y, nbx, nby, N, dvol, tw, work, out, as_stream(stream));
 }
 
 // ------------------------------------------------------------------ response

@@ -109,6 +109,15 @@ struct RowTile {
   static constexpr int V = tile_vectors(26624, L, (int)sizeof(cplx<T>));
 };
 
+// Tile of work-group b out of nb: the dispatcher places work-group b on XCD
+// b mod 8, so consecutive tiles are given to the work-groups of one XCD and
+// neighbouring tiles that share cache lines meet in that XCD's L2.
+__device__ __forceinline__ int xcd_tile(int b, int nb) {
+  constexpr int kXcds = 8;
+  if (nb % kXcds) return b;
+  return (b % kXcds) * (nb / kXcds) + b / kXcds;
+}
+
 // Block-wide sum of one double per thread (NT a multiple of 64).
 template <int NT>
 __device__ double block_sum(double v, double* red) {

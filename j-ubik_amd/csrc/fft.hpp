@@ -298,6 +298,9 @@ struct RegDft<T, N, INV, false> {
 __host__ __device__ constexpr int plan_radix(int L, int s) {
   int r0 = 0, r1 = 0, r2 = 0;
   switch (L) {
+    case 2: r0 = 2; break;
+    case 4: r0 = 4; break;
+    case 8: r0 = 8; break;
     case 16: r0 = 16; break;
     case 24: r0 = 24; break;
     case 32: r0 = 32; break;

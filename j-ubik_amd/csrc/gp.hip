@@ -34,7 +34,7 @@ __global__ __launch_bounds__(GNT) void gp_cols(GpGeom g, const T* __restrict__ a
   __shared__ cplx<T> buf[V * VS];
   __shared__ cplx<T> tw[N];
   const int tid = threadIdx.x;
-  const int c0 = blockIdx.x * V;
+  const int c0 = xcd_tile(blockIdx.x, gridDim.x) * V;
   load_twiddles(tw, twg, N, tid, GNT);
   for (int e = tid; e < V * N; e += GNT) {
     const int x1 = e / V, v = e - (e / V) * V;

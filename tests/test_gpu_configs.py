@@ -1,0 +1,126 @@
+"""The geometries of the benchmark configurations on the GPU.
+
+C3: 7 modules x 3 energies, n = 10, margin 60 (L = 324 = 2^2 3^4).
+C4: three observations on 256^2 / 512^2 / 768^2 grids with n = 6 / 8 / 12
+    and margins 22 / 32 / 48 (L = 128 / 192 / 224) and metric products.
+C5: 2048^2 x 4 energies, n = 16, margin 64 (L = 384).
+Small instances are checked against the float64 oracle; the full sizes
+through FP32 vs FP64 kernels, adjointness and linearity.
+"""
+import numpy as np
+import pytest
+import torch
+
+from oracle import convolve as cv
+from oracle import model as om
+from oracle.data import Domain as ODomain
+from tests.problems import make_problem, psf_windows, to_product
+
+pytestmark = pytest.mark.gpu
+
+
+def _rel(a, b):
+    return float(np.linalg.norm(np.ravel(a) - np.ravel(b)) / np.linalg.norm(np.ravel(b)))
+
+
+@pytest.mark.parametrize("dtype,tol", [(torch.float64, 1e-12), (torch.float32, 2e-6)])
+def test_c3_geometry_L324_modules_vs_oracle(ja, dtype, tol):
+    """L = 324 on a 303^2 grid (n = 3, stride 101, margin 61): the last
+    window start 202 is clamped to H + 2h - w = 201; 2 modules x 3 energies."""
+    H, n, m, E, M = 303, 3, 61, 3, 2
+    rng = np.random.default_rng(31)
+    x = rng.random((E, H, H))
+    wins = np.stack([psf_windows(rng, n, E, 2 * (H // n), sigma=(2, 6),
+                                 rotation=k * np.pi / 7) for k in range(M)])
+    dom = ODomain((E, H, H), (1, 1, 1))
+    op = ja.LinpatchOperator((E, H, H), n, m, n_modules=M, dtype=dtype)
+    assert op.L == 324
+    op.set_kernel(wins, layout="window")
+    got = op.forward(torch.as_tensor(x, device="cuda")).double().cpu().numpy()
+    want = np.stack([cv.linpatch_convolve_window(x, dom, wins[k], n, m) for k in range(M)])
+    assert _rel(got, want) <= tol
+    g = rng.normal(size=(M, E, H, H))
+    got = op.adjoint(torch.as_tensor(g, device="cuda")).double().cpu().numpy()
+    want = sum(cv.linpatch_adjoint_window(g[k], dom, wins[k], n, m) for k in range(M))
+    assert _rel(got, want) <= tol
+
+
+def test_c3_full_size_adjointness(ja):
+    """1024^2 x 3 energies, 7 modules, n = 10, margin 60 (FP32)."""
+    H, n, m, E, M = 1024, 10, 60, 3, 7
+    rng = np.random.default_rng(3)
+    wins = np.stack([psf_windows(rng, n, E, 2 * (H // n), sigma=(2, 6),
+                                 rotation=k * np.pi / 7) for k in range(M)])
+    op = ja.LinpatchOperator((E, H, H), n, m, n_modules=M, dtype=torch.float32)
+    op.set_kernel(wins, layout="window")
+    x = torch.as_tensor(rng.random((E, H, H)), dtype=torch.float32, device="cuda")
+    y = torch.as_tensor(rng.random((M, E, H, H)), dtype=torch.float32, device="cuda")
+    lhs = float((y.double() * op.forward(x).double()).sum())
+    rhs = float((op.adjoint(y).double() * x.double()).sum())
+    assert abs(lhs - rhs) <= 1e-5 * abs(lhs)
+
+
+@pytest.mark.parametrize("H,n,m,L", [(256, 6, 22, 128), (512, 8, 32, 192), (768, 12, 48, 224)])
+def test_c4_grids_vs_oracle(ja, H, n, m, L):
+    rng = np.random.default_rng(H)
+    x = rng.random((1, H, H))
+    win = psf_windows(rng, n, 1, 2 * (H // n))
+    dom = ODomain((1, H, H), (1, 1, 1))
+    want = cv.linpatch_convolve_window(x, dom, win, n, m)
+    for dtype, tol in ((torch.float64, 1e-12), (torch.float32, 2e-6)):
+        op = ja.LinpatchOperator((1, H, H), n, m, dtype=dtype)
+        assert op.L == L
+        op.set_kernel(win, layout="window")
+        got = op.forward(torch.as_tensor(x, device="cuda")).double().cpu().numpy()
+        assert _rel(got, want) <= tol
+
+
+def test_c4_metric_products_vs_oracle(ja):
+    """Metric products J^T diag(1/lam) J v for several antithetic samples on
+    the 256^2 observation (n = 6, margin 22)."""
+    prob, xi = make_problem(H=256, n=6, margin=22, E=1, M=1, seed=41, N=256)
+    model = to_product(ja, prob, torch.float32)
+    model.nll_and_grad(torch.as_tensor(xi[0], device="cuda"))
+    rng = np.random.default_rng(4)
+    for s in range(2):
+        v = rng.normal(size=xi[0].shape)
+        for sign in (1.0, -1.0):
+            want, _ = om.metric_product(prob, xi[0], None, sign * v)
+            got, _ = model.metric(torch.as_tensor(sign * v, device="cuda"))
+            assert _rel(got.double().cpu().numpy(), want) <= 1e-5
+
+
+def test_c5_full_size_fp32_vs_fp64(ja):
+    """2048^2 x 4 energies, n = 16, margin 64 (1024 patch-energies)."""
+    H, n, m, E = 2048, 16, 64, 4
+    rng = np.random.default_rng(5)
+    win = psf_windows(rng, n, E, 2 * (H // n))
+    x = rng.random((E, H, H))
+    g = rng.normal(size=(E, H, H))
+    out = {}
+    for dtype in (torch.float64, torch.float32):
+        op = ja.LinpatchOperator((E, H, H), n, m, dtype=dtype)
+        op.set_kernel(win, layout="window")
+        y = op.forward(torch.as_tensor(x, device="cuda")).double().cpu().numpy()
+        gx = op.adjoint(torch.as_tensor(g, device="cuda")).double().cpu().numpy()
+        out[dtype] = (y, gx)
+        del op
+        torch.cuda.empty_cache()
+    assert _rel(out[torch.float32][0], out[torch.float64][0]) <= 2e-6
+    assert _rel(out[torch.float32][1], out[torch.float64][1]) <= 2e-6
+
+
+def test_c5_gp_2048_vs_oracle(ja):
+    """The 2048^2 spectral GP (FP32) against the oracle."""
+    from oracle import sky as osky
+    from tests.problems import powerlaw_amp
+    N, E = 2048, 4
+    rng = np.random.default_rng(6)
+    amp = powerlaw_amp(N)
+    dnu = np.log(np.array([1.0, 1.6, 2.5, 4.0]))
+    xr, xa = rng.normal(size=(N, N)), rng.normal(size=(N, N))
+    want = osky.gp_sky(xr, xa, amp, 0.5 * amp, -2.0, -1.0, dnu)
+    sky = ja.CorrelatedFieldSky(N, (N, N), amp, 0.5 * amp, zm=-2.0, mu=-1.0, dnu=dnu,
+                                dtype=torch.float32)
+    got = sky(torch.as_tensor(xr, device="cuda"), torch.as_tensor(xa, device="cuda"))
+    assert _rel(got.double().cpu().numpy(), want) <= 2e-6
