@@ -157,6 +157,13 @@ int jubik_gp_vjp(jubik_gp* plan, const void* amp_ref, const void* amp_alpha, con
 typedef struct jubik_model jubik_model;
 
 int jubik_model_create(jubik_model** model, jubik_gp* gp, jubik_linpatch* lp, double pixel_area);
+/* A model that evaluates up to max_samples positions per call: the VI
+ * samples of nifty.re's optimize_kl (demos/x-ray-imaging.py:404-430, sample
+ * counts from jubik/minimization_parser.py:240-287), at each of which the
+ * likelihood gradient and metric are evaluated.  It owns workspaces for
+ * max_samples sets of jobs. */
+int jubik_model_create_batched(jubik_model** model, jubik_gp* gp, jubik_linpatch* lp,
+                               double pixel_area, int64_t max_samples);
 int jubik_model_destroy(jubik_model* model);
 int jubik_model_set_data(jubik_model* model, const void* exposure, const int32_t* counts);
 /* One gradient evaluation: nll (device double) and the gradient with
@@ -171,6 +178,18 @@ int jubik_model_grad(jubik_model* model, const void* amp_ref, const void* amp_al
 int jubik_model_metric(jubik_model* model, const void* amp_ref, const void* amp_alpha,
                        const void* dnu, const void* v_ref, const void* v_alpha, void* out_ref,
                        void* out_alpha, void* stream);
+/* S gradient evaluations in one pass: xi_ref, xi_alpha, g_ref, g_alpha are
+ * (S, N, N) stacks, nll holds S device doubles.  The metric base point is
+ * not changed. */
+int jubik_model_grad_batch(jubik_model* model, int64_t S, const void* amp_ref,
+                           const void* amp_alpha, const void* xi_ref, const void* xi_alpha,
+                           double zm, double mu, const void* dnu, double* nll, void* g_ref,
+                           void* g_alpha, void* stream);
+/* S metric products at the base point in one pass: v and out are (S, N, N)
+ * stacks (the S conjugate-gradient systems of an MGVI sample draw). */
+int jubik_model_metric_batch(jubik_model* model, int64_t S, const void* amp_ref,
+                             const void* amp_alpha, const void* dnu, const void* v_ref,
+                             const void* v_alpha, void* out_ref, void* out_alpha, void* stream);
 
 #ifdef __cplusplus
 }

@@ -168,18 +168,26 @@ struct jubik_gp {
   }
 };
 
+// A model owns the per-evaluation buffers.  Slot 0 of sky / lam holds the
+// base point of the metric (the last jubik_model_grad); batched gradients
+// use slots 1 .. S.  With max_samples > 1 the model also owns linpatch and GP
+// workspaces for S jobs sets; with max_samples = 1 it borrows the plans'.
 struct jubik_model {
   jubik_gp* gp = nullptr;
   jubik_linpatch* lp = nullptr;
   double pixel_area = 1.0;
+  int max_samples = 1;
   const void* expo = nullptr;
   const int32_t* counts = nullptr;
-  void* sky = nullptr;
-  void* dsky = nullptr;
-  void* resid = nullptr;
-  void* lam = nullptr;
-  void* partial = nullptr;
-  void* u = nullptr;
+  void* sky = nullptr;      // (1 + max_samples) x E x H x W
+  void* lam = nullptr;      // (1 + max_samples) x J x H x W
+  void* dsky = nullptr;     // max_samples x E x H x W
+  void* resid = nullptr;    // max_samples x J x H x W
+  void* partial = nullptr;  // max_samples x J x H doubles
+  void* u = nullptr;        // max_samples x E x H x W complex
+  void* R1 = nullptr;       // own linpatch workspaces (max_samples > 1)
+  void* R2 = nullptr;
+  void* gwork = nullptr;    // own GP workspace (max_samples > 1)
   bool have_point = false;
   void release() {
     dev_free(sky);
@@ -188,7 +196,26 @@ struct jubik_model {
     dev_free(lam);
     dev_free(partial);
     dev_free(u);
+    dev_free(R1);
+    dev_free(R2);
+    dev_free(gwork);
   }
+  LinBuffers lbufs() const {
+    LinBuffers b = lp->bufs();
+    if (R1) b.R1 = R1;
+    if (R2) b.R2 = R2;
+    return b;
+  }
+  GpBuffers gbufs() const {
+    GpBuffers b = gp->bufs();
+    if (gwork) b.work = gwork;
+    return b;
+  }
+  size_t img_bytes() const {
+    return (size_t)lp->g.H * lp->g.W * (lp->dtype == 0 ? sizeof(float) : sizeof(double));
+  }
+  void* sky_slot(int k) const { return (char*)sky + (size_t)k * lp->g.E * img_bytes(); }
+  void* lam_slot(int k) const { return (char*)lam + (size_t)k * lp->g.J * img_bytes(); }
 };
 
 static hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
@@ -284,6 +311,8 @@ int jubik_linpatch_create(jubik_linpatch** plan, int dtype, int64_t H, int64_t W
   g.J = g.M * g.E;
   g.x_shared = x_shared ? 1 : 0;
   g.Ek = kernel_per_energy ? g.E : 1;
+  g.S = 1;
+  g.sky_shared = 1;
   if (g.limx < 0 || g.limy < 0)
     return fail(kInvalidArgument, "patch window of " + std::to_string(g.w) +
                                       " does not fit the padded grid");
@@ -470,7 +499,7 @@ int jubik_gp_create(jubik_gp** plan, int dtype, int64_t N, int64_t H, int64_t W,
     return fail(kInvalidArgument, "several energies need the spectral-index field");
   auto* p = new jubik_gp();
   p->dtype = dtype;
-  p->g = GpGeom{(int)N, (int)H, (int)W, (int)E, has_alpha ? 1 : 0, 0.0, 0.0, 1};
+  p->g = GpGeom{(int)N, (int)H, (int)W, (int)E, has_alpha ? 1 : 0, 0.0, 0.0, 1, 1, 1};
   if (hipGetDevice(&p->device) != hipSuccess) {
     delete p;
     return fail(kRuntimeError, "no HIP device available");
@@ -524,9 +553,12 @@ int jubik_gp_vjp(jubik_gp* plan, const void* amp_ref, const void* amp_alpha, con
 }
 
 // ------------------------------------------------------------------ model
-int jubik_model_create(jubik_model** model, jubik_gp* gp, jubik_linpatch* lp, double pixel_area) {
+int jubik_model_create_batched(jubik_model** model, jubik_gp* gp, jubik_linpatch* lp,
+                               double pixel_area, int64_t max_samples) {
   if (!model || !gp || !lp) return fail(kInvalidArgument, "NULL argument");
   *model = nullptr;
+  if (max_samples < 1 || max_samples > 4096)
+    return fail(kInvalidArgument, "max_samples must be in [1, 4096]");
   if (gp->dtype != lp->dtype) return fail(kInvalidArgument, "gp and linpatch dtypes differ");
   if (gp->device != lp->device) return fail(kInvalidArgument, "gp and linpatch devices differ");
   const LinGeom& g = lp->g;
@@ -537,15 +569,22 @@ int jubik_model_create(jubik_model** model, jubik_gp* gp, jubik_linpatch* lp, do
   m->gp = gp;
   m->lp = lp;
   m->pixel_area = pixel_area;
+  m->max_samples = (int)max_samples;
   DeviceGuard dg(lp->device);
+  const size_t S = (size_t)max_samples;
   const size_t es = elem_size(lp->dtype);
   const size_t img = (size_t)g.H * g.W;
-  int st = dev_alloc(&m->sky, img * g.E * es, "sky");
-  if (!st) st = dev_alloc(&m->dsky, img * g.E * es, "sky tangent");
-  if (!st) st = dev_alloc(&m->resid, img * g.J * es, "residual");
-  if (!st) st = dev_alloc(&m->lam, img * g.J * es, "lambda");
-  if (!st) st = dev_alloc(&m->partial, (size_t)g.J * g.H * sizeof(double), "NLL partials");
-  if (!st) st = dev_alloc(&m->u, img * g.E * 2 * es, "GP cotangent planes");
+  int st = dev_alloc(&m->sky, img * g.E * (S + 1) * es, "sky");
+  if (!st) st = dev_alloc(&m->lam, img * g.J * (S + 1) * es, "lambda");
+  if (!st) st = dev_alloc(&m->dsky, img * g.E * S * es, "sky tangent");
+  if (!st) st = dev_alloc(&m->resid, img * g.J * S * es, "residual");
+  if (!st) st = dev_alloc(&m->partial, (size_t)g.J * g.H * S * sizeof(double), "NLL partials");
+  if (!st) st = dev_alloc(&m->u, img * g.E * 2 * S * es, "GP cotangent planes");
+  if (!st && S > 1) {
+    st = dev_alloc(&m->R1, lp->ws_bytes * S, "batched workspace R1");
+    if (!st) st = dev_alloc(&m->R2, lp->ws_bytes * S, "batched workspace R2");
+    if (!st) st = dev_alloc(&m->gwork, (size_t)gp->g.N * gp->g.N * 2 * es * S, "batched GP workspace");
+  }
   if (st) {
     m->release();
     delete m;
@@ -553,6 +592,10 @@ int jubik_model_create(jubik_model** model, jubik_gp* gp, jubik_linpatch* lp, do
   }
   *model = m;
   return kOk;
+}
+
+int jubik_model_create(jubik_model** model, jubik_gp* gp, jubik_linpatch* lp, double pixel_area) {
+  return jubik_model_create_batched(model, gp, lp, pixel_area, 1);
 }
 
 int jubik_model_destroy(jubik_model* model) {
@@ -571,60 +614,132 @@ int jubik_model_set_data(jubik_model* model, const void* exposure, const int32_t
   return kOk;
 }
 
+namespace {
+
+// S gradient evaluations at the positions xi[s]; sky and lam go to slots
+// slot0 .. slot0 + S - 1, nll[s] is the NLL of position s.
+int model_grad(jubik_model* model, int S, int slot0, const void* amp_ref, const void* amp_alpha,
+               const void* xi_ref, const void* xi_alpha, double zm, double mu, const void* dnu,
+               double* nll, void* g_ref, void* g_alpha, hipStream_t s) {
+  jubik_linpatch* lp = model->lp;
+  jubik_gp* gp = model->gp;
+  GpGeom gg = gp->g;
+  gg.zm = zm;
+  gg.mu = mu;
+  gg.S = S;
+  gg.sky_shared = 0;
+  LinGeom g = lp->g;
+  g.S = S;
+  g.sky_shared = 0;
+  void* sky = model->sky_slot(slot0);
+  int st = gp_forward(gp->dtype, gg, model->gbufs(), amp_ref, amp_alpha, xi_ref, xi_alpha, dnu,
+                      sky, s);
+  if (st) return st;
+  st = linpatch_forward_poisson(lp->dtype, g, model->lbufs(), sky, model->pixel_area, model->expo,
+                                model->counts, model->resid, model->lam_slot(slot0),
+                                (double*)model->partial, s);
+  if (st) return st;
+  st = reduce_partials((const double*)model->partial, (int64_t)g.J * g.H, nll, s, S);
+  if (st) return st;
+  st = linpatch_adjoint_gp(lp->dtype, g, model->lbufs(), model->resid, model->pixel_area, sky, dnu,
+                           model->u, s);
+  if (st) return st;
+  gg.uplanes = g.E;
+  return gp_adjoint_u(gp->dtype, gg, model->gbufs(), amp_ref, amp_alpha, model->u, g_ref, g_alpha,
+                      s);
+}
+
+int check_model(jubik_model* model) {
+  if (!model->expo) return fail(kNotReady, "no data set (call jubik_model_set_data)");
+  if (!model->lp->ready) return fail(kNotReady, "no kernel set (call jubik_linpatch_set_kernel)");
+  return kOk;
+}
+
+int check_alpha(const jubik_model* model, const void* amp_alpha, const void* a, const void* b,
+                const void* dnu) {
+  if (model->gp->g.has_alpha && (!amp_alpha || !a || !b || !dnu))
+    return fail(kInvalidArgument, "spectral-index inputs are NULL");
+  return kOk;
+}
+
+}  // namespace
+
 int jubik_model_grad(jubik_model* model, const void* amp_ref, const void* amp_alpha,
                      const void* xi_ref, const void* xi_alpha, double zm, double mu,
                      const void* dnu, double* nll, void* g_ref, void* g_alpha, void* stream) {
-  if (!model || !nll || !g_ref) return fail(kInvalidArgument, "NULL argument");
-  if (!model->expo) return fail(kNotReady, "no data set (call jubik_model_set_data)");
-  jubik_linpatch* lp = model->lp;
-  jubik_gp* gp = model->gp;
-  if (!lp->ready) return fail(kNotReady, "no kernel set (call jubik_linpatch_set_kernel)");
-  DeviceGuard dg(lp->device);
-  hipStream_t s = as_stream(stream);
-  int st = jubik_gp_forward(gp, amp_ref, amp_alpha, xi_ref, xi_alpha, zm, mu, dnu, model->sky,
-                            stream);
+  if (!model || !nll || !g_ref || !amp_ref || !xi_ref)
+    return fail(kInvalidArgument, "NULL argument");
+  int st = check_model(model);
+  if (!st) st = check_alpha(model, amp_alpha, xi_alpha, g_alpha, dnu);
   if (st) return st;
-  const LinGeom& g = lp->g;
-  st = linpatch_forward_poisson(lp->dtype, g, lp->bufs(), model->sky, model->pixel_area,
-                                model->expo, model->counts, model->resid, model->lam,
-                                (double*)model->partial, s);
-  if (st) return st;
-  st = reduce_partials((const double*)model->partial, (int64_t)g.J * g.H, nll, s);
-  if (st) return st;
-  st = linpatch_adjoint_gp(lp->dtype, g, lp->bufs(), model->resid, model->pixel_area, model->sky,
-                           dnu, model->u, s);
-  if (st) return st;
-  GpGeom gg = gp->g;
-  gg.uplanes = g.E;
-  st = gp_adjoint_u(gp->dtype, gg, gp->bufs(), amp_ref, amp_alpha, model->u, g_ref, g_alpha, s);
+  DeviceGuard dg(model->lp->device);
+  model->have_point = false;
+  st = model_grad(model, 1, 0, amp_ref, amp_alpha, xi_ref, xi_alpha, zm, mu, dnu, nll, g_ref,
+                  g_alpha, as_stream(stream));
   if (st == kOk) model->have_point = true;
   return st;
 }
 
-int jubik_model_metric(jubik_model* model, const void* amp_ref, const void* amp_alpha,
-                       const void* dnu, const void* v_ref, const void* v_alpha, void* out_ref,
-                       void* out_alpha, void* stream) {
-  if (!model || !v_ref || !out_ref) return fail(kInvalidArgument, "NULL argument");
+int jubik_model_grad_batch(jubik_model* model, int64_t S, const void* amp_ref,
+                           const void* amp_alpha, const void* xi_ref, const void* xi_alpha,
+                           double zm, double mu, const void* dnu, double* nll, void* g_ref,
+                           void* g_alpha, void* stream) {
+  if (!model || !nll || !g_ref || !amp_ref || !xi_ref)
+    return fail(kInvalidArgument, "NULL argument");
+  if (S < 1 || S > model->max_samples)
+    return fail(kInvalidArgument, "batch of " + std::to_string(S) +
+                                      " samples outside [1, max_samples = " +
+                                      std::to_string(model->max_samples) + "]");
+  int st = check_model(model);
+  if (!st) st = check_alpha(model, amp_alpha, xi_alpha, g_alpha, dnu);
+  if (st) return st;
+  DeviceGuard dg(model->lp->device);
+  return model_grad(model, (int)S, 1, amp_ref, amp_alpha, xi_ref, xi_alpha, zm, mu, dnu, nll,
+                    g_ref, g_alpha, as_stream(stream));
+}
+
+int jubik_model_metric_batch(jubik_model* model, int64_t S, const void* amp_ref,
+                             const void* amp_alpha, const void* dnu, const void* v_ref,
+                             const void* v_alpha, void* out_ref, void* out_alpha, void* stream) {
+  if (!model || !v_ref || !out_ref || !amp_ref) return fail(kInvalidArgument, "NULL argument");
+  if (S < 1 || S > model->max_samples)
+    return fail(kInvalidArgument, "batch of " + std::to_string(S) +
+                                      " samples outside [1, max_samples = " +
+                                      std::to_string(model->max_samples) + "]");
+  int st = check_alpha(model, amp_alpha, v_alpha, out_alpha, dnu);
+  if (st) return st;
   if (!model->have_point)
     return fail(kNotReady, "no base point (call jubik_model_grad first)");
   jubik_linpatch* lp = model->lp;
   jubik_gp* gp = model->gp;
   DeviceGuard dg(lp->device);
   hipStream_t s = as_stream(stream);
-  int st = gp_tangent(gp->dtype, gp->g, gp->bufs(), amp_ref, amp_alpha, v_ref, v_alpha, dnu,
-                      model->sky, model->dsky, s);
+  GpGeom gg = gp->g;
+  gg.S = (int)S;
+  gg.sky_shared = 1;
+  LinGeom g = lp->g;
+  g.S = (int)S;
+  g.sky_shared = 1;
+  const void* sky0 = model->sky_slot(0);
+  st = gp_tangent(gp->dtype, gg, model->gbufs(), amp_ref, amp_alpha, v_ref, v_alpha, dnu, sky0,
+                  model->dsky, s);
   if (st) return st;
-  const LinGeom& g = lp->g;
-  st = linpatch_forward_metric(lp->dtype, g, lp->bufs(), model->dsky, model->pixel_area,
-                               model->expo, model->counts, model->lam, model->resid, s);
+  st = linpatch_forward_metric(lp->dtype, g, model->lbufs(), model->dsky, model->pixel_area,
+                               model->expo, model->counts, model->lam_slot(0), model->resid, s);
   if (st) return st;
-  st = linpatch_adjoint_gp(lp->dtype, g, lp->bufs(), model->resid, model->pixel_area, model->sky,
+  st = linpatch_adjoint_gp(lp->dtype, g, model->lbufs(), model->resid, model->pixel_area, sky0,
                            dnu, model->u, s);
   if (st) return st;
-  GpGeom gg = gp->g;
   gg.uplanes = g.E;
-  return gp_adjoint_u(gp->dtype, gg, gp->bufs(), amp_ref, amp_alpha, model->u, out_ref,
+  return gp_adjoint_u(gp->dtype, gg, model->gbufs(), amp_ref, amp_alpha, model->u, out_ref,
                       out_alpha, s);
+}
+
+int jubik_model_metric(jubik_model* model, const void* amp_ref, const void* amp_alpha,
+                       const void* dnu, const void* v_ref, const void* v_alpha, void* out_ref,
+                       void* out_alpha, void* stream) {
+  return jubik_model_metric_batch(model, 1, amp_ref, amp_alpha, dnu, v_ref, v_alpha, out_ref,
+                                  out_alpha, stream);
 }
 
 }  // extern "C"

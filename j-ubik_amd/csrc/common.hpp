@@ -78,13 +78,18 @@ struct LinGeom {
   int J;          // jobs M * E
   int x_shared;   // 1: the input image of job (mod, e) is x[e]
   int Ek;         // kernel energies: E, or 1 when one kernel serves all
+  int S;          // samples per launch: jobs (s, mod, e) = s * J + mod * E + e
+  int sky_shared; // adjoint into the GP: 1 = one sky (E, H, W) for every
+                  // sample (metric at a base point), 0 = one sky per sample
 };
 
 __device__ __forceinline__ int start_x(const LinGeom& g, int i) { return min(i * g.d, g.limx); }
 __device__ __forceinline__ int start_y(const LinGeom& g, int i) { return min(i * g.d, g.limy); }
-// Spectrum index of (job, patch): kernels are laid out (M, P, Ek).
+// Spectrum index of (job, patch): kernels are laid out (M, P, Ek); every
+// sample of a batch shares them.
 __device__ __forceinline__ int64_t spec_index(const LinGeom& g, int job, int p) {
-  const int mod = job / g.E, e = job - mod * g.E;
+  const int jm = job % g.J;
+  const int mod = jm / g.E, e = jm - mod * g.E;
   return ((int64_t)mod * g.P + p) * g.Ek + (g.Ek == 1 ? 0 : e);
 }
 

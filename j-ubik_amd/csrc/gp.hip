@@ -18,9 +18,9 @@ namespace jubik {
 
 constexpr int GNT = 256;
 
-// grid (N / V).  MODE 0: c = amp * xi (forward / tangent);  MODE 1: c = the
-// sum of the uplanes complex (H, W) cotangent planes, zero outside the sky
-// grid.
+// grid (N / V, S).  MODE 0: c = amp * xi (forward / tangent);  MODE 1: c =
+// the sum of the uplanes complex (H, W) cotangent planes, zero outside the
+// sky grid.  Sample blockIdx.y reads xi[s] / u[s] and writes work[s].
 template <typename T, int N, int MODE>
 __global__ __launch_bounds__(GNT) void gp_cols(GpGeom g, const T* __restrict__ amp_ref,
                                                const T* __restrict__ amp_alpha,
@@ -35,6 +35,11 @@ __global__ __launch_bounds__(GNT) void gp_cols(GpGeom g, const T* __restrict__ a
   __shared__ cplx<T> tw[N];
   const int tid = threadIdx.x;
   const int c0 = xcd_tile(blockIdx.x, gridDim.x) * V;
+  const int64_t smp = blockIdx.y;
+  if (xi_ref) xi_ref += smp * N * N;
+  if (xi_alpha) xi_alpha += smp * N * N;
+  if (u) u += smp * g.uplanes * g.H * g.W;
+  Tw += smp * N * N;
   load_twiddles(tw, twg, N, tid, GNT);
   for (int e = tid; e < V * N; e += GNT) {
     const int x1 = e / V, v = e - (e / V) * V;
@@ -59,9 +64,11 @@ __global__ __launch_bounds__(GNT) void gp_cols(GpGeom g, const T* __restrict__ a
   }
 }
 
-// grid (N / 2 + 1).  MODE 0: sky = exp(log s) cropped to (E, H, W);
+// grid (N / 2 + 1, S).  MODE 0: sky = exp(log s) cropped to (E, H, W);
 // MODE 1: g_ref = amp_ref HT(u_ref), g_alpha = amp_alpha HT(u_alpha) on N x N;
 // MODE 2: tangent ds_e = sky_e (HT(a) + HT(b) dnu_e) on (E, H, W).
+// Sample blockIdx.y reads work[s] and writes output s; the tangent's sky is
+// the same base-point sky for every sample when g.sky_shared.
 template <typename T, int N, int MODE>
 __global__ __launch_bounds__(GNT) void gp_rows(GpGeom g, const cplx<T>* __restrict__ Tw,
                                                const cplx<T>* __restrict__ twg,
@@ -76,6 +83,15 @@ __global__ __launch_bounds__(GNT) void gp_rows(GpGeom g, const cplx<T>* __restri
   const int tid = threadIdx.x;
   const int r0 = blockIdx.x;
   const int r1 = (N - r0) % N;
+  const int64_t smp = blockIdx.y;
+  Tw += smp * N * N;
+  if constexpr (MODE == 1) {
+    out_a += smp * N * N;
+    if (g.has_alpha) out_b += smp * N * N;
+  } else {
+    out_a += smp * g.E * g.H * g.W;
+    if (MODE == 2 && !g.sky_shared) sky_in += smp * g.E * g.H * g.W;
+  }
   load_twiddles(tw, twg, N, tid, GNT);
   for (int e = tid; e < 2 * N; e += GNT) {
     const int v = e / N, k = e - (e / N) * N;
@@ -155,13 +171,13 @@ struct GpOps {
                      const void* dnu, void* sky, hipStream_t s) {
     {
       TimedLaunch _t("gp_cols_fwd", s);
-      gp_cols<T, N, 0><<<N / V, GNT, 0, s>>>(g, (const T*)amp_ref, (const T*)amp_alpha,
+      gp_cols<T, N, 0><<<dim3(N / V, g.S), GNT, 0, s>>>(g, (const T*)amp_ref, (const T*)amp_alpha,
                                              (const T*)xi_ref, (const T*)xi_alpha, nullptr,
                                              (const C*)b.tw, (C*)b.work);
     }
     {
       TimedLaunch _t("gp_rows_fwd", s);
-      gp_rows<T, N, 0><<<N / 2 + 1, GNT, 0, s>>>(g, (const C*)b.work, (const C*)b.tw,
+      gp_rows<T, N, 0><<<dim3(N / 2 + 1, g.S), GNT, 0, s>>>(g, (const C*)b.work, (const C*)b.tw,
                                                  (const T*)dnu, nullptr, nullptr, nullptr,
                                                  (T*)sky, nullptr);
     }
@@ -172,13 +188,13 @@ struct GpOps {
                      const void* dnu, const void* sky, void* dsky, hipStream_t s) {
     {
       TimedLaunch _t("gp_cols_fwd", s);
-      gp_cols<T, N, 0><<<N / V, GNT, 0, s>>>(g, (const T*)amp_ref, (const T*)amp_alpha,
+      gp_cols<T, N, 0><<<dim3(N / V, g.S), GNT, 0, s>>>(g, (const T*)amp_ref, (const T*)amp_alpha,
                                              (const T*)v_ref, (const T*)v_alpha, nullptr,
                                              (const C*)b.tw, (C*)b.work);
     }
     {
       TimedLaunch _t("gp_rows_tangent", s);
-      gp_rows<T, N, 2><<<N / 2 + 1, GNT, 0, s>>>(g, (const C*)b.work, (const C*)b.tw,
+      gp_rows<T, N, 2><<<dim3(N / 2 + 1, g.S), GNT, 0, s>>>(g, (const C*)b.work, (const C*)b.tw,
                                                  (const T*)dnu, nullptr, nullptr, (const T*)sky,
                                                  (T*)dsky, nullptr);
     }
@@ -189,12 +205,12 @@ struct GpOps {
                        hipStream_t s) {
     {
       TimedLaunch _t("gp_cols_adj", s);
-      gp_cols<T, N, 1><<<N / V, GNT, 0, s>>>(g, nullptr, nullptr, nullptr, nullptr, (const C*)u,
+      gp_cols<T, N, 1><<<dim3(N / V, g.S), GNT, 0, s>>>(g, nullptr, nullptr, nullptr, nullptr, (const C*)u,
                                              (const C*)b.tw, (C*)b.work);
     }
     {
       TimedLaunch _t("gp_rows_adj", s);
-      gp_rows<T, N, 1><<<N / 2 + 1, GNT, 0, s>>>(g, (const C*)b.work, (const C*)b.tw, nullptr,
+      gp_rows<T, N, 1><<<dim3(N / 2 + 1, g.S), GNT, 0, s>>>(g, (const C*)b.work, (const C*)b.tw, nullptr,
                                                  (const T*)amp_ref, (const T*)amp_alpha, nullptr,
                                                  (T*)g_ref, (T*)g_alpha);
     }

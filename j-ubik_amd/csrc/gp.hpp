@@ -11,11 +11,14 @@ struct GpGeom {
   int has_alpha;  // spectral-index field present
   double zm, mu;  // zero mode and spectral-index mean
   int uplanes;    // adjoint input: number of (H, W) complex planes summed on load
+  int S;          // samples per launch (grid axis y); inputs and outputs are
+                  // stacked per sample, the amplitude fields are shared
+  int sky_shared; // tangent: 1 = one base-point sky for every sample
 };
 
 struct GpBuffers {
   const void* tw;  // N twiddles
-  void* work;      // N x N complex
+  void* work;      // S x N x N complex
 };
 
 bool gp_length_supported(int N);

@@ -9,8 +9,8 @@ struct LinBuffers {
   const void* tw;    // L twiddles exp(-2 pi i e / L)
   const void* tent;  // w tent weights (jubik/convolve.py:18-25)
   void* KT;          // prepared spectra, (M, P, Ek) x L2 x L, column-major
-  void* R1;          // row-pass workspace, J x P x L2 x L
-  void* R2;          // column-pass workspace, J x P x L x L2
+  void* R1;          // row-pass workspace, S x J x P x L x L2p
+  void* R2;          // column-pass workspace, S x J x P x L x L2p
 };
 
 // FFT lengths L = 2 (H / n) + 2 margin compiled into the library.

@@ -56,8 +56,8 @@ __global__ __launch_bounds__(NT) void lp_rows(LinGeom g, const T* __restrict__ s
   if constexpr (MODE == ROW_SIGNAL || MODE == ROW_COTANGENT) {
     const int job = z;
     r1base = (int64_t)job * g.P + p;
-    int im = job;
-    if (MODE == ROW_SIGNAL && g.x_shared) im = job % g.E;
+    int im = job;  // shared input: image (sample, energy) of the sky stack
+    if (MODE == ROW_SIGNAL && g.x_shared) im = (job / g.J) * g.E + job % g.E;
     img = src + (int64_t)im * g.H * g.W;
   } else {
     const int mod = z / g.Ek, ek = z - mod * g.Ek;
@@ -303,7 +303,8 @@ __device__ __forceinline__ int first_cover(int C, int len, int d) {
   return C >= len ? (C - len) / d + 1 : 0;
 }
 
-// grid (H, J).
+// grid (H, S * J).  Exposure, counts and (metric) the base-point lam are
+// per (mod, e) and shared by the samples; y, resid and partial are per job.
 template <typename T, int L, int EPI, int JPT>
 __global__ __launch_bounds__(NT) void lp_fwd_out(LinGeom g, const cplx<T>* __restrict__ R2,
                                                  const cplx<T>* __restrict__ twg,
@@ -345,15 +346,16 @@ __global__ __launch_bounds__(NT) void lp_fwd_out(LinGeom g, const cplx<T>* __res
   // Epilogue operands are fetched up front so their latency hides behind the
   // spectrum loads and the inverse FFT.
   const int64_t row = ((int64_t)job * g.H + i) * g.W;
+  const int64_t rowd = ((int64_t)(job % g.J) * g.H + i) * g.W;
   T ex_v[JPT], lam_v[JPT];
   int cnt_v[JPT];
   if constexpr (EPI != 0) {
 #pragma unroll
     for (int t = 0; t < JPT; ++t) {
       const int j = tid + t * NT;
-      ex_v[t] = j < g.W ? expo[row + j] : T(0);
-      cnt_v[t] = j < g.W ? counts[row + j] : -1;
-      if (EPI == 2) lam_v[t] = j < g.W ? y[row + j] : T(1);
+      ex_v[t] = j < g.W ? expo[rowd + j] : T(0);
+      cnt_v[t] = j < g.W ? counts[rowd + j] : -1;
+      if (EPI == 2) lam_v[t] = j < g.W ? y[rowd + j] : T(1);
     }
   }
   __syncthreads();
@@ -458,9 +460,10 @@ __global__ __launch_bounds__(NT) void lp_fwd_out(LinGeom g, const cplx<T>* __res
 }
 
 // EPI 0: grid (H, nimg); gx[img] = scale * sum over the jobs of img.
-// EPI 1: grid (H, E);  u[e] = sum_mod scale * sky[e] * xbar_(mod,e) * (1, dnu[e]),
-//        one complex (u_ref, u_alpha) plane per energy (the GP adjoint sums
-//        the planes while loading them).
+// EPI 1: grid (H, S * E);  u[s, e] = sum_mod scale * sky[s, e] * xbar_(s,mod,e)
+//        * (1, dnu[e]), one complex (u_ref, u_alpha) plane per sample and
+//        energy (the GP adjoint sums a sample's planes while loading them);
+//        sky[s, e] is sky[e] when g.sky_shared.
 template <typename T, int L, int EPI, int JPT>
 __global__ __launch_bounds__(NT) void lp_adj_out(LinGeom g, const cplx<T>* __restrict__ R2,
                                                  const cplx<T>* __restrict__ twg,
@@ -506,10 +509,13 @@ __global__ __launch_bounds__(NT) void lp_adj_out(LinGeom g, const cplx<T>* __res
   const int nc = nc_s;
   const int npair = (g.n + 1) / 2;
 
-  // EPI 0 with a shared input and EPI 1: the jobs (mod, e) of energy e =
-  // blockIdx.y, summed over modules; EPI 0 otherwise: job blockIdx.y.
-  int jb = blockIdx.y, je = g.J, jstep = g.E;
+  // EPI 0 with a shared input and EPI 1: the jobs (s, mod, e) of image
+  // blockIdx.y = s * E + e, summed over modules; EPI 0 otherwise: job
+  // blockIdx.y.
+  const int smp = blockIdx.y / g.E, eb = blockIdx.y - smp * g.E;
+  int jb = smp * g.J + eb, je = (smp + 1) * g.J, jstep = g.E;
   if (EPI == 0 && !g.x_shared) {
+    jb = blockIdx.y;
     je = blockIdx.y + 1;
     jstep = 1;
   }
@@ -517,7 +523,8 @@ __global__ __launch_bounds__(NT) void lp_adj_out(LinGeom g, const cplx<T>* __res
   // EPI 1: the sky row of this energy, fetched before the spectrum loads.
   T sky_v[JPT];
   if (EPI == 1) {
-    const T* srow = sky + ((int64_t)blockIdx.y * g.H + i) * g.W;
+    const int simg = g.sky_shared ? eb : blockIdx.y;
+    const T* srow = sky + ((int64_t)simg * g.H + i) * g.W;
 #pragma unroll
     for (int t = 0; t < JPT; ++t) {
       const int j = tid + t * NT;
@@ -658,13 +665,13 @@ struct LinOps {
 
   static void fwd_front(const LinGeom& g, const LinBuffers& b, const void* x, double scale,
                         hipStream_t s) {
-    dim3 gr((g.w / 2 + VR - 1) / VR, g.P, g.J);
+    dim3 gr((g.w / 2 + VR - 1) / VR, g.P, g.S * g.J);
     {
       TimedLaunch _t("lp_rows_fwd", s);
       lp_rows<T, L, ROW_SIGNAL><<<gr, NT, 0, s>>>(g, (const T*)x, (const T*)b.tent, (const C*)b.tw,
                                                   (C*)b.R1, T(scale), g.w);
     }
-    dim3 gc((L2 + V - 1) / V, g.P, g.J);
+    dim3 gc((L2 + V - 1) / V, g.P, g.S * g.J);
     {
       TimedLaunch _t("lp_cols_fwd", s);
       lp_cols<T, L, COL_FWD><<<gc, NT, 0, s>>>(g, (const C*)b.R1, (const C*)b.KT, (const C*)b.tw,
@@ -673,13 +680,13 @@ struct LinOps {
   }
 
   static void adj_front(const LinGeom& g, const LinBuffers& b, const void* gy, hipStream_t s) {
-    dim3 gr((L / 2 + VR - 1) / VR, g.P, g.J);
+    dim3 gr((L / 2 + VR - 1) / VR, g.P, g.S * g.J);
     {
       TimedLaunch _t("lp_rows_adj", s);
       lp_rows<T, L, ROW_COTANGENT><<<gr, NT, 0, s>>>(g, (const T*)gy, (const T*)b.tent,
                                                      (const C*)b.tw, (C*)b.R1, T(1), L);
     }
-    dim3 gc((L2 + V - 1) / V, g.P, g.J);
+    dim3 gc((L2 + V - 1) / V, g.P, g.S * g.J);
     {
       TimedLaunch _t("lp_cols_adj", s);
       lp_cols<T, L, COL_ADJ><<<gc, NT, 0, s>>>(g, (const C*)b.R1, (const C*)b.KT, (const C*)b.tw,
@@ -702,7 +709,7 @@ struct LinOps {
                       const char* name) {
     TimedLaunch _t(name, s);
     with_jpt(g, [&](auto jpt) {
-      lp_fwd_out<T, L, EPI, decltype(jpt)::value><<<dim3(g.H, g.J), NT, 0, s>>>(
+      lp_fwd_out<T, L, EPI, decltype(jpt)::value><<<dim3(g.H, g.S * g.J), NT, 0, s>>>(
           g, (const C*)b.R2, (const C*)b.tw, y, expo, counts, resid, partial);
     });
   }
@@ -744,7 +751,7 @@ struct LinOps {
   static int adjoint(const LinGeom& g, const LinBuffers& b, const void* gy, void* gx,
                      double scale, hipStream_t s) {
     adj_front(g, b, gy, s);
-    adj_out<0>(g, b, g.x_shared ? g.E : g.J, (T*)gx, scale, nullptr, nullptr, nullptr, s,
+    adj_out<0>(g, b, g.S * (g.x_shared ? g.E : g.J), (T*)gx, scale, nullptr, nullptr, nullptr, s,
                "lp_adj_out");
     return launch_status("linpatch_adjoint");
   }
@@ -752,7 +759,7 @@ struct LinOps {
   static int adjoint_gp(const LinGeom& g, const LinBuffers& b, const void* gy, double scale,
                         const void* sky, const void* dnu, void* u, hipStream_t s) {
     adj_front(g, b, gy, s);
-    adj_out<1>(g, b, g.E, nullptr, scale, (const T*)sky, (const T*)dnu, (C*)u, s,
+    adj_out<1>(g, b, g.S * g.E, nullptr, scale, (const T*)sky, (const T*)dnu, (C*)u, s,
                "lp_adj_out_gp");
     return launch_status("linpatch_adjoint_gp");
   }

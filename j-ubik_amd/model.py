@@ -6,7 +6,9 @@ jubik/instruments/erosita/erosita_likelihood.py:51-89 and
 jubik/instruments/chandra/chandra_likelihood.py:15-37 composed with the GP
 sky.  ``nll_and_grad`` runs forward, Poisson residual and the hand-written
 adjoint chain in libjubik_hip.so; ``metric`` gives the MGVI metric product
-J^T diag(1/lam) J v at the last gradient position.
+J^T diag(1/lam) J v at the last gradient position.  ``nll_and_grad_batch``
+and ``metric_batch`` evaluate S positions / tangents in one pass (the VI
+samples of nifty.re's optimize_kl, demos/x-ray-imaging.py:404-430).
 """
 import ctypes
 
@@ -41,7 +43,7 @@ class XrayModel:
     """GP sky + per-module linear-patching PSF + exposure + Poisson data."""
 
     def __init__(self, sky: CorrelatedFieldSky, psf: LinpatchOperator, pixel_area, exposure,
-                 counts):
+                 counts, max_samples=1):
         if psf.dtype != sky.dtype:
             raise ValueError("sky and PSF operators must share a dtype")
         if not psf.x_shared:
@@ -56,8 +58,9 @@ class XrayModel:
             counts, torch.Tensor) else counts.to(torch.int32), device="cuda").reshape(
                 shape).contiguous()
         self._model = ctypes.c_void_p()
-        _lib.call("jubik_model_create", ctypes.byref(self._model), sky._plan, psf._plan,
-                  self.pixel_area)
+        self.max_samples = int(max_samples)
+        _lib.call("jubik_model_create_batched", ctypes.byref(self._model), sky._plan, psf._plan,
+                  self.pixel_area, self.max_samples)
         _lib.call("jubik_model_set_data", self._model, dev.ptr(self.exposure),
                   dev.ptr(self.counts))
         self._nll = torch.empty(1, dtype=torch.float64, device="cuda")
@@ -106,5 +109,50 @@ class XrayModel:
         _lib.call("jubik_model_metric", self._model, dev.ptr(s.amp_ref), dev.ptr(s.amp_alpha),
                   dev.ptr(s.dnu_dev), dev.ptr(vr), dev.ptr(va), dev.ptr(o_ref),
                   dev.ptr(o_alpha), dev.stream())
+        return dev.like_input(o_ref, v_ref), (
+            dev.like_input(o_alpha, v_ref) if o_alpha is not None else None)
+
+    def _stack(self, a, S, name):
+        N = self.sky.N
+        t = dev.to_device(a, self.dtype)
+        if tuple(t.shape) != (S, N, N):
+            raise ValueError(f"{name} must have shape ({S}, {N}, {N}), got {tuple(t.shape)}")
+        return t.contiguous()
+
+    def _batch_size(self, a):
+        S = int(a.shape[0])
+        if not 1 <= S <= self.max_samples:
+            raise ValueError(f"batch of {S} samples outside [1, max_samples={self.max_samples}]")
+        return S
+
+    def nll_and_grad_batch(self, xi_ref, xi_alpha=None):
+        """S gradient evaluations: xi of shape (S, N, N); returns (nll float64
+        numpy (S,), g_ref (S, N, N), g_alpha (S, N, N) or None).  The metric
+        base point is not changed."""
+        s = self.sky
+        S = self._batch_size(xi_ref)
+        xr = self._stack(xi_ref, S, "xi_ref")
+        xa = self._stack(xi_alpha, S, "xi_alpha") if s.has_alpha else None
+        g_ref = torch.empty((S, s.N, s.N), dtype=self.dtype, device="cuda")
+        g_alpha = torch.empty_like(g_ref) if s.has_alpha else None
+        nll = torch.empty(S, dtype=torch.float64, device="cuda")
+        _lib.call("jubik_model_grad_batch", self._model, S, dev.ptr(s.amp_ref),
+                  dev.ptr(s.amp_alpha), dev.ptr(xr), dev.ptr(xa), s.zm, s.mu, dev.ptr(s.dnu_dev),
+                  dev.ptr(nll), dev.ptr(g_ref), dev.ptr(g_alpha), dev.stream())
+        return nll.cpu().numpy(), dev.like_input(g_ref, xi_ref), (
+            dev.like_input(g_alpha, xi_ref) if g_alpha is not None else None)
+
+    def metric_batch(self, v_ref, v_alpha=None):
+        """S metric products J^T diag(1/lam) J v at the last gradient position;
+        v of shape (S, N, N)."""
+        s = self.sky
+        S = self._batch_size(v_ref)
+        vr = self._stack(v_ref, S, "v_ref")
+        va = self._stack(v_alpha, S, "v_alpha") if s.has_alpha else None
+        o_ref = torch.empty((S, s.N, s.N), dtype=self.dtype, device="cuda")
+        o_alpha = torch.empty_like(o_ref) if s.has_alpha else None
+        _lib.call("jubik_model_metric_batch", self._model, S, dev.ptr(s.amp_ref),
+                  dev.ptr(s.amp_alpha), dev.ptr(s.dnu_dev), dev.ptr(vr), dev.ptr(va),
+                  dev.ptr(o_ref), dev.ptr(o_alpha), dev.stream())
         return dev.like_input(o_ref, v_ref), (
             dev.like_input(o_alpha, v_ref) if o_alpha is not None else None)

@@ -84,7 +84,7 @@ def make_problem(H=64, n=4, margin=8, E=1, M=1, seed=0, N=None, mask_frac=0.1,
     return prob, (xi, xa)
 
 
-def to_product(ja, prob, dtype):
+def to_product(ja, prob, dtype, max_samples=1):
     """The same problem as a jubik_amd XrayModel."""
     E, H, W = prob.shape
     N = prob.amp_ref.shape[0]
@@ -97,4 +97,4 @@ def to_product(ja, prob, dtype):
     psf.set_kernel(win if M > 1 else win[0], layout="window")
     expo = np.stack([m.exposure for m in prob.modules])
     counts = ja.dense_counts([m.data for m in prob.modules], np.stack([m.keep for m in prob.modules]))
-    return ja.XrayModel(sky, psf, prob.pixel_area, expo, counts)
+    return ja.XrayModel(sky, psf, prob.pixel_area, expo, counts, max_samples=max_samples)
