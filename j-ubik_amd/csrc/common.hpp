@@ -93,6 +93,17 @@ __device__ __forceinline__ int64_t spec_index(const LinGeom& g, int job, int p) 
   return ((int64_t)mod * g.P + p) * g.Ek + (g.Ek == 1 ? 0 : e);
 }
 
+// Tent weight t of a 2d window (jubik/convolve.py:18-25): numpy
+// linspace(0, 1, d) then its mirror image; entry i of linspace is i * step
+// with step = 1 / (d - 1) in double and the last entry exactly 1.  IEEE
+// double division and multiply give the same values as the host table.
+template <typename T>
+__device__ __forceinline__ T tent_weight(int d, double step, int t) {
+  const int i = t < d ? t : 2 * d - 1 - t;
+  return (T)((d > 1 && i == d - 1) ? 1.0 : i * step);
+}
+__device__ __forceinline__ double tent_step(int d) { return d > 1 ? 1.0 / (d - 1) : 0.0; }
+
 // Tile sizes: vectors per work-group so a tile of V vectors of length L at
 // the odd LDS stride occupies at most 52 KiB (16 float vectors at L = 384).
 #ifndef JUBIK_COL_TILE_BYTES

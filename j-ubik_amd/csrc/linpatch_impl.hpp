@@ -29,7 +29,6 @@ enum RowMode { ROW_SIGNAL = 0, ROW_COTANGENT = 1, ROW_KERNEL_WIN = 2, ROW_KERNEL
 enum ColMode { COL_FWD = 0, COL_ADJ = 1, COL_KERNEL = 2 };
 
 constexpr int NT = 256;
-constexpr int MAXN = 64;
 
 // ------------------------------------------------------------------ rows
 // grid (ceil(nrows/2 / V), P, Z); Z = jobs (signal modes) or M*Ek (kernel).
@@ -66,20 +65,25 @@ __global__ __launch_bounds__(NT) void lp_rows(LinGeom g, const T* __restrict__ s
     img = src + spec * (MODE == ROW_KERNEL_WIN ? (int64_t)g.w * g.w : (int64_t)g.H * g.W);
   }
 
-  load_twiddles(tw, twg, L, tid, NT);
-  __shared__ T tent_s[L];  // w <= L tent weights
-  if constexpr (MODE == ROW_SIGNAL) {
-    for (int t = tid; t < g.w; t += NT) tent_s[t] = tent[t];
-    __syncthreads();
-  }
+  // Every global load (window values, tent weights, twiddles) is issued
+  // before the first LDS store, so their latencies overlap.
   constexpr int EPT = (V * L + NT - 1) / NT;
+  constexpr int TPT = (L + NT - 1) / NT;
+  cplx<T> twr[TPT];
+#pragma unroll
+  for (int u = 0; u < TPT; ++u) {
+    const int e = tid + u * NT;
+    if (e < L) twr[u] = twg[e];
+  }
+  T av[EPT], bv[EPT];
+  const double step = tent_step(g.d);
 #pragma unroll
   for (int u = 0; u < EPT; ++u) {
     const int e = tid + u * NT;
+    av[u] = bv[u] = T(0);
     if (V * L % NT != 0 && e >= V * L) break;
     const int v = e / L, f = e - (e / L) * L;
     const int r0 = 2 * (q0 + v);
-    T a = T(0), b = T(0);
     if (r0 < nrows) {
       if constexpr (MODE == ROW_SIGNAL) {
         const int c = f - g.m;
@@ -87,23 +91,24 @@ __global__ __launch_bounds__(NT) void lp_rows(LinGeom g, const T* __restrict__ s
           const int j = sy + c - g.h;
           const int i0 = sx + r0 - g.h;
           if (j >= 0 && j < g.W) {
-            const T wc = tent_s[c] * scale;
-            if (i0 >= 0 && i0 < g.H) a = tent_s[r0] * wc * img[i0 * g.W + j];
-            if (i0 + 1 >= 0 && i0 + 1 < g.H) b = tent_s[r0 + 1] * wc * img[(i0 + 1) * g.W + j];
+            const T wc = tent_weight<T>(g.d, step, c) * scale;
+            if (i0 >= 0 && i0 < g.H) av[u] = tent_weight<T>(g.d, step, r0) * wc * img[i0 * g.W + j];
+            if (i0 + 1 >= 0 && i0 + 1 < g.H)
+              bv[u] = tent_weight<T>(g.d, step, r0 + 1) * wc * img[(i0 + 1) * g.W + j];
           }
         }
       } else if constexpr (MODE == ROW_COTANGENT) {
         const int j = sy + f - g.h - g.m;
         const int i0 = sx + r0 - g.h - g.m;
         if (j >= 0 && j < g.W) {
-          if (i0 >= 0 && i0 < g.H) a = img[i0 * g.W + j];
-          if (i0 + 1 >= 0 && i0 + 1 < g.H) b = img[(i0 + 1) * g.W + j];
+          if (i0 >= 0 && i0 < g.H) av[u] = img[i0 * g.W + j];
+          if (i0 + 1 >= 0 && i0 + 1 < g.H) bv[u] = img[(i0 + 1) * g.W + j];
         }
       } else if constexpr (MODE == ROW_KERNEL_WIN) {
         const int c = f - g.m;
         if (c >= 0 && c < g.w) {
-          a = img[r0 * g.w + c];
-          b = img[(r0 + 1) * g.w + c];
+          av[u] = img[r0 * g.w + c];
+          bv[u] = img[(r0 + 1) * g.w + c];
         }
       } else {  // ROW_KERNEL_FULL: cut[r][c] = K[(r - d) mod H][(c - d) mod W]
         const int c = f - g.m;
@@ -111,12 +116,23 @@ __global__ __launch_bounds__(NT) void lp_rows(LinGeom g, const T* __restrict__ s
           const int j = (c - g.d + g.W) % g.W;
           const int ia = (r0 - g.d + g.H) % g.H;
           const int ib = (r0 + 1 - g.d + g.H) % g.H;
-          a = img[ia * g.W + j];
-          b = img[ib * g.W + j];
+          av[u] = img[ia * g.W + j];
+          bv[u] = img[ib * g.W + j];
         }
       }
     }
-    buf[v * VS + f] = cplx<T>{a, b};
+  }
+#pragma unroll
+  for (int u = 0; u < TPT; ++u) {
+    const int e = tid + u * NT;
+    if (e < L) tw[e] = twr[u];
+  }
+#pragma unroll
+  for (int u = 0; u < EPT; ++u) {
+    const int e = tid + u * NT;
+    if (V * L % NT != 0 && e >= V * L) break;
+    const int v = e / L, f = e - (e / L) * L;
+    buf[v * VS + f] = cplx<T>{av[u], bv[u]};
   }
   __syncthreads();
   fft_lds<T, L, V, VS, NT, false>(buf, tw, tid);
@@ -295,12 +311,107 @@ __global__ __launch_bounds__(NT) void lp_cols(LinGeom g, const cplx<T>* __restri
 template <typename T, int L>
 struct OutTile {
   static constexpr int V = 4;  // iy pairs per pass: n <= 8 in one pass
-  static constexpr int EPT = (V * L + NT - 1) / NT;
+  static constexpr int L2 = L / 2 + 1;
+  static constexpr int GPT = (V * L2 + NT - 1) / NT;  // gather tasks (v, kk) per thread
+  static constexpr int TPT = (L + NT - 1) / NT;       // twiddles per thread
 };
 
 // First patch column whose window [s, s + len) can contain canvas column C.
 __device__ __forceinline__ int first_cover(int C, int len, int d) {
   return C >= len ? (C - len) / d + 1 : 0;
+}
+
+// Patch rows covering canvas row R: windows [start_x(ix), start_x(ix) + len)
+// with ix in [lo, lo + count).  Starts are strictly increasing, so the set
+// is contiguous.  Every argument is uniform over the work-group.
+__device__ __forceinline__ void cover_range(const LinGeom& g, int R, int len, int& lo,
+                                            int& count) {
+  lo = g.n;
+  count = 0;
+  for (int ix = 0; ix < g.n; ++ix) {
+    const int s = start_x(g, ix);
+    if (R >= s && R < s + len) {
+      lo = min(lo, ix);
+      ++count;
+    }
+  }
+}
+
+// buf[v][k] for v < V, k < L: the sum over the covering patch rows ci of
+// wt_ci * (A_ci(k) + i B_ci(k)), where A and B are the full spectra of patch
+// columns iy = 2(q0 + v) and 2(q0 + v) + 1 rebuilt from their half spectra
+// (X(L - kk) = conj X(kk)).  Patch (ix, iy) row f starts at
+// base + ((iy n + ix) nrows + f) L2p.  Each half-spectrum element is loaded
+// once and fills k = kk and k = L - kk; the loads of up to three covering
+// rows are in flight together.  Weights are tent[f] when WEIGHTED.
+// Patch offsets (2q + 1) n nrows L2p stay below 2^31 for n <= 64, L <= 384.
+template <typename T, int L, bool WEIGHTED>
+__device__ __forceinline__ void sum_pair_spectra(cplx<T>* buf, const cplx<T>* __restrict__ base,
+                                                 const LinGeom& g, int nrows, int R, int lo,
+                                                 int nc, const T* __restrict__ tent, int q0,
+                                                 int tid) {
+  using OT = OutTile<T, L>;
+  constexpr int V = OT::V, L2 = OT::L2, GPT = OT::GPT;
+  constexpr int VS = odd_stride(L);
+  constexpr int L2P = half_stride(L);
+  constexpr int CH = 3;  // covering rows per load batch (L <= 3d in the common geometries)
+  const int npair = (g.n + 1) / 2;
+  const int pstride = nrows * L2P;  // n <= 64, L <= 384: offsets fit in 31 bits
+  int oa[GPT], ob[GPT];
+#pragma unroll
+  for (int u = 0; u < GPT; ++u) {
+    const int e = tid + u * NT;
+    const int v = e / L2, kk = e - (e / L2) * L2;
+    const int q = q0 + v;
+    const bool ok = (V * L2 % NT == 0 || e < V * L2) && q < npair;
+    oa[u] = ok ? 2 * q * g.n * pstride + kk : -1;
+    ob[u] = ok && 2 * q + 1 < g.n ? (2 * q + 1) * g.n * pstride + kk : -1;
+  }
+  cplx<T> A[GPT], B[GPT];
+#pragma unroll
+  for (int u = 0; u < GPT; ++u) A[u] = B[u] = cplx<T>{T(0), T(0)};
+  for (int c0 = 0; c0 < nc; c0 += CH) {
+    cplx<T> a[CH][GPT], b[CH][GPT];
+    T wt[CH];
+#pragma unroll
+    for (int c = 0; c < CH; ++c) {
+      wt[c] = T(0);
+#pragma unroll
+      for (int u = 0; u < GPT; ++u) a[c][u] = b[c][u] = cplx<T>{T(0), T(0)};
+      if (c0 + c < nc) {
+        const int ix = lo + c0 + c;
+        const int f = R - start_x(g, ix);
+        wt[c] = WEIGHTED ? tent[f] : T(1);
+        const cplx<T>* rowp = base + ((int64_t)ix * nrows + f) * L2P;
+#pragma unroll
+        for (int u = 0; u < GPT; ++u) {
+          if (oa[u] >= 0) a[c][u] = rowp[oa[u]];
+          if (ob[u] >= 0) b[c][u] = rowp[ob[u]];
+        }
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < CH; ++c)
+#pragma unroll
+      for (int u = 0; u < GPT; ++u) {
+        if (WEIGHTED) {
+          A[u] = cadd(A[u], cscale(a[c][u], wt[c]));
+          B[u] = cadd(B[u], cscale(b[c][u], wt[c]));
+        } else {
+          A[u] = cadd(A[u], a[c][u]);
+          B[u] = cadd(B[u], b[c][u]);
+        }
+      }
+  }
+#pragma unroll
+  for (int u = 0; u < GPT; ++u) {
+    const int e = tid + u * NT;
+    if (V * L2 % NT != 0 && e >= V * L2) break;
+    const int v = e / L2, kk = e - (e / L2) * L2;
+    cplx<T>* dst = buf + v * VS;
+    dst[kk] = cplx<T>{A[u].re - B[u].im, A[u].im + B[u].re};
+    if (kk > 0 && 2 * kk < L) dst[L - kk] = cplx<T>{A[u].re + B[u].im, B[u].re - A[u].im};
+  }
 }
 
 // grid (H, S * J).  Exposure, counts and (metric) the base-point lam are
@@ -312,39 +423,31 @@ __global__ __launch_bounds__(NT) void lp_fwd_out(LinGeom g, const cplx<T>* __res
                                                  const int* __restrict__ counts,
                                                  T* __restrict__ resid,
                                                  double* __restrict__ partial) {
-  constexpr int V = OutTile<T, L>::V;
-  constexpr int EPT = OutTile<T, L>::EPT;
+  using OT = OutTile<T, L>;
+  constexpr int V = OT::V, TPT = OT::TPT;
   constexpr int VS = odd_stride(L);
-  constexpr int L2 = L / 2 + 1;
   constexpr int L2P = half_stride(L);
   __shared__ cplx<T> buf[V * VS];
   __shared__ cplx<T> tw[L];
-  __shared__ int lst_ix[MAXN], lst_f[MAXN];
-  __shared__ int nc_s;
   __shared__ double red[NT / 64];
   const int tid = threadIdx.x;
   const int i = blockIdx.x;
   const int job = blockIdx.y;
   const int R = i + g.h + g.m;
+  int lo, nc;
+  cover_range(g, R, L, lo, nc);
 
-  load_twiddles(tw, twg, L, tid, NT);
-  if (tid == 0) {
-    int nc = 0;
-    for (int ix = 0; ix < g.n; ++ix) {
-      const int s = start_x(g, ix);
-      if (R >= s && R < s + L) {
-        lst_ix[nc] = ix;
-        lst_f[nc] = R - s;
-        ++nc;
-      }
-    }
-    nc_s = nc;
+  // Twiddles and epilogue operands are fetched up front; their latency
+  // overlaps the spectrum loads.
+  cplx<T> twr[TPT];
+#pragma unroll
+  for (int u = 0; u < TPT; ++u) {
+    const int e = tid + u * NT;
+    if (e < L) twr[u] = twg[e];
   }
   T acc[JPT];
 #pragma unroll
   for (int t = 0; t < JPT; ++t) acc[t] = T(0);
-  // Epilogue operands are fetched up front so their latency hides behind the
-  // spectrum loads and the inverse FFT.
   const int64_t row = ((int64_t)job * g.H + i) * g.W;
   const int64_t rowd = ((int64_t)(job % g.J) * g.H + i) * g.W;
   T ex_v[JPT], lam_v[JPT];
@@ -358,41 +461,16 @@ __global__ __launch_bounds__(NT) void lp_fwd_out(LinGeom g, const cplx<T>* __res
       if (EPI == 2) lam_v[t] = j < g.W ? y[rowd + j] : T(1);
     }
   }
-  __syncthreads();
-  const int nc = nc_s;
   const int npair = (g.n + 1) / 2;
   const cplx<T>* base = R2 + (int64_t)job * g.P * L * L2P;
 
   for (int q0 = 0; q0 < npair; q0 += V) {
-    // Per-element offsets of patch columns 2q and 2q+1 (-1: absent) and the
-    // conjugation sign of the mirrored half, fixed for every patch row.
-    int offa[EPT], offb[EPT];
-    T sg[EPT];
+    sum_pair_spectra<T, L, false>(buf, base, g, L, R, lo, nc, nullptr, q0, tid);
+    if (q0 == 0) {
 #pragma unroll
-    for (int u = 0; u < EPT; ++u) {
-      const int e = tid + u * NT;
-      const int v = e / L, k = e - (e / L) * L;
-      const int q = q0 + v;
-      const bool cj = k > L / 2;
-      const int kk = cj ? L - k : k;
-      const bool ok = (V * L % NT == 0 || e < V * L) && q < npair;
-      offa[u] = ok ? 2 * q * g.n * L * L2P + kk : -1;
-      offb[u] = ok && 2 * q + 1 < g.n ? (2 * q + 1) * g.n * L * L2P + kk : -1;
-      sg[u] = cj ? T(-1) : T(1);
-    }
-    // buf[v][k] = sum over the covering patch rows of A + iB (thread-owned
-    // elements: no barrier between the contributions)
-    for (int ci = 0; ci < nc; ++ci) {
-      const cplx<T>* rowp = base + (lst_ix[ci] * L + lst_f[ci]) * L2P;
-#pragma unroll
-      for (int u = 0; u < EPT; ++u) {
+      for (int u = 0; u < TPT; ++u) {
         const int e = tid + u * NT;
-        if (V * L % NT != 0 && e >= V * L) break;
-        const cplx<T> a = offa[u] >= 0 ? rowp[offa[u]] : cplx<T>{T(0), T(0)};
-        const cplx<T> b = offb[u] >= 0 ? rowp[offb[u]] : cplx<T>{T(0), T(0)};
-        const cplx<T> z = {a.re - sg[u] * b.im, sg[u] * a.im + b.re};
-        cplx<T>& slot = buf[(e / L) * VS + e - (e / L) * L];
-        slot = ci == 0 ? z : cadd(slot, z);
+        if (e < L) tw[e] = twr[u];
       }
     }
     __syncthreads();
@@ -472,41 +550,34 @@ __global__ __launch_bounds__(NT) void lp_adj_out(LinGeom g, const cplx<T>* __res
                                                  const T* __restrict__ sky,
                                                  const T* __restrict__ dnu,
                                                  cplx<T>* __restrict__ u) {
-  constexpr int V = OutTile<T, L>::V;
-  constexpr int EPT = OutTile<T, L>::EPT;
+  using OT = OutTile<T, L>;
+  constexpr int V = OT::V, TPT = OT::TPT;
   constexpr int VS = odd_stride(L);
-  constexpr int L2 = L / 2 + 1;
   constexpr int L2P = half_stride(L);
   __shared__ cplx<T> buf[V * VS];
   __shared__ cplx<T> tw[L];
-  __shared__ int lst_ix[MAXN], lst_r[MAXN];
-  __shared__ T lst_w[MAXN];
   __shared__ T tent_s[L];  // w <= L tent weights
-  __shared__ int nc_s;
   const int tid = threadIdx.x;
   const int i = blockIdx.x;
   const int Rp = i + g.h;
+  int lo, nc;
+  cover_range(g, Rp, g.w, lo, nc);
 
-  load_twiddles(tw, twg, L, tid, NT);
-  for (int t = tid; t < g.w; t += NT) tent_s[t] = tent[t];
-  if (tid == 0) {
-    int nc = 0;
-    for (int ix = 0; ix < g.n; ++ix) {
-      const int s = start_x(g, ix);
-      if (Rp >= s && Rp < s + g.w) {
-        lst_ix[nc] = ix;
-        lst_r[nc] = Rp - s;
-        lst_w[nc] = tent[Rp - s];
-        ++nc;
-      }
-    }
-    nc_s = nc;
+  cplx<T> twr[TPT];
+#pragma unroll
+  for (int uu = 0; uu < TPT; ++uu) {
+    const int e = tid + uu * NT;
+    if (e < L) twr[uu] = twg[e];
+  }
+  T tnr[TPT];
+#pragma unroll
+  for (int uu = 0; uu < TPT; ++uu) {
+    const int e = tid + uu * NT;
+    if (e < g.w) tnr[uu] = tent[e];
   }
   T acc[JPT], acc2[JPT], tmp[JPT];
 #pragma unroll
   for (int t = 0; t < JPT; ++t) acc[t] = acc2[t] = T(0);
-  __syncthreads();
-  const int nc = nc_s;
   const int npair = (g.n + 1) / 2;
 
   // EPI 0 with a shared input and EPI 1: the jobs (s, mod, e) of image
@@ -520,7 +591,7 @@ __global__ __launch_bounds__(NT) void lp_adj_out(LinGeom g, const cplx<T>* __res
     jstep = 1;
   }
 
-  // EPI 1: the sky row of this energy, fetched before the spectrum loads.
+  // EPI 1: the sky row of this image, fetched before the spectrum loads.
   T sky_v[JPT];
   if (EPI == 1) {
     const int simg = g.sky_shared ? eb : blockIdx.y;
@@ -531,38 +602,21 @@ __global__ __launch_bounds__(NT) void lp_adj_out(LinGeom g, const cplx<T>* __res
       sky_v[t] = j < g.W ? srow[j] : T(0);
     }
   }
+  bool first = true;
   for (int job = jb; job < je; job += jstep) {
     const cplx<T>* base = R2 + (int64_t)job * g.P * g.w * L2P;
 #pragma unroll
     for (int t = 0; t < JPT; ++t) tmp[t] = T(0);
     for (int q0 = 0; q0 < npair; q0 += V) {
-      int offa[EPT], offb[EPT];
-      T sg[EPT];
+      sum_pair_spectra<T, L, true>(buf, base, g, g.w, Rp, lo, nc, tent, q0, tid);
+      if (first) {
 #pragma unroll
-      for (int uu = 0; uu < EPT; ++uu) {
-        const int e = tid + uu * NT;
-        const int v = e / L, k = e - (e / L) * L;
-        const int q = q0 + v;
-        const bool cj = k > L / 2;
-        const int kk = cj ? L - k : k;
-        const bool ok = (V * L % NT == 0 || e < V * L) && q < npair;
-        offa[uu] = ok ? 2 * q * g.n * g.w * L2P + kk : -1;
-        offb[uu] = ok && 2 * q + 1 < g.n ? (2 * q + 1) * g.n * g.w * L2P + kk : -1;
-        sg[uu] = cj ? T(-1) : T(1);
-      }
-      for (int ci = 0; ci < nc; ++ci) {
-        const cplx<T>* rowp = base + (lst_ix[ci] * g.w + lst_r[ci]) * L2P;
-        const T wr = lst_w[ci];
-#pragma unroll
-        for (int uu = 0; uu < EPT; ++uu) {
+        for (int uu = 0; uu < TPT; ++uu) {
           const int e = tid + uu * NT;
-          if (V * L % NT != 0 && e >= V * L) break;
-          const cplx<T> a = offa[uu] >= 0 ? rowp[offa[uu]] : cplx<T>{T(0), T(0)};
-          const cplx<T> b = offb[uu] >= 0 ? rowp[offb[uu]] : cplx<T>{T(0), T(0)};
-          const cplx<T> z = {wr * (a.re - sg[uu] * b.im), wr * (sg[uu] * a.im + b.re)};
-          cplx<T>& slot = buf[(e / L) * VS + e - (e / L) * L];
-          slot = ci == 0 ? z : cadd(slot, z);
+          if (e < L) tw[e] = twr[uu];
+          if (e < g.w) tent_s[e] = tnr[uu];
         }
+        first = false;
       }
       __syncthreads();
       fft_lds<T, L, V, VS, NT, true>(buf, tw, tid);
