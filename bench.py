@@ -111,7 +111,7 @@ def kernel_bytes(H, E, n, m, M=1):
     (inputs read once, outputs written once; FP32, complex = 8 B)."""
     dx = H // n
     w, L = 2 * dx, 2 * dx + 2 * m
-    L2 = L // 2 + 1
+    L2 = L // 2  # stored half-spectrum columns (DC and Nyquist packed)
     J, P = M * E, n * n
     img = J * H * H * 4
     c = 8

@@ -141,15 +141,17 @@ struct jubik_linpatch {
   void* KT = nullptr;
   void* R1 = nullptr;
   void* R2 = nullptr;
+  void* KB = nullptr;
   size_t ws_bytes = 0, spec_bytes = 0;
   bool ready = false;
-  LinBuffers bufs() const { return LinBuffers{tw, tent, KT, R1, R2}; }
+  LinBuffers bufs() const { return LinBuffers{tw, tent, KT, R1, R2, KB}; }
   void release() {
     dev_free(tw);
     dev_free(tent);
     dev_free(KT);
     dev_free(R1);
     dev_free(R2);
+    dev_free(KB);
   }
 };
 
@@ -302,7 +304,7 @@ int jubik_linpatch_create(jubik_linpatch** plan, int dtype, int64_t H, int64_t W
   g.w = (int)(2 * dx);
   g.m = (int)margin;
   g.L = g.w + 2 * g.m;
-  g.L2 = g.L / 2 + 1;
+  g.L2 = g.L / 2;
   g.limx = g.H + 2 * g.h - g.w;
   g.limy = g.W + 2 * g.h - g.w;
   g.P = g.n * g.n;
@@ -338,6 +340,7 @@ int jubik_linpatch_create(jubik_linpatch** plan, int dtype, int64_t H, int64_t W
   if (!st) st = dev_alloc(&p->KT, p->spec_bytes, "kernel spectra");
   if (!st) st = dev_alloc(&p->R1, std::max(p->ws_bytes, nspec * l2p * g.w * cs), "workspace R1");
   if (!st) st = dev_alloc(&p->R2, p->ws_bytes, "workspace R2");
+  if (!st) st = dev_alloc(&p->KB, nspec * g.L * cs, "packed-column kernel factors");
   if (st) {
     p->release();
     delete p;

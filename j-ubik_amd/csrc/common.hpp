@@ -57,10 +57,17 @@ struct TimedLaunch {
 // margin m to the FFT length L = w + 2m.  Window starts are
 // min(i * d, lim), the clamping of jax.lax.dynamic_slice; the same starts
 // serve the slicing of the image and the overlap-add canvas.
-// Row stride of the half-spectrum arrays (R1, R2, kernel spectra): L/2 + 1
-// complex values rounded up to 16 (128 bytes in FP32), so the 16-column
-// tiles of the column pass are single aligned cache lines.
-__host__ __device__ constexpr int half_stride(int L) { return (L / 2 + 1 + 15) / 16 * 16; }
+// Row stride of the half-spectrum arrays (R1, R2, kernel spectra): the L/2
+// stored complex values (DC and Nyquist packed in slot 0) rounded up to a
+// multiple of 16 (128 bytes in FP32), so the 16-column tiles of the column
+// pass are single aligned cache lines, and to an odd number of such blocks:
+// a column tile walking down the rows then spreads over the HBM channels
+// (an even stride, e.g. 12 lines at L = 384, concentrates it on few of them
+// and measured 10 % slower).  Padding blocks are never read or written.
+__host__ __device__ constexpr int half_stride(int L) {
+  const int q = (L / 2 + 15) / 16;
+  return 16 * (q % 2 ? q : q + 1);
+}
 
 struct LinGeom {
   int H, W;       // sky grid
@@ -70,7 +77,7 @@ struct LinGeom {
   int w;          // window 2d
   int m;          // margin
   int L;          // FFT length w + 2m
-  int L2;         // L / 2 + 1 half-spectrum length
+  int L2;         // L / 2 stored half-spectrum columns (DC and Nyquist packed)
   int limx, limy; // clamp of the window starts: H + 2h - w, W + 2h - w
   int P;          // n * n
   int E;          // energies per module

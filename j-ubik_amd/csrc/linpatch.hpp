@@ -8,9 +8,10 @@ namespace jubik {
 struct LinBuffers {
   const void* tw;    // L twiddles exp(-2 pi i e / L)
   const void* tent;  // w tent weights (jubik/convolve.py:18-25)
-  void* KT;          // prepared spectra, (M, P, Ek) x L2 x L, column-major
+  void* KT;          // prepared spectra, (M, P, Ek) x L x L2p (alpha in column 0)
   void* R1;          // row-pass workspace, S x J x P x L x L2p
   void* R2;          // column-pass workspace, S x J x P x L x L2p
+  void* KB;          // beta of the packed DC/Nyquist column, (M, P, Ek) x L
 };
 
 // FFT lengths L = 2 (H / n) + 2 margin compiled into the library.

@@ -3,11 +3,17 @@
 //
 //   rows : gather two real window rows per complex vector (tent weights and
 //          zero margins applied on the fly), FFT along the row, split the two
-//          half spectra and store them row-major (R1[.][r][k], k < L/2 + 1).
+//          half spectra and store them row-major (R1[.][r][k], k < L/2).  The
+//          real DC and Nyquist terms of a row share slot k = 0 as
+//          (X[0], X[L/2]), so a row is exactly L/2 complex values.
 //   cols : read V adjacent spectrum columns straight into registers, FFT
 //          along the columns, multiply by the prepared kernel spectrum in
 //          registers (conjugated for the adjoint), inverse FFT starting from
-//          the same registers, store row-major (R2[.][f][k]).
+//          the same registers, store row-major (R2[.][f][k]).  The packed
+//          column P = C0 + i CN (DC and Nyquist columns, both real) becomes
+//          Q[f] = alpha[f] P[f] + beta[f] conj(P[-f]) with alpha = (K0 + KN)/2,
+//          beta = (K0 - KN)/2: the product spectra of both real columns,
+//          whose inverse is again the packed pair.
 //   out  : per output image row, rebuild the full spectra of two patches per
 //          complex vector, inverse FFT, and sum the patch rows that cover the
 //          row in a fixed order (gather-form overlap-add, no atomics), with a
@@ -17,7 +23,8 @@
 // Kernel spectra are prepared by the same rows/cols passes from the cut
 // window, with the ifftshift applied as (-1)^(k1+k2) and the 1/(sum L^2)
 // normalisation folded in (convolve.py:104-131 and 162-176; dvol cancels);
-// they are stored KT[spec][f][k].
+// they are stored KT[spec][f][k] with alpha in column 0, and beta in
+// KB[spec][f].
 #pragma once
 #include <type_traits>
 
@@ -39,7 +46,7 @@ __global__ __launch_bounds__(NT) void lp_rows(LinGeom g, const T* __restrict__ s
                                               cplx<T>* __restrict__ R1, T scale, int nrows) {
   constexpr int V = RowTile<T, L>::V;
   constexpr int VS = odd_stride(L);
-  constexpr int L2 = L / 2 + 1;
+  constexpr int LH = L / 2;
   constexpr int L2P = half_stride(L);
   __shared__ cplx<T> buf[V * VS];
   __shared__ cplx<T> tw[L];
@@ -138,14 +145,22 @@ __global__ __launch_bounds__(NT) void lp_rows(LinGeom g, const T* __restrict__ s
   fft_lds<T, L, V, VS, NT, false>(buf, tw, tid);
 
   cplx<T>* out = R1 + r1base * L2P * nrows;
-  for (int e = tid; e < V * L2; e += NT) {
-    const int v = e / L2, k = e - (e / L2) * L2;
+  for (int e = tid; e < V * LH; e += NT) {
+    const int v = e / LH, k = e - (e / LH) * LH;
     const int r0 = 2 * (q0 + v);
     if (r0 >= nrows) continue;
     const cplx<T> zk = buf[v * VS + k];
-    const cplx<T> zn = buf[v * VS + (k == 0 ? 0 : L - k)];
-    const cplx<T> A = {T(0.5) * (zk.re + zn.re), T(0.5) * (zk.im - zn.im)};
-    const cplx<T> B = {T(0.5) * (zk.im + zn.im), T(0.5) * (zn.re - zk.re)};
+    cplx<T> A, B;
+    if (k == 0) {
+      // X[0] and X[L/2] of both real rows are real: packed (X[0], X[L/2]).
+      const cplx<T> zh = buf[v * VS + L / 2];
+      A = {zk.re, zh.re};
+      B = {zk.im, zh.im};
+    } else {
+      const cplx<T> zn = buf[v * VS + L - k];
+      A = {T(0.5) * (zk.re + zn.re), T(0.5) * (zk.im - zn.im)};
+      B = {T(0.5) * (zk.im + zn.im), T(0.5) * (zn.re - zk.re)};
+    }
     out[r0 * L2P + k] = A;
     out[(r0 + 1) * L2P + k] = B;
   }
@@ -169,15 +184,28 @@ __device__ __forceinline__ void stages_mid(cplx<T>* buf, const cplx<T>* tw, int 
   }
 }
 
-// grid (ceil(L2 / V), P, Z).
+// Exchange the forward spectrum of the packed column (tile 0, vector 0)
+// through LDS so each position can read its mirror P[-f].  Called by the
+// whole work-group once every thread has finished reading buf.
+template <class FL, typename T, int L>
+__device__ __forceinline__ void share_packed(cplx<T>* buf, typename FL::Regs& last, int tid) {
+  __syncthreads();
+  FL::for_each(last, tid, [&](int vec, int pos, cplx<T>& val) {
+    if (vec == 0) buf[pos] = val;
+  });
+  __syncthreads();
+}
+
+// grid (ceil(L/2 / V), P, Z).
 template <typename T, int L, int MODE>
 __global__ __launch_bounds__(NT) void lp_cols(LinGeom g, const cplx<T>* __restrict__ R1,
                                               const cplx<T>* __restrict__ KT,
                                               const cplx<T>* __restrict__ twg,
-                                              cplx<T>* __restrict__ out) {
+                                              cplx<T>* __restrict__ out,
+                                              cplx<T>* __restrict__ KB) {
   constexpr int V = Tile<T, L>::V;
   constexpr int VS = odd_stride(L);
-  constexpr int L2 = L / 2 + 1;
+  constexpr int LH = L / 2;
   constexpr int L2P = half_stride(L);
   constexpr int NS = plan_stages(L);
   __shared__ cplx<T> buf[V * VS];
@@ -224,12 +252,12 @@ __global__ __launch_bounds__(NT) void lp_cols(LinGeom g, const cplx<T>* __restri
     const cplx<T>* ks = KT + spec * L2P * L;
     FL::gather(kreg, tid, [&](int vec, int pos) -> cplx<T> {
       const int k = k0 + vec;
-      return k < L2 ? ks[pos * L2P + k] : cplx<T>{T(0), T(0)};
+      return k < LH ? ks[pos * L2P + k] : cplx<T>{T(0), T(0)};
     });
   }
   auto load_col = [&](int vec, int pos) -> cplx<T> {
     const int k = k0 + vec;
-    if (k >= L2) return cplx<T>{T(0), T(0)};
+    if (k >= LH) return cplx<T>{T(0), T(0)};
     if constexpr (MODE == COL_ADJ) {
       return in[pos * L2P + k];
     } else {
@@ -251,11 +279,23 @@ __global__ __launch_bounds__(NT) void lp_cols(LinGeom g, const cplx<T>* __restri
     FL::compute(last, tw, tid);
   }
 
+  if (k0 == 0) share_packed<FL, T, L>(buf, last, tid);
   if constexpr (MODE == COL_KERNEL) {
     cplx<T>* kt = out + spec * L2P * L;
     FL::emit(last, tid, [&](int vec, int pos, cplx<T> val) {
       const int k = k0 + vec;
-      if (k < L2) {
+      if (k >= LH) return;
+      if (k == 0) {
+        // K0 = (P + conj P-)/2, KN = (P - conj P-)/(2i), each with its
+        // ifftshift sign (-1)^(f + k) and the normalisation.
+        const cplx<T> pm = buf[pos == 0 ? 0 : L - pos];
+        const T s0 = (pos & 1) ? -factor : factor;
+        const T sn = ((pos + LH) & 1) ? -factor : factor;
+        const cplx<T> k0v = {T(0.5) * s0 * (val.re + pm.re), T(0.5) * s0 * (val.im - pm.im)};
+        const cplx<T> knv = {T(0.5) * sn * (val.im + pm.im), T(0.5) * sn * (pm.re - val.re)};
+        kt[pos * L2P] = {T(0.5) * (k0v.re + knv.re), T(0.5) * (k0v.im + knv.im)};
+        KB[spec * L + pos] = {T(0.5) * (k0v.re - knv.re), T(0.5) * (k0v.im - knv.im)};
+      } else {
         const T sgn = ((k + pos) & 1) ? -factor : factor;
         kt[pos * L2P + k] = cscale(val, sgn);
       }
@@ -267,6 +307,24 @@ __global__ __launch_bounds__(NT) void lp_cols(LinGeom g, const cplx<T>* __restri
 #pragma unroll
       for (int t = 0; t < FL::R; ++t)
         last[b][t] = (MODE == COL_ADJ) ? cmulc(last[b][t], kreg[b][t]) : cmul(last[b][t], kreg[b][t]);
+    if (k0 == 0) {
+      // Packed column: alpha P is done above; add beta conj(P[-f]) (the
+      // conjugated factor for the adjoint).  buf holds the raw P.
+#pragma unroll
+      for (int b = 0; b < FL::BPT; ++b) {
+        int vec, j;
+        if (FL::task(tid, b, vec, j) && vec == 0) {
+#pragma unroll
+          for (int t = 0; t < FL::R; ++t) {
+            const int pos = j + t * FL::NB;
+            const cplx<T> pm = buf[pos == 0 ? 0 : L - pos];
+            const cplx<T> pc = {pm.re, -pm.im};
+            const cplx<T> be = KB[spec * L + pos];
+            last[b][t] = cadd(last[b][t], (MODE == COL_ADJ) ? cmulc(pc, be) : cmul(pc, be));
+          }
+        }
+      }
+    }
     using I0 = Stage<T, L, true, 0, V, VS, NT>;
     using IL = Stage<T, L, true, NS - 1, V, VS, NT>;
     static_assert(I0::R == FL::R && I0::BPT == FL::BPT, "shared register layout");
@@ -274,7 +332,7 @@ __global__ __launch_bounds__(NT) void lp_cols(LinGeom g, const cplx<T>* __restri
     cplx<T>* o = out + idx * L2P * (MODE == COL_ADJ ? w : L);
     auto store = [&](int vec, int pos, cplx<T> val) {
       const int k = k0 + vec;
-      if (k >= L2) return;
+      if (k >= LH) return;
       if constexpr (MODE == COL_ADJ) {
         const int r = pos - m;
         if (r >= 0 && r < w) o[r * L2P + k] = val;
@@ -311,8 +369,8 @@ __global__ __launch_bounds__(NT) void lp_cols(LinGeom g, const cplx<T>* __restri
 template <typename T, int L>
 struct OutTile {
   static constexpr int V = 4;  // iy pairs per pass: n <= 8 in one pass
-  static constexpr int L2 = L / 2 + 1;
-  static constexpr int GPT = (V * L2 + NT - 1) / NT;  // gather tasks (v, kk) per thread
+  static constexpr int LH = L / 2;
+  static constexpr int GPT = (V * LH + NT - 1) / NT;  // gather tasks (v, kk) per thread
   static constexpr int TPT = (L + NT - 1) / NT;       // twiddles per thread
 };
 
@@ -340,10 +398,10 @@ __device__ __forceinline__ void cover_range(const LinGeom& g, int R, int len, in
 // buf[v][k] for v < V, k < L: the sum over the covering patch rows ci of
 // wt_ci * (A_ci(k) + i B_ci(k)), where A and B are the full spectra of patch
 // columns iy = 2(q0 + v) and 2(q0 + v) + 1 rebuilt from their half spectra
-// (X(L - kk) = conj X(kk)).  Patch (ix, iy) row f starts at
-// base + ((iy n + ix) nrows + f) L2p.  Each half-spectrum element is loaded
-// once and fills k = kk and k = L - kk; the loads of up to three covering
-// rows are in flight together.  Weights are tent[f] when WEIGHTED.
+// (X(L - kk) = conj X(kk); slot 0 holds the real pair (X[0], X[L/2])).
+// Patch (ix, iy) row f starts at base + ((iy n + ix) nrows + f) L2p.  Each
+// half-spectrum element is loaded once and fills k = kk and k = L - kk; the
+// loads of up to three covering rows are in flight together.  Weights are tent[f] when WEIGHTED.
 // Patch offsets (2q + 1) n nrows L2p stay below 2^31 for n <= 64, L <= 384.
 template <typename T, int L, bool WEIGHTED>
 __device__ __forceinline__ void sum_pair_spectra(cplx<T>* buf, const cplx<T>* __restrict__ base,
@@ -351,7 +409,7 @@ __device__ __forceinline__ void sum_pair_spectra(cplx<T>* buf, const cplx<T>* __
                                                  int nc, const T* __restrict__ tent, int q0,
                                                  int tid) {
   using OT = OutTile<T, L>;
-  constexpr int V = OT::V, L2 = OT::L2, GPT = OT::GPT;
+  constexpr int V = OT::V, LH = OT::LH, GPT = OT::GPT;
   constexpr int VS = odd_stride(L);
   constexpr int L2P = half_stride(L);
   constexpr int CH = 3;  // covering rows per load batch (L <= 3d in the common geometries)
@@ -361,9 +419,9 @@ __device__ __forceinline__ void sum_pair_spectra(cplx<T>* buf, const cplx<T>* __
 #pragma unroll
   for (int u = 0; u < GPT; ++u) {
     const int e = tid + u * NT;
-    const int v = e / L2, kk = e - (e / L2) * L2;
+    const int v = e / LH, kk = e - (e / LH) * LH;
     const int q = q0 + v;
-    const bool ok = (V * L2 % NT == 0 || e < V * L2) && q < npair;
+    const bool ok = (V * LH % NT == 0 || e < V * LH) && q < npair;
     oa[u] = ok ? 2 * q * g.n * pstride + kk : -1;
     ob[u] = ok && 2 * q + 1 < g.n ? (2 * q + 1) * g.n * pstride + kk : -1;
   }
@@ -406,11 +464,16 @@ __device__ __forceinline__ void sum_pair_spectra(cplx<T>* buf, const cplx<T>* __
 #pragma unroll
   for (int u = 0; u < GPT; ++u) {
     const int e = tid + u * NT;
-    if (V * L2 % NT != 0 && e >= V * L2) break;
-    const int v = e / L2, kk = e - (e / L2) * L2;
+    if (V * LH % NT != 0 && e >= V * LH) break;
+    const int v = e / LH, kk = e - (e / LH) * LH;
     cplx<T>* dst = buf + v * VS;
-    dst[kk] = cplx<T>{A[u].re - B[u].im, A[u].im + B[u].re};
-    if (kk > 0 && 2 * kk < L) dst[L - kk] = cplx<T>{A[u].re + B[u].im, B[u].re - A[u].im};
+    if (kk == 0) {
+      dst[0] = cplx<T>{A[u].re, B[u].re};
+      dst[LH] = cplx<T>{A[u].im, B[u].im};
+    } else {
+      dst[kk] = cplx<T>{A[u].re - B[u].im, A[u].im + B[u].re};
+      dst[L - kk] = cplx<T>{A[u].re + B[u].im, B[u].re - A[u].im};
+    }
   }
 }
 
@@ -690,7 +753,7 @@ template <typename T, int L>
 struct LinOps {
   static constexpr int V = Tile<T, L>::V;
   static constexpr int VR = RowTile<T, L>::V;
-  static constexpr int L2 = L / 2 + 1;
+  static constexpr int LH = L / 2;
   using C = cplx<T>;
 
   static int prepare(const LinGeom& g, const LinBuffers& b, const void* kernel, int layout,
@@ -708,11 +771,11 @@ struct LinOps {
         lp_rows<T, L, ROW_KERNEL_WIN><<<gr, NT, 0, s>>>(g, (const T*)kernel, (const T*)b.tent,
                                                         (const C*)b.tw, (C*)b.R1, T(1), g.w);
       }
-    dim3 gc((L2 + V - 1) / V, g.P, g.M * g.Ek);
+    dim3 gc((LH + V - 1) / V, g.P, g.M * g.Ek);
     {
       TimedLaunch _t("lp_cols_kernel", s);
       lp_cols<T, L, COL_KERNEL><<<gc, NT, 0, s>>>(g, (const C*)b.R1, nullptr, (const C*)b.tw,
-                                                  (C*)b.KT);
+                                                  (C*)b.KT, (C*)b.KB);
     }
     return launch_status("linpatch_prepare");
   }
@@ -725,11 +788,11 @@ struct LinOps {
       lp_rows<T, L, ROW_SIGNAL><<<gr, NT, 0, s>>>(g, (const T*)x, (const T*)b.tent, (const C*)b.tw,
                                                   (C*)b.R1, T(scale), g.w);
     }
-    dim3 gc((L2 + V - 1) / V, g.P, g.S * g.J);
+    dim3 gc((LH + V - 1) / V, g.P, g.S * g.J);
     {
       TimedLaunch _t("lp_cols_fwd", s);
       lp_cols<T, L, COL_FWD><<<gc, NT, 0, s>>>(g, (const C*)b.R1, (const C*)b.KT, (const C*)b.tw,
-                                               (C*)b.R2);
+                                               (C*)b.R2, (C*)b.KB);
     }
   }
 
@@ -740,11 +803,11 @@ struct LinOps {
       lp_rows<T, L, ROW_COTANGENT><<<gr, NT, 0, s>>>(g, (const T*)gy, (const T*)b.tent,
                                                      (const C*)b.tw, (C*)b.R1, T(1), L);
     }
-    dim3 gc((L2 + V - 1) / V, g.P, g.S * g.J);
+    dim3 gc((LH + V - 1) / V, g.P, g.S * g.J);
     {
       TimedLaunch _t("lp_cols_adj", s);
       lp_cols<T, L, COL_ADJ><<<gc, NT, 0, s>>>(g, (const C*)b.R1, (const C*)b.KT, (const C*)b.tw,
-                                               (C*)b.R2);
+                                               (C*)b.R2, (C*)b.KB);
     }
   }
 
