@@ -127,7 +127,6 @@ def kernel_bytes(H, E, n, m, M=1):
         "lp_rows_fwd": E * H * H * 4 + r1_fwd,
         "lp_cols_fwd": r1_fwd + kt + r2_fwd,
         "lp_fwd_out_poisson": r2_fwd + 4 * img,
-        "reduce_partials": J * H * 8,
         "lp_rows_adj": img + r1_adj,
         "lp_cols_adj": r1_adj + kt + r2_adj,
         "lp_adj_out_gp": r2_adj + E * H * H * 4 + gp,
@@ -236,6 +235,20 @@ def main():
         return
 
     t_mixed = time_steps(torch, step, args.steps, args.warmup)
+
+    # the same evaluation replayed from a captured HIP graph
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        step()
+    torch.cuda.current_stream().wait_stream(side)
+    torch.cuda.synchronize()
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph):
+        step()
+    t_graph = time_steps(torch, graph.replay, args.steps, args.warmup)
+    del graph
+
     nll32, g32r, g32a = mixed.nll_and_grad_device(*xi32)
     torch.cuda.synchronize()
     g32 = np.concatenate([g32r.double().cpu().numpy().ravel(), g32a.double().cpu().numpy().ravel()])
@@ -336,6 +349,7 @@ def main():
         "plain_fp64": plain,
         "speedup_vs_plain_fp64": (value / best_plain["evals_per_s"]) if best_plain else None,
         "grad_err_vs_cpu_oracle": grad_vs_oracle,
+        "hip_graph": {"ms_per_step": t_graph * 1e3, "evals_per_s": 1.0 / t_graph},
         "kernels": kernels,
     }
     print(json.dumps(line))

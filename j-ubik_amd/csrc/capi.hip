@@ -642,10 +642,13 @@ int model_grad(jubik_model* model, int S, int slot0, const void* amp_ref, const 
                                 model->counts, model->resid, model->lam_slot(slot0),
                                 (double*)model->partial, s);
   if (st) return st;
-  st = reduce_partials((const double*)model->partial, (int64_t)g.J * g.H, nll, s, S);
-  if (st) return st;
+  NllFinal fin;
+  fin.partial = (const double*)model->partial;
+  fin.seglen = (int64_t)g.J * g.H;
+  fin.nseg = S;
+  fin.out = nll;
   st = linpatch_adjoint_gp(lp->dtype, g, model->lbufs(), model->resid, model->pixel_area, sky, dnu,
-                           model->u, s);
+                           model->u, s, fin);
   if (st) return st;
   gg.uplanes = g.E;
   return gp_adjoint_u(gp->dtype, gg, model->gbufs(), amp_ref, amp_alpha, model->u, g_ref, g_alpha,

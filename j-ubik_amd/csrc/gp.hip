@@ -40,21 +40,49 @@ __global__ __launch_bounds__(GNT) void gp_cols(GpGeom g, const T* __restrict__ a
   if (xi_alpha) xi_alpha += smp * N * N;
   if (u) u += smp * g.uplanes * g.H * g.W;
   Tw += smp * N * N;
-  load_twiddles(tw, twg, N, tid, GNT);
-  for (int e = tid; e < V * N; e += GNT) {
-    const int x1 = e / V, v = e - (e / V) * V;
-    const int x2 = c0 + v;
-    cplx<T> val = {T(0), T(0)};
-    if constexpr (MODE == 0) {
-      const int64_t o = (int64_t)x1 * N + x2;
-      val.re = amp_ref[o] * xi_ref[o];
-      if (g.has_alpha) val.im = amp_alpha[o] * xi_alpha[o];
-    } else {
-      if (x1 < g.H && x2 < g.W)
-        for (int pl = 0; pl < g.uplanes; ++pl)
-          val = cadd(val, u[((int64_t)pl * g.H + x1) * g.W + x2]);
+  // Loads are issued in unrolled chunks of CH elements per thread (their
+  // latencies overlap), twiddles first.
+  constexpr int TPT = (N + GNT - 1) / GNT;
+  cplx<T> twr[TPT];
+#pragma unroll
+  for (int u2 = 0; u2 < TPT; ++u2) {
+    const int e = tid + u2 * GNT;
+    if (e < N) twr[u2] = twg[e];
+  }
+  constexpr int EPT = (V * N + GNT - 1) / GNT;
+  constexpr int CH = (MODE == 0 ? 16 : 8) / (sizeof(T) / 4);
+  constexpr int CHK = EPT < CH ? EPT : CH;
+  for (int u0 = 0; u0 < EPT; u0 += CHK) {
+    cplx<T> val[CHK];
+#pragma unroll
+    for (int c = 0; c < CHK; ++c) {
+      const int e = tid + (u0 + c) * GNT;
+      const int x1 = e / V, v = e - (e / V) * V;
+      const int x2 = c0 + v;
+      val[c] = cplx<T>{T(0), T(0)};
+      if (e >= V * N) continue;
+      if constexpr (MODE == 0) {
+        const int64_t o = (int64_t)x1 * N + x2;
+        val[c].re = amp_ref[o] * xi_ref[o];
+        if (g.has_alpha) val[c].im = amp_alpha[o] * xi_alpha[o];
+      } else {
+        if (x1 < g.H && x2 < g.W)
+          for (int pl = 0; pl < g.uplanes; ++pl)
+            val[c] = cadd(val[c], u[((int64_t)pl * g.H + x1) * g.W + x2]);
+      }
     }
-    buf[v * VS + x1] = val;
+#pragma unroll
+    for (int c = 0; c < CHK; ++c) {
+      const int e = tid + (u0 + c) * GNT;
+      if (e >= V * N) continue;
+      const int x1 = e / V, v = e - (e / V) * V;
+      buf[v * VS + x1] = val[c];
+    }
+  }
+#pragma unroll
+  for (int u2 = 0; u2 < TPT; ++u2) {
+    const int e = tid + u2 * GNT;
+    if (e < N) tw[e] = twr[u2];
   }
   __syncthreads();
   fft_lds<T, N, V, VS, GNT, false>(buf, tw, tid);
@@ -92,10 +120,34 @@ __global__ __launch_bounds__(GNT) void gp_rows(GpGeom g, const cplx<T>* __restri
     out_a += smp * g.E * g.H * g.W;
     if (MODE == 2 && !g.sky_shared) sky_in += smp * g.E * g.H * g.W;
   }
-  load_twiddles(tw, twg, N, tid, GNT);
-  for (int e = tid; e < 2 * N; e += GNT) {
-    const int v = e / N, k = e - (e / N) * N;
-    buf[v * VS + k] = Tw[(int64_t)(v ? r1 : r0) * N + k];
+  constexpr int TPT = (N + GNT - 1) / GNT;
+  constexpr int EPT = (2 * N + GNT - 1) / GNT;
+  cplx<T> twr[TPT], rv[EPT];
+#pragma unroll
+  for (int u2 = 0; u2 < EPT; ++u2) {
+    const int e = tid + u2 * GNT;
+    if (e < 2 * N) {
+      const int v = e / N, k = e - (e / N) * N;
+      rv[u2] = Tw[(int64_t)(v ? r1 : r0) * N + k];
+    }
+  }
+#pragma unroll
+  for (int u2 = 0; u2 < TPT; ++u2) {
+    const int e = tid + u2 * GNT;
+    if (e < N) twr[u2] = twg[e];
+  }
+#pragma unroll
+  for (int u2 = 0; u2 < EPT; ++u2) {
+    const int e = tid + u2 * GNT;
+    if (e < 2 * N) {
+      const int v = e / N, k = e - (e / N) * N;
+      buf[v * VS + k] = rv[u2];
+    }
+  }
+#pragma unroll
+  for (int u2 = 0; u2 < TPT; ++u2) {
+    const int e = tid + u2 * GNT;
+    if (e < N) tw[e] = twr[u2];
   }
   __syncthreads();
   fft_lds<T, N, 2, VS, GNT, false>(buf, tw, tid);

@@ -19,7 +19,7 @@ template <typename T>
 int lp_adjoint_t(const LinGeom&, const LinBuffers&, const void*, void*, double, hipStream_t);
 template <typename T>
 int lp_adjoint_gp_t(const LinGeom&, const LinBuffers&, const void*, double, const void*,
-                    const void*, void*, hipStream_t);
+                    const void*, void*, hipStream_t, NllFinal);
 
 bool linpatch_length_supported(int L) {
   switch (L) {
@@ -67,9 +67,10 @@ int linpatch_adjoint(int dtype, const LinGeom& g, const LinBuffers& b, const voi
   return fail(kInvalidArgument, "dtype must be 0 (float32) or 1 (float64)");
 }
 int linpatch_adjoint_gp(int dtype, const LinGeom& g, const LinBuffers& b, const void* gy,
-                        double scale, const void* sky, const void* dnu, void* u, hipStream_t s) {
-  if (dtype == 0) return lp_adjoint_gp_t<float>(g, b, gy, scale, sky, dnu, u, s);
-  if (dtype == 1) return lp_adjoint_gp_t<double>(g, b, gy, scale, sky, dnu, u, s);
+                        double scale, const void* sky, const void* dnu, void* u, hipStream_t s,
+                        NllFinal fin) {
+  if (dtype == 0) return lp_adjoint_gp_t<float>(g, b, gy, scale, sky, dnu, u, s, fin);
+  if (dtype == 1) return lp_adjoint_gp_t<double>(g, b, gy, scale, sky, dnu, u, s, fin);
   return fail(kInvalidArgument, "dtype must be 0 (float32) or 1 (float64)");
 }
 

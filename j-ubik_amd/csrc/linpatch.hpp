@@ -41,9 +41,20 @@ int linpatch_forward_metric(int dtype, const LinGeom& g, const LinBuffers& b, co
 // over modules) else (J, H, W).
 int linpatch_adjoint(int dtype, const LinGeom& g, const LinBuffers& b, const void* gy, void* gx,
                      double scale, hipStream_t s);
+// NLL finalisation folded into the adjoint row pass, which starts after the
+// forward has written every partial: out[b] = sum of partial[b * seglen ..
+// (b + 1) * seglen) for b < nseg, in a fixed order (no extra launch).
+struct NllFinal {
+  const double* partial = nullptr;
+  int64_t seglen = 0;
+  int nseg = 0;
+  double* out = nullptr;
+};
+
 // Adjoint chained into the GP: u = sum_jobs scale * sky[e] * PSF^T(gy)_job *
-// (1, dnu[e]) stored as complex (H, W).
+// (1, dnu[e]) stored as complex (H, W); optionally finalises the NLL.
 int linpatch_adjoint_gp(int dtype, const LinGeom& g, const LinBuffers& b, const void* gy,
-                        double scale, const void* sky, const void* dnu, void* u, hipStream_t s);
+                        double scale, const void* sky, const void* dnu, void* u, hipStream_t s,
+                        NllFinal fin = NllFinal{});
 
 }  // namespace jubik

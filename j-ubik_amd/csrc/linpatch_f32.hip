@@ -13,5 +13,5 @@ template int lp_forward_metric_t<float>(const LinGeom&, const LinBuffers&, const
 template int lp_adjoint_t<float>(const LinGeom&, const LinBuffers&, const void*, void*, double,
                                hipStream_t);
 template int lp_adjoint_gp_t<float>(const LinGeom&, const LinBuffers&, const void*, double,
-                                  const void*, const void*, void*, hipStream_t);
+                                  const void*, const void*, void*, hipStream_t, NllFinal);
 }  // namespace jubik

@@ -43,7 +43,8 @@ template <typename T, int L, int MODE>
 __global__ __launch_bounds__(NT) void lp_rows(LinGeom g, const T* __restrict__ src,
                                               const T* __restrict__ tent,
                                               const cplx<T>* __restrict__ twg,
-                                              cplx<T>* __restrict__ R1, T scale, int nrows) {
+                                              cplx<T>* __restrict__ R1, T scale, int nrows,
+                                              NllFinal fin) {
   constexpr int V = RowTile<T, L>::V;
   constexpr int VS = odd_stride(L);
   constexpr int LH = L / 2;
@@ -163,6 +164,19 @@ __global__ __launch_bounds__(NT) void lp_rows(LinGeom g, const T* __restrict__ s
     }
     out[r0 * L2P + k] = A;
     out[(r0 + 1) * L2P + k] = B;
+  }
+  if constexpr (MODE == ROW_COTANGENT) {
+    // The first nseg work-groups also finalise the NLL of the forward pass
+    // that wrote the partials (same order as a one-block reduction).
+    const int lin = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+    if (fin.out && lin < fin.nseg) {
+      __shared__ double red[NT / 64];
+      const double* pp = fin.partial + (int64_t)lin * fin.seglen;
+      double acc = 0.0;
+      for (int64_t k = tid; k < fin.seglen; k += NT) acc += pp[k];
+      acc = block_sum<NT>(acc, red);
+      if (tid == 0) fin.out[lin] = acc;
+    }
   }
 }
 
@@ -763,13 +777,15 @@ struct LinOps {
       {
         TimedLaunch _t("lp_rows_kernel", s);
         lp_rows<T, L, ROW_KERNEL_FULL><<<gr, NT, 0, s>>>(g, (const T*)kernel, (const T*)b.tent,
-                                                         (const C*)b.tw, (C*)b.R1, T(1), g.w);
+                                                         (const C*)b.tw, (C*)b.R1, T(1), g.w,
+                                                         NllFinal{});
       }
     else
       {
         TimedLaunch _t("lp_rows_kernel", s);
         lp_rows<T, L, ROW_KERNEL_WIN><<<gr, NT, 0, s>>>(g, (const T*)kernel, (const T*)b.tent,
-                                                        (const C*)b.tw, (C*)b.R1, T(1), g.w);
+                                                        (const C*)b.tw, (C*)b.R1, T(1), g.w,
+                                                        NllFinal{});
       }
     dim3 gc((LH + V - 1) / V, g.P, g.M * g.Ek);
     {
@@ -786,7 +802,7 @@ struct LinOps {
     {
       TimedLaunch _t("lp_rows_fwd", s);
       lp_rows<T, L, ROW_SIGNAL><<<gr, NT, 0, s>>>(g, (const T*)x, (const T*)b.tent, (const C*)b.tw,
-                                                  (C*)b.R1, T(scale), g.w);
+                                                  (C*)b.R1, T(scale), g.w, NllFinal{});
     }
     dim3 gc((LH + V - 1) / V, g.P, g.S * g.J);
     {
@@ -796,12 +812,13 @@ struct LinOps {
     }
   }
 
-  static void adj_front(const LinGeom& g, const LinBuffers& b, const void* gy, hipStream_t s) {
+  static void adj_front(const LinGeom& g, const LinBuffers& b, const void* gy, hipStream_t s,
+                        NllFinal fin = NllFinal{}) {
     dim3 gr((L / 2 + VR - 1) / VR, g.P, g.S * g.J);
     {
       TimedLaunch _t("lp_rows_adj", s);
       lp_rows<T, L, ROW_COTANGENT><<<gr, NT, 0, s>>>(g, (const T*)gy, (const T*)b.tent,
-                                                     (const C*)b.tw, (C*)b.R1, T(1), L);
+                                                     (const C*)b.tw, (C*)b.R1, T(1), L, fin);
     }
     dim3 gc((LH + V - 1) / V, g.P, g.S * g.J);
     {
@@ -874,8 +891,8 @@ struct LinOps {
   }
 
   static int adjoint_gp(const LinGeom& g, const LinBuffers& b, const void* gy, double scale,
-                        const void* sky, const void* dnu, void* u, hipStream_t s) {
-    adj_front(g, b, gy, s);
+                        const void* sky, const void* dnu, void* u, hipStream_t s, NllFinal fin) {
+    adj_front(g, b, gy, s, fin);
     adj_out<1>(g, b, g.S * g.E, nullptr, scale, (const T*)sky, (const T*)dnu, (C*)u, s,
                "lp_adj_out_gp");
     return launch_status("linpatch_adjoint_gp");
@@ -936,9 +953,9 @@ int lp_adjoint_t(const LinGeom& g, const LinBuffers& b, const void* gy, void* gx
 }
 template <typename T>
 int lp_adjoint_gp_t(const LinGeom& g, const LinBuffers& b, const void* gy, double scale,
-                    const void* sky, const void* dnu, void* u, hipStream_t s) {
+                    const void* sky, const void* dnu, void* u, hipStream_t s, NllFinal fin) {
   return dispatch_len<LinOps, T>(g.L, [&](auto ops) {
-    return decltype(ops)::adjoint_gp(g, b, gy, scale, sky, dnu, u, s);
+    return decltype(ops)::adjoint_gp(g, b, gy, scale, sky, dnu, u, s, fin);
   });
 }
 

@@ -61,16 +61,14 @@ __global__ __launch_bounds__(RNT) void k_poisson(const T* __restrict__ lam,
   if (threadIdx.x == 0) partial[blockIdx.x] = acc;
 }
 
-// grid (segments): out[b] = sum of partial[b * n .. b * n + n) in a fixed
-// order.
+// out[0] = sum of partial[0 .. n) in a fixed order.
 __global__ __launch_bounds__(RNT) void k_reduce(const double* __restrict__ partial, int64_t n,
                                                 double* __restrict__ out) {
   __shared__ double red[RNT / 64];
-  partial += (int64_t)blockIdx.x * n;
   double acc = 0.0;
   for (int64_t k = threadIdx.x; k < n; k += RNT) acc += partial[k];
   acc = block_sum<RNT>(acc, red);
-  if (threadIdx.x == 0) out[blockIdx.x] = acc;
+  if (threadIdx.x == 0) out[0] = acc;
 }
 
 static int grid_for(int64_t n) {
@@ -131,13 +129,6 @@ int poisson_nll_grad(int dtype, const void* lam, const void* d, int64_t n, doubl
                                              (double*)grad, work);
   k_reduce<<<1, RNT, 0, s>>>(work, blocks, nll);
   return check_hip(hipGetLastError(), "poisson_nll_grad");
-}
-
-int reduce_partials(const double* partial, int64_t n, double* out, hipStream_t s,
-                    int segments) {
-  TimedLaunch t("reduce_partials", s);
-  k_reduce<<<segments, RNT, 0, s>>>(partial, n, out);
-  return check_hip(hipGetLastError(), "reduce_partials");
 }
 
 }  // namespace jubik

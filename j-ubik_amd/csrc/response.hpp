@@ -14,8 +14,5 @@ int readout_scatter(int dtype, const void* v, const int64_t* idx, int64_t nnz, i
                     void* out, hipStream_t s);
 int poisson_nll_grad(int dtype, const void* lam, const void* d, int64_t n, double* nll,
                      void* grad, double* work, hipStream_t s);
-// out[b] = sum of partial[b * n, (b + 1) * n) for b < segments.
-int reduce_partials(const double* partial, int64_t n, double* out, hipStream_t s,
-                    int segments = 1);
 
 }  // namespace jubik
