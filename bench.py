@@ -45,6 +45,8 @@ def parse():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--profile-only", action="store_true",
                     help="only run the mixed gradient steps (for rocprofv3)")
+    ap.add_argument("--samples", type=int, default=8,
+                    help="VI samples per call of the batched gradient leg (0: skip)")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="with --profile-only: first copy a buffer of this many MiB "
                          "(a known-size reference for the HBM byte counters)")
@@ -73,13 +75,14 @@ def build_inputs(H, E, n, m, seed=2):
                 H=H, E=E, n=n, m=m, pixel_area=1.0)
 
 
-def make_model(ja, torch, inp, dtype, counts):
+def make_model(ja, torch, inp, dtype, counts, max_samples=1):
     H, E = inp["H"], inp["E"]
     sky = ja.CorrelatedFieldSky(H, (H, H), inp["amp_ref"], inp["amp_alpha"], zm=inp["zm"],
                                 mu=inp["mu"], dnu=inp["dnu"], dtype=dtype)
     psf = ja.LinpatchOperator((E, H, H), inp["n"], inp["m"], dtype=dtype)
     psf.set_kernel(inp["win"], layout="window")
-    return ja.XrayModel(sky, psf, inp["pixel_area"], inp["expo"][None], counts[None])
+    return ja.XrayModel(sky, psf, inp["pixel_area"], inp["expo"][None], counts[None],
+                        max_samples=max_samples)
 
 
 def draw_counts(ja, torch, inp):
@@ -249,6 +252,19 @@ def main():
     t_graph = time_steps(torch, graph.replay, args.steps, args.warmup)
     del graph
 
+    # S VI-sample positions per call (jubik_model_grad_batch)
+    batched = None
+    if args.samples > 1:
+        S = args.samples
+        bm = make_model(ja, torch, inp, torch.float32, counts, max_samples=S)
+        brng = np.random.default_rng(17)
+        xb = [torch.as_tensor(v[None] + 0.05 * brng.normal(size=(S,) + v.shape),
+                              dtype=torch.float32, device="cuda") for v in inp["xi"]]
+        t_b = time_steps(torch, lambda: bm.nll_and_grad_batch(*xb), max(3, args.steps // S), 2)
+        batched = {"samples": S, "ms_per_call": t_b * 1e3, "evals_per_s": S / t_b}
+        del bm, xb
+        torch.cuda.empty_cache()
+
     nll32, g32r, g32a = mixed.nll_and_grad_device(*xi32)
     torch.cuda.synchronize()
     g32 = np.concatenate([g32r.double().cpu().numpy().ravel(), g32a.double().cpu().numpy().ravel()])
@@ -350,6 +366,7 @@ def main():
         "speedup_vs_plain_fp64": (value / best_plain["evals_per_s"]) if best_plain else None,
         "grad_err_vs_cpu_oracle": grad_vs_oracle,
         "hip_graph": {"ms_per_step": t_graph * 1e3, "evals_per_s": 1.0 / t_graph},
+        "batched_samples": batched,
         "kernels": kernels,
     }
     print(json.dumps(line))

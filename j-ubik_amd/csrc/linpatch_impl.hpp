@@ -210,7 +210,11 @@ __device__ __forceinline__ void share_packed(cplx<T>* buf, typename FL::Regs& la
   __syncthreads();
 }
 
-// grid (ceil(L/2 / V), P, Z).
+// 1-D grid over (sample, column tile, patch, z) with the sample fastest;
+// z < M * Ek spectra (kernel mode) or J jobs per sample.  Work-groups are
+// remapped so that consecutive items run on one XCD: the S samples of a
+// tile follow each other there and read its kernel-spectrum tile from that
+// XCD's L2 after the first.
 template <typename T, int L, int MODE>
 __global__ __launch_bounds__(NT) void lp_cols(LinGeom g, const cplx<T>* __restrict__ R1,
                                               const cplx<T>* __restrict__ KT,
@@ -227,9 +231,15 @@ __global__ __launch_bounds__(NT) void lp_cols(LinGeom g, const cplx<T>* __restri
   __shared__ double red[NT / 64];
   __shared__ double total_s;
   const int tid = threadIdx.x;
-  const int p = blockIdx.y;
-  const int z = blockIdx.z;
-  const int k0 = blockIdx.x * V;
+  constexpr int NTILE = (LH + V - 1) / V;
+  const int nsmp = (MODE == COL_KERNEL) ? 1 : g.S;
+  int item = xcd_tile(blockIdx.x, gridDim.x);
+  const int smp = item % nsmp;
+  item /= nsmp;
+  const int k0 = (item % NTILE) * V;
+  item /= NTILE;
+  const int p = item % g.P;
+  const int z = item / g.P;
 
   int64_t idx, spec;
   if constexpr (MODE == COL_KERNEL) {
@@ -237,7 +247,7 @@ __global__ __launch_bounds__(NT) void lp_cols(LinGeom g, const cplx<T>* __restri
     spec = ((int64_t)mod * g.P + p) * g.Ek + ek;
     idx = spec;
   } else {
-    idx = (int64_t)z * g.P + p;
+    idx = (int64_t)(smp * g.J + z) * g.P + p;
     spec = spec_index(g, z, p);
   }
   const int nin = (MODE == COL_ADJ) ? L : g.w;
@@ -787,7 +797,7 @@ struct LinOps {
                                                         (const C*)b.tw, (C*)b.R1, T(1), g.w,
                                                         NllFinal{});
       }
-    dim3 gc((LH + V - 1) / V, g.P, g.M * g.Ek);
+    const int gc = (LH + V - 1) / V * g.P * g.M * g.Ek;
     {
       TimedLaunch _t("lp_cols_kernel", s);
       lp_cols<T, L, COL_KERNEL><<<gc, NT, 0, s>>>(g, (const C*)b.R1, nullptr, (const C*)b.tw,
@@ -804,7 +814,7 @@ struct LinOps {
       lp_rows<T, L, ROW_SIGNAL><<<gr, NT, 0, s>>>(g, (const T*)x, (const T*)b.tent, (const C*)b.tw,
                                                   (C*)b.R1, T(scale), g.w, NllFinal{});
     }
-    dim3 gc((LH + V - 1) / V, g.P, g.S * g.J);
+    const int gc = g.S * ((LH + V - 1) / V) * g.P * g.J;
     {
       TimedLaunch _t("lp_cols_fwd", s);
       lp_cols<T, L, COL_FWD><<<gc, NT, 0, s>>>(g, (const C*)b.R1, (const C*)b.KT, (const C*)b.tw,
@@ -820,7 +830,7 @@ struct LinOps {
       lp_rows<T, L, ROW_COTANGENT><<<gr, NT, 0, s>>>(g, (const T*)gy, (const T*)b.tent,
                                                      (const C*)b.tw, (C*)b.R1, T(1), L, fin);
     }
-    dim3 gc((LH + V - 1) / V, g.P, g.S * g.J);
+    const int gc = g.S * ((LH + V - 1) / V) * g.P * g.J;
     {
       TimedLaunch _t("lp_cols_adj", s);
       lp_cols<T, L, COL_ADJ><<<gc, NT, 0, s>>>(g, (const C*)b.R1, (const C*)b.KT, (const C*)b.tw,
