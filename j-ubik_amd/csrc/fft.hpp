@@ -340,6 +340,14 @@ __host__ __device__ constexpr int stage_ns(int L, bool inv, int s) {
 }
 // Odd vector stride (in complex elements) for V vectors of length L.
 __host__ __device__ constexpr int odd_stride(int L) { return L | 1; }
+// LDS vector stride for a tile of V vectors with the vector-fastest task
+// mapping: a 16-lane group of 8-byte accesses then holds min(V, 16) vectors
+// times 16 / V consecutive positions, which land in distinct bank slots
+// when the stride is congruent to 16 / V modulo 16 (odd for V >= 16).
+__host__ __device__ constexpr int lds_stride(int L, int V) {
+  const int r = V >= 16 ? 1 : 16 / V;
+  return L + ((r - L % 16) % 16 + 16) % 16;
+}
 
 // ---------------------------------------------------------------- stages
 template <typename T, int L, bool INV, int S, int V, int VS, int NT>

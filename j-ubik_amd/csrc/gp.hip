@@ -30,7 +30,7 @@ __global__ __launch_bounds__(GNT) void gp_cols(GpGeom g, const T* __restrict__ a
                                                const cplx<T>* __restrict__ twg,
                                                cplx<T>* __restrict__ Tw) {
   constexpr int V = Tile<T, N>::V;
-  constexpr int VS = odd_stride(N);
+  constexpr int VS = lds_stride(N, V);
   __shared__ cplx<T> buf[V * VS];
   __shared__ cplx<T> tw[N];
   const int tid = threadIdx.x;
@@ -105,7 +105,7 @@ __global__ __launch_bounds__(GNT) void gp_rows(GpGeom g, const cplx<T>* __restri
                                                const T* __restrict__ amp_alpha,
                                                const T* __restrict__ sky_in,
                                                T* __restrict__ out_a, T* __restrict__ out_b) {
-  constexpr int VS = odd_stride(N);
+  constexpr int VS = lds_stride(N, 2);
   __shared__ cplx<T> buf[2 * VS];
   __shared__ cplx<T> tw[N];
   const int tid = threadIdx.x;

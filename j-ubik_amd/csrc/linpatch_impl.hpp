@@ -46,7 +46,7 @@ __global__ __launch_bounds__(NT) void lp_rows(LinGeom g, const T* __restrict__ s
                                               cplx<T>* __restrict__ R1, T scale, int nrows,
                                               NllFinal fin) {
   constexpr int V = RowTile<T, L>::V;
-  constexpr int VS = odd_stride(L);
+  constexpr int VS = lds_stride(L, V);
   constexpr int LH = L / 2;
   constexpr int L2P = half_stride(L);
   __shared__ cplx<T> buf[V * VS];
@@ -222,7 +222,7 @@ __global__ __launch_bounds__(NT) void lp_cols(LinGeom g, const cplx<T>* __restri
                                               cplx<T>* __restrict__ out,
                                               cplx<T>* __restrict__ KB) {
   constexpr int V = Tile<T, L>::V;
-  constexpr int VS = odd_stride(L);
+  constexpr int VS = lds_stride(L, V);
   constexpr int LH = L / 2;
   constexpr int L2P = half_stride(L);
   constexpr int NS = plan_stages(L);
@@ -434,7 +434,7 @@ __device__ __forceinline__ void sum_pair_spectra(cplx<T>* buf, const cplx<T>* __
                                                  int tid) {
   using OT = OutTile<T, L>;
   constexpr int V = OT::V, LH = OT::LH, GPT = OT::GPT;
-  constexpr int VS = odd_stride(L);
+  constexpr int VS = lds_stride(L, V);
   constexpr int L2P = half_stride(L);
   constexpr int CH = 3;  // covering rows per load batch (L <= 3d in the common geometries)
   const int npair = (g.n + 1) / 2;
@@ -512,7 +512,7 @@ __global__ __launch_bounds__(NT) void lp_fwd_out(LinGeom g, const cplx<T>* __res
                                                  double* __restrict__ partial) {
   using OT = OutTile<T, L>;
   constexpr int V = OT::V, TPT = OT::TPT;
-  constexpr int VS = odd_stride(L);
+  constexpr int VS = lds_stride(L, V);
   constexpr int L2P = half_stride(L);
   __shared__ cplx<T> buf[V * VS];
   __shared__ cplx<T> tw[L];
@@ -639,7 +639,7 @@ __global__ __launch_bounds__(NT) void lp_adj_out(LinGeom g, const cplx<T>* __res
                                                  cplx<T>* __restrict__ u) {
   using OT = OutTile<T, L>;
   constexpr int V = OT::V, TPT = OT::TPT;
-  constexpr int VS = odd_stride(L);
+  constexpr int VS = lds_stride(L, V);
   constexpr int L2P = half_stride(L);
   __shared__ cplx<T> buf[V * VS];
   __shared__ cplx<T> tw[L];
