@@ -41,9 +41,9 @@ int linpatch_forward_metric(int dtype, const LinGeom& g, const LinBuffers& b, co
 // over modules) else (J, H, W).
 int linpatch_adjoint(int dtype, const LinGeom& g, const LinBuffers& b, const void* gy, void* gx,
                      double scale, hipStream_t s);
-// NLL finalisation folded into the adjoint row pass, which starts after the
-// forward has written every partial: out[b] = sum of partial[b * seglen ..
-// (b + 1) * seglen) for b < nseg, in a fixed order (no extra launch).
+// NLL finalisation folded into the adjoint column pass, which starts after
+// the forward has written every partial: out[b] = sum of partial[b * seglen
+// .. (b + 1) * seglen) for b < nseg, in a fixed order (no extra launch).
 struct NllFinal {
   const double* partial = nullptr;
   int64_t seglen = 0;

@@ -43,8 +43,7 @@ template <typename T, int L, int MODE>
 __global__ __launch_bounds__(NT) void lp_rows(LinGeom g, const T* __restrict__ src,
                                               const T* __restrict__ tent,
                                               const cplx<T>* __restrict__ twg,
-                                              cplx<T>* __restrict__ R1, T scale, int nrows,
-                                              NllFinal fin) {
+                                              cplx<T>* __restrict__ R1, T scale, int nrows) {
   constexpr int V = RowTile<T, L>::V;
   constexpr int VS = lds_stride(L, V);
   constexpr int LH = L / 2;
@@ -57,6 +56,13 @@ __global__ __launch_bounds__(NT) void lp_rows(LinGeom g, const T* __restrict__ s
   const int q0 = blockIdx.x * V;
   const int ix = p % g.n, iy = p / g.n;
   const int sx = start_x(g, ix), sy = start_y(g, iy);
+  // Cotangent windows: rows whose image row sx + r - h - m lies outside the
+  // image are zero; the adjoint column pass never reads them, so they are
+  // neither transformed nor stored (the whole tile returns when all are).
+  const int row0 = (MODE == ROW_COTANGENT) ? sx - g.h - g.m : 0;
+  if (MODE == ROW_COTANGENT &&
+      (row0 + 2 * q0 + 2 * V <= 0 || row0 + 2 * q0 >= g.H))
+    return;
 
   int64_t r1base;
   const T* img;
@@ -150,6 +156,8 @@ __global__ __launch_bounds__(NT) void lp_rows(LinGeom g, const T* __restrict__ s
     const int v = e / LH, k = e - (e / LH) * LH;
     const int r0 = 2 * (q0 + v);
     if (r0 >= nrows) continue;
+    const bool st0 = MODE != ROW_COTANGENT || (row0 + r0 >= 0 && row0 + r0 < g.H);
+    const bool st1 = MODE != ROW_COTANGENT || (row0 + r0 + 1 >= 0 && row0 + r0 + 1 < g.H);
     const cplx<T> zk = buf[v * VS + k];
     cplx<T> A, B;
     if (k == 0) {
@@ -162,21 +170,8 @@ __global__ __launch_bounds__(NT) void lp_rows(LinGeom g, const T* __restrict__ s
       A = {T(0.5) * (zk.re + zn.re), T(0.5) * (zk.im - zn.im)};
       B = {T(0.5) * (zk.im + zn.im), T(0.5) * (zn.re - zk.re)};
     }
-    out[r0 * L2P + k] = A;
-    out[(r0 + 1) * L2P + k] = B;
-  }
-  if constexpr (MODE == ROW_COTANGENT) {
-    // The first nseg work-groups also finalise the NLL of the forward pass
-    // that wrote the partials (same order as a one-block reduction).
-    const int lin = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
-    if (fin.out && lin < fin.nseg) {
-      __shared__ double red[NT / 64];
-      const double* pp = fin.partial + (int64_t)lin * fin.seglen;
-      double acc = 0.0;
-      for (int64_t k = tid; k < fin.seglen; k += NT) acc += pp[k];
-      acc = block_sum<NT>(acc, red);
-      if (tid == 0) fin.out[lin] = acc;
-    }
+    if (st0) out[r0 * L2P + k] = A;
+    if (st1) out[(r0 + 1) * L2P + k] = B;
   }
 }
 
@@ -220,7 +215,7 @@ __global__ __launch_bounds__(NT) void lp_cols(LinGeom g, const cplx<T>* __restri
                                               const cplx<T>* __restrict__ KT,
                                               const cplx<T>* __restrict__ twg,
                                               cplx<T>* __restrict__ out,
-                                              cplx<T>* __restrict__ KB) {
+                                              cplx<T>* __restrict__ KB, NllFinal fin) {
   constexpr int V = Tile<T, L>::V;
   constexpr int VS = lds_stride(L, V);
   constexpr int LH = L / 2;
@@ -279,11 +274,14 @@ __global__ __launch_bounds__(NT) void lp_cols(LinGeom g, const cplx<T>* __restri
       return k < LH ? ks[pos * L2P + k] : cplx<T>{T(0), T(0)};
     });
   }
+  // adjoint: window rows outside the image are zero and are never stored
+  const int row0 = start_x(g, p % g.n) - g.h - g.m;
   auto load_col = [&](int vec, int pos) -> cplx<T> {
     const int k = k0 + vec;
     if (k >= LH) return cplx<T>{T(0), T(0)};
     if constexpr (MODE == COL_ADJ) {
-      return in[pos * L2P + k];
+      const int ir = row0 + pos;
+      return (ir >= 0 && ir < g.H) ? in[pos * L2P + k] : cplx<T>{T(0), T(0)};
     } else {
       const int rr = pos - m;
       return (rr >= 0 && rr < w) ? in[rr * L2P + k] : cplx<T>{T(0), T(0)};
@@ -375,6 +373,17 @@ __global__ __launch_bounds__(NT) void lp_cols(LinGeom g, const cplx<T>* __restri
       IL::gather_lds(buf, v, tid);
       IL::compute(v, tw, tid);
       IL::emit(v, tid, store);
+    }
+  }
+  if constexpr (MODE == COL_ADJ) {
+    // The first nseg work-groups also finalise the NLL of the forward pass
+    // that wrote the partials (same order as a one-block reduction).
+    if (fin.out && (int)blockIdx.x < fin.nseg) {
+      const double* pp = fin.partial + (int64_t)blockIdx.x * fin.seglen;
+      double acc = 0.0;
+      for (int64_t k = tid; k < fin.seglen; k += NT) acc += pp[k];
+      acc = block_sum<NT>(acc, red);
+      if (tid == 0) fin.out[blockIdx.x] = acc;
     }
   }
 }
@@ -787,21 +796,19 @@ struct LinOps {
       {
         TimedLaunch _t("lp_rows_kernel", s);
         lp_rows<T, L, ROW_KERNEL_FULL><<<gr, NT, 0, s>>>(g, (const T*)kernel, (const T*)b.tent,
-                                                         (const C*)b.tw, (C*)b.R1, T(1), g.w,
-                                                         NllFinal{});
+                                                         (const C*)b.tw, (C*)b.R1, T(1), g.w);
       }
     else
       {
         TimedLaunch _t("lp_rows_kernel", s);
         lp_rows<T, L, ROW_KERNEL_WIN><<<gr, NT, 0, s>>>(g, (const T*)kernel, (const T*)b.tent,
-                                                        (const C*)b.tw, (C*)b.R1, T(1), g.w,
-                                                        NllFinal{});
+                                                        (const C*)b.tw, (C*)b.R1, T(1), g.w);
       }
     const int gc = (LH + V - 1) / V * g.P * g.M * g.Ek;
     {
       TimedLaunch _t("lp_cols_kernel", s);
       lp_cols<T, L, COL_KERNEL><<<gc, NT, 0, s>>>(g, (const C*)b.R1, nullptr, (const C*)b.tw,
-                                                  (C*)b.KT, (C*)b.KB);
+                                                  (C*)b.KT, (C*)b.KB, NllFinal{});
     }
     return launch_status("linpatch_prepare");
   }
@@ -812,13 +819,13 @@ struct LinOps {
     {
       TimedLaunch _t("lp_rows_fwd", s);
       lp_rows<T, L, ROW_SIGNAL><<<gr, NT, 0, s>>>(g, (const T*)x, (const T*)b.tent, (const C*)b.tw,
-                                                  (C*)b.R1, T(scale), g.w, NllFinal{});
+                                                  (C*)b.R1, T(scale), g.w);
     }
     const int gc = g.S * ((LH + V - 1) / V) * g.P * g.J;
     {
       TimedLaunch _t("lp_cols_fwd", s);
       lp_cols<T, L, COL_FWD><<<gc, NT, 0, s>>>(g, (const C*)b.R1, (const C*)b.KT, (const C*)b.tw,
-                                               (C*)b.R2, (C*)b.KB);
+                                               (C*)b.R2, (C*)b.KB, NllFinal{});
     }
   }
 
@@ -828,13 +835,13 @@ struct LinOps {
     {
       TimedLaunch _t("lp_rows_adj", s);
       lp_rows<T, L, ROW_COTANGENT><<<gr, NT, 0, s>>>(g, (const T*)gy, (const T*)b.tent,
-                                                     (const C*)b.tw, (C*)b.R1, T(1), L, fin);
+                                                     (const C*)b.tw, (C*)b.R1, T(1), L);
     }
     const int gc = g.S * ((LH + V - 1) / V) * g.P * g.J;
     {
       TimedLaunch _t("lp_cols_adj", s);
       lp_cols<T, L, COL_ADJ><<<gc, NT, 0, s>>>(g, (const C*)b.R1, (const C*)b.KT, (const C*)b.tw,
-                                               (C*)b.R2, (C*)b.KB);
+                                               (C*)b.R2, (C*)b.KB, fin);
     }
   }
 
