@@ -190,6 +190,15 @@ int jubik_model_grad_batch(jubik_model* model, int64_t S, const void* amp_ref,
 int jubik_model_metric_batch(jubik_model* model, int64_t S, const void* amp_ref,
                              const void* amp_alpha, const void* dnu, const void* v_ref,
                              const void* v_alpha, void* out_ref, void* out_alpha, void* stream);
+/* Left square root of the likelihood metric at the base point, for S noise
+ * vectors: out = J^T diag(lam^-1/2) noise, noise (S, M, E, H, W) in data
+ * space (ignored where counts < 0).  With noise ~ N(0, 1) this draws from
+ * N(0, J^T diag(1/lam) J): the likelihood part of nifty.re's linear sample
+ * draw (draw_linear_residual, called through optimize_kl at
+ * demos/x-ray-imaging.py:404-430). */
+int jubik_model_sqrt_metric_batch(jubik_model* model, int64_t S, const void* amp_ref,
+                                  const void* amp_alpha, const void* dnu, const void* noise,
+                                  void* out_ref, void* out_alpha, void* stream);
 
 #ifdef __cplusplus
 }

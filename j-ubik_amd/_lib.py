@@ -52,6 +52,7 @@ SIGNATURES = {
     "jubik_model_grad_batch": [_vp, _i64, _vp, _vp, _vp, _vp, _dbl, _dbl, _vp, _vp, _vp, _vp,
                                _vp],
     "jubik_model_metric_batch": [_vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
+    "jubik_model_sqrt_metric_batch": [_vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
 }
 
 _lock = threading.Lock()

@@ -741,6 +741,40 @@ int jubik_model_metric_batch(jubik_model* model, int64_t S, const void* amp_ref,
                       out_alpha, s);
 }
 
+int jubik_model_sqrt_metric_batch(jubik_model* model, int64_t S, const void* amp_ref,
+                                  const void* amp_alpha, const void* dnu, const void* noise,
+                                  void* out_ref, void* out_alpha, void* stream) {
+  if (!model || !noise || !out_ref || !amp_ref) return fail(kInvalidArgument, "NULL argument");
+  if (S < 1 || S > model->max_samples)
+    return fail(kInvalidArgument, "batch of " + std::to_string(S) +
+                                      " samples outside [1, max_samples = " +
+                                      std::to_string(model->max_samples) + "]");
+  if (model->gp->g.has_alpha && (!amp_alpha || !out_alpha || !dnu))
+    return fail(kInvalidArgument, "spectral-index inputs are NULL");
+  if (!model->have_point)
+    return fail(kNotReady, "no base point (call jubik_model_grad first)");
+  jubik_linpatch* lp = model->lp;
+  jubik_gp* gp = model->gp;
+  DeviceGuard dg(lp->device);
+  hipStream_t s = as_stream(stream);
+  LinGeom g = lp->g;
+  g.S = (int)S;
+  g.sky_shared = 1;
+  GpGeom gg = gp->g;
+  gg.S = (int)S;
+  gg.sky_shared = 1;
+  gg.uplanes = g.E;
+  const int64_t n = (int64_t)g.J * g.H * g.W;
+  int st = sqrt_metric_cot(lp->dtype, model->expo, model->counts, model->lam_slot(0), noise,
+                           model->resid, n, (int)S, s);
+  if (st) return st;
+  st = linpatch_adjoint_gp(lp->dtype, g, model->lbufs(), model->resid, model->pixel_area,
+                           model->sky_slot(0), dnu, model->u, s);
+  if (st) return st;
+  return gp_adjoint_u(gp->dtype, gg, model->gbufs(), amp_ref, amp_alpha, model->u, out_ref,
+                      out_alpha, s);
+}
+
 int jubik_model_metric(jubik_model* model, const void* amp_ref, const void* amp_alpha,
                        const void* dnu, const void* v_ref, const void* v_alpha, void* out_ref,
                        void* out_alpha, void* stream) {

@@ -18,6 +18,12 @@ namespace jubik {
 
 constexpr int GNT = 256;
 
+// Columns per work-group of the column pass: the LDS tile width, at most N.
+template <typename T, int N>
+struct GpTile {
+  static constexpr int V = Tile<T, N>::V < N ? Tile<T, N>::V : N;
+};
+
 // grid (N / V, S).  MODE 0: c = amp * xi (forward / tangent);  MODE 1: c =
 // the sum of the uplanes complex (H, W) cotangent planes, zero outside the
 // sky grid.  Sample blockIdx.y reads xi[s] / u[s] and writes work[s].
@@ -29,7 +35,7 @@ __global__ __launch_bounds__(GNT) void gp_cols(GpGeom g, const T* __restrict__ a
                                                const cplx<T>* __restrict__ u,
                                                const cplx<T>* __restrict__ twg,
                                                cplx<T>* __restrict__ Tw) {
-  constexpr int V = Tile<T, N>::V;
+  constexpr int V = GpTile<T, N>::V;
   constexpr int VS = lds_stride(N, V);
   __shared__ cplx<T> buf[V * VS];
   __shared__ cplx<T> tw[N];
@@ -216,7 +222,7 @@ bool gp_length_supported(int N) {
 
 template <typename T, int N>
 struct GpOps {
-  static constexpr int V = Tile<T, N>::V;
+  static constexpr int V = GpTile<T, N>::V;
   using C = cplx<T>;
   static int forward(const GpGeom& g, const GpBuffers& b, const void* amp_ref,
                      const void* amp_alpha, const void* xi_ref, const void* xi_alpha,

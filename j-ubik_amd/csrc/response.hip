@@ -71,6 +71,23 @@ __global__ __launch_bounds__(RNT) void k_reduce(const double* __restrict__ parti
   if (threadIdx.x == 0) out[0] = acc;
 }
 
+// Cotangent of the metric's left square root J^T diag(lam^-1/2) at the base
+// point: cot[s, i] = expo[i] * noise[s, i] / sqrt(lam[i]) where counts[i] >= 0,
+// else 0; i runs over the n = J H W pixels of one sample, s over S samples.
+template <typename T>
+__global__ __launch_bounds__(RNT) void k_sqrt_metric_cot(const T* __restrict__ expo,
+                                                         const int* __restrict__ counts,
+                                                         const T* __restrict__ lam,
+                                                         const T* __restrict__ noise,
+                                                         T* __restrict__ cot, int64_t n,
+                                                         int64_t total) {
+  for (int64_t k = blockIdx.x * (int64_t)RNT + threadIdx.x; k < total;
+       k += (int64_t)gridDim.x * RNT) {
+    const int64_t i = k % n;
+    cot[k] = counts[i] >= 0 ? expo[i] * noise[k] / sqrt(lam[i]) : T(0);
+  }
+}
+
 static int grid_for(int64_t n) {
   int64_t b = (n + RNT - 1) / RNT;
   if (b < 1) b = 1;
@@ -116,6 +133,20 @@ int readout_scatter(int dtype, const void* v, const int64_t* idx, int64_t nnz, i
       k_scatter<double><<<grid_for(nnz), RNT, 0, s>>>((const double*)v, idx, (double*)out, nnz);
   }
   return check_hip(hipGetLastError(), "readout_scatter");
+}
+
+int sqrt_metric_cot(int dtype, const void* expo, const int* counts, const void* lam,
+                    const void* noise, void* cot, int64_t n, int S, hipStream_t s) {
+  const int64_t total = n * S;
+  if (total <= 0) return kOk;
+  if (dtype == 0)
+    k_sqrt_metric_cot<float><<<grid_for(total), RNT, 0, s>>>(
+        (const float*)expo, counts, (const float*)lam, (const float*)noise, (float*)cot, n, total);
+  else
+    k_sqrt_metric_cot<double><<<grid_for(total), RNT, 0, s>>>(
+        (const double*)expo, counts, (const double*)lam, (const double*)noise, (double*)cot, n,
+        total);
+  return check_hip(hipGetLastError(), "sqrt_metric_cot");
 }
 
 int poisson_nll_grad(int dtype, const void* lam, const void* d, int64_t n, double* nll,

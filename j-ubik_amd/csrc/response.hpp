@@ -12,6 +12,10 @@ int readout_gather(int dtype, const void* x, const int64_t* idx, int64_t nnz, vo
                    hipStream_t s);
 int readout_scatter(int dtype, const void* v, const int64_t* idx, int64_t nnz, int64_t n,
                     void* out, hipStream_t s);
+// cot[s, i] = expo[i] noise[s, i] / sqrt(lam[i]) where counts[i] >= 0, else 0
+// (n pixels per sample, S samples).
+int sqrt_metric_cot(int dtype, const void* expo, const int* counts, const void* lam,
+                    const void* noise, void* cot, int64_t n, int S, hipStream_t s);
 int poisson_nll_grad(int dtype, const void* lam, const void* d, int64_t n, double* nll,
                      void* grad, double* work, hipStream_t s);
 

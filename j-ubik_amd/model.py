@@ -156,3 +156,21 @@ class XrayModel:
                   dev.ptr(o_ref), dev.ptr(o_alpha), dev.stream())
         return dev.like_input(o_ref, v_ref), (
             dev.like_input(o_alpha, v_ref) if o_alpha is not None else None)
+
+    def sqrt_metric_batch(self, noise):
+        """J^T diag(lam^-1/2) noise at the last gradient position for S
+        data-space noise vectors of shape (S, M, E, H, W) (entries where the
+        counts are masked are ignored); returns (S, N, N) stacks."""
+        s = self.sky
+        S = self._batch_size(noise)
+        shape = (S,) + tuple(self.exposure.shape)
+        nz = dev.to_device(noise, self.dtype)
+        if tuple(nz.shape) != shape:
+            raise ValueError(f"noise must have shape {shape}, got {tuple(nz.shape)}")
+        o_ref = torch.empty((S, s.N, s.N), dtype=self.dtype, device="cuda")
+        o_alpha = torch.empty_like(o_ref) if s.has_alpha else None
+        _lib.call("jubik_model_sqrt_metric_batch", self._model, S, dev.ptr(s.amp_ref),
+                  dev.ptr(s.amp_alpha), dev.ptr(s.dnu_dev), dev.ptr(nz), dev.ptr(o_ref),
+                  dev.ptr(o_alpha), dev.stream())
+        return dev.like_input(o_ref, noise), (
+            dev.like_input(o_alpha, noise) if o_alpha is not None else None)
