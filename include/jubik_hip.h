@@ -114,6 +114,13 @@ int jubik_readout_gather(int dtype, const void* x, const int64_t* index, int64_t
                          void* stream);
 int jubik_readout_scatter(int dtype, const void* v, const int64_t* index, int64_t nnz, int64_t n,
                           void* out, void* stream);
+/* Mock counts: counts[i] ~ Poisson(lam[i]) for n device values, as the
+ * mock-data generator draws them per read-out key
+ * (jubik/data.py:139-145, jax.random.poisson).  A draw is a pure function of
+ * (seed, i, lam[i]); the generator differs from JAX's, so the counts match
+ * the reference in distribution, not bit for bit. */
+int jubik_poisson_sample(int dtype, const void* lam, int64_t n, uint64_t seed, int32_t* counts,
+                         void* stream);
 /* Poisson NLL of nifty.re's Poissonian(data) as composed at
  * jubik/instruments/chandra/chandra_likelihood.py:37 and
  * jubik/instruments/erosita/erosita_likelihood.py:89:

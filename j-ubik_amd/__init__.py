@@ -14,11 +14,13 @@ from .likelihood import Poissonian
 from .sky import CorrelatedFieldSky, powerlaw_amplitude
 from .model import XrayModel, dense_counts
 from .sampling import conjugate_gradient, draw_linear_samples, metric_solve
+from .mock import create_mock_counts, poisson_sample
 
 __all__ = [
     "Domain", "Vector", "LinpatchOperator", "convolve", "integrate", "linpatch_convolve",
     "slice_patches", "slice_patches_transpose",
     "linpatch_convolve_modules", "build_exposure_function", "build_readout_function",
     "Poissonian", "CorrelatedFieldSky", "powerlaw_amplitude", "XrayModel", "dense_counts",
-    "conjugate_gradient", "draw_linear_samples", "metric_solve",
+    "conjugate_gradient", "draw_linear_samples", "metric_solve", "create_mock_counts",
+    "poisson_sample",
 ]

@@ -53,6 +53,7 @@ SIGNATURES = {
                                _vp],
     "jubik_model_metric_batch": [_vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
     "jubik_model_sqrt_metric_batch": [_vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
+    "jubik_poisson_sample": [_int, _vp, _i64, C.c_uint64, _vp, _vp],
 }
 
 _lock = threading.Lock()

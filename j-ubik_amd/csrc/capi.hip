@@ -479,6 +479,14 @@ int jubik_readout_scatter(int dtype, const void* v, const int64_t* index, int64_
   return readout_scatter(dtype, v, index, nnz, n, out, as_stream(stream));
 }
 
+int jubik_poisson_sample(int dtype, const void* lam, int64_t n, uint64_t seed, int32_t* counts,
+                         void* stream) {
+  if (dtype != 0 && dtype != 1) return fail(kInvalidArgument, "dtype must be 0 or 1");
+  if (n < 0) return fail(kInvalidArgument, "negative size");
+  if (n && (!lam || !counts)) return fail(kInvalidArgument, "NULL argument");
+  return poisson_sample(dtype, lam, n, seed, counts, as_stream(stream));
+}
+
 int jubik_poisson_nll_grad(int dtype, const void* lam, const void* data, int64_t n, double* nll,
                            void* grad, void* work, void* stream) {
   if (dtype != 0 && dtype != 1) return fail(kInvalidArgument, "dtype must be 0 or 1");

@@ -88,6 +88,65 @@ __global__ __launch_bounds__(RNT) void k_sqrt_metric_cot(const T* __restrict__ e
   }
 }
 
+// ---------------------------------------------------------------- Poisson
+// Counter-based uniforms: the SplitMix64 output function (Steele, Lea and
+// Flood, "Fast splittable pseudorandom number generators", OOPSLA 2014)
+// applied to seed + counter * golden gamma; 53-bit doubles in (0, 1).
+__device__ __forceinline__ double uniform01(uint64_t seed, uint64_t ctr) {
+  uint64_t z = seed + (ctr + 1) * 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  z ^= z >> 31;
+  return ((double)(z >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+}
+
+// One Poisson(lam) draw from the uniforms of stream ``base``.  lam < 12:
+// inversion by sequential search of the CDF.  lam >= 12: the transformed
+// rejection method with squeeze (PTRS) of W. Hormann, "The transformed
+// rejection method for generating Poisson random variables", Insurance:
+// Mathematics and Economics 12 (1993) 39-45.  Both loops are bounded; the
+// caps are never reached in practice (probability below 1e-40).
+__device__ __forceinline__ int poisson_draw(double lam, uint64_t seed, uint64_t base) {
+  if (!(lam > 0.0)) return 0;
+  if (lam < 12.0) {
+    const double u = uniform01(seed, base);
+    double p = exp(-lam), F = p;
+    int k = 0;
+    while (u > F && k < 200) {
+      ++k;
+      p *= lam / k;
+      F += p;
+    }
+    return k;
+  }
+  const double slam = sqrt(lam), loglam = log(lam);
+  const double b = 0.931 + 2.53 * slam;
+  const double a = -0.059 + 0.02483 * b;
+  const double invalpha = 1.1239 + 1.1328 / (b - 3.4);
+  const double vr = 0.9277 - 3.6224 / (b - 2.0);
+  for (int it = 0; it < 128; ++it) {
+    const double U = uniform01(seed, base + 2 * it) - 0.5;
+    const double V = uniform01(seed, base + 2 * it + 1);
+    const double us = 0.5 - fabs(U);
+    const double kf = floor((2.0 * a / us + b) * U + lam + 0.43);
+    if (us >= 0.07 && V <= vr) return (int)kf;
+    if (kf < 0.0 || (us < 0.013 && V > us)) continue;
+    if (log(V) + log(invalpha) - log(a / (us * us) + b) <= -lam + kf * loglam - lgamma(kf + 1.0))
+      return (int)kf;
+  }
+  return (int)floor(lam + 0.5);
+}
+
+// counts[i] ~ Poisson(lam[i]); element i uses the uniform stream starting at
+// counter 256 i, so a draw depends only on (seed, i, lam[i]).
+template <typename T>
+__global__ __launch_bounds__(RNT) void k_poisson_draw(const T* __restrict__ lam, int64_t n,
+                                                      uint64_t seed, int* __restrict__ counts) {
+  for (int64_t i = blockIdx.x * (int64_t)RNT + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * RNT)
+    counts[i] = poisson_draw((double)lam[i], seed, (uint64_t)i * 256u);
+}
+
 static int grid_for(int64_t n) {
   int64_t b = (n + RNT - 1) / RNT;
   if (b < 1) b = 1;
@@ -147,6 +206,16 @@ int sqrt_metric_cot(int dtype, const void* expo, const int* counts, const void* 
         (const double*)expo, counts, (const double*)lam, (const double*)noise, (double*)cot, n,
         total);
   return check_hip(hipGetLastError(), "sqrt_metric_cot");
+}
+
+int poisson_sample(int dtype, const void* lam, int64_t n, uint64_t seed, int* counts,
+                   hipStream_t s) {
+  if (n <= 0) return kOk;
+  if (dtype == 0)
+    k_poisson_draw<float><<<grid_for(n), RNT, 0, s>>>((const float*)lam, n, seed, counts);
+  else
+    k_poisson_draw<double><<<grid_for(n), RNT, 0, s>>>((const double*)lam, n, seed, counts);
+  return check_hip(hipGetLastError(), "poisson_sample");
 }
 
 int poisson_nll_grad(int dtype, const void* lam, const void* d, int64_t n, double* nll,

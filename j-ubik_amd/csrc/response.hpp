@@ -12,6 +12,9 @@ int readout_gather(int dtype, const void* x, const int64_t* idx, int64_t nnz, vo
                    hipStream_t s);
 int readout_scatter(int dtype, const void* v, const int64_t* idx, int64_t nnz, int64_t n,
                     void* out, hipStream_t s);
+// counts[i] ~ Poisson(lam[i]), a pure function of (seed, i, lam[i]).
+int poisson_sample(int dtype, const void* lam, int64_t n, uint64_t seed, int* counts,
+                   hipStream_t s);
 // cot[s, i] = expo[i] noise[s, i] / sqrt(lam[i]) where counts[i] >= 0, else 0
 // (n pixels per sample, S samples).
 int sqrt_metric_cot(int dtype, const void* expo, const int* counts, const void* lam,
