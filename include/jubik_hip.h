@@ -197,6 +197,19 @@ int jubik_model_grad_batch(jubik_model* model, int64_t S, const void* amp_ref,
 int jubik_model_metric_batch(jubik_model* model, int64_t S, const void* amp_ref,
                              const void* amp_alpha, const void* dnu, const void* v_ref,
                              const void* v_alpha, void* out_ref, void* out_alpha, void* stream);
+/* Graph capture and launch (no counterpart in the reference, which relies
+ * on XLA's compilation of the jitted likelihood): the gradient evaluation of
+ * jubik_model_grad (S = 0) or jubik_model_grad_batch (S >= 1) with these
+ * fixed buffers, captured once into an executable HIP graph.  Each launch
+ * reads the buffers' current contents; launches of an S = 0 graph also set
+ * the metric base point.  Capture fails while kernel timing is enabled. */
+typedef struct jubik_graph jubik_graph;
+int jubik_model_grad_graph(jubik_graph** graph, jubik_model* model, int64_t S,
+                           const void* amp_ref, const void* amp_alpha, const void* xi_ref,
+                           const void* xi_alpha, double zm, double mu, const void* dnu,
+                           double* nll, void* g_ref, void* g_alpha);
+int jubik_graph_launch(jubik_graph* graph, void* stream);
+int jubik_graph_destroy(jubik_graph* graph);
 /* Left square root of the likelihood metric at the base point, for S noise
  * vectors: out = J^T diag(lam^-1/2) noise, noise (S, M, E, H, W) in data
  * space (ignored where counts < 0).  With noise ~ N(0, 1) this draws from

@@ -54,6 +54,10 @@ SIGNATURES = {
     "jubik_model_metric_batch": [_vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
     "jubik_model_sqrt_metric_batch": [_vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
     "jubik_poisson_sample": [_int, _vp, _i64, C.c_uint64, _vp, _vp],
+    "jubik_model_grad_graph": [C.POINTER(_vp), _vp, _i64, _vp, _vp, _vp, _vp, _dbl, _dbl, _vp,
+                               _vp, _vp, _vp],
+    "jubik_graph_launch": [_vp, _vp],
+    "jubik_graph_destroy": [_vp],
 }
 
 _lock = threading.Lock()

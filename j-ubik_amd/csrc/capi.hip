@@ -222,6 +222,16 @@ struct jubik_model {
 
 static hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
 
+// An executable HIP graph of one fused gradient evaluation with fixed
+// buffers; ``single`` graphs refresh the model's metric base point.
+struct jubik_graph {
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  jubik_model* model = nullptr;
+  bool single = false;
+  int device = 0;
+};
+
 extern "C" {
 
 const char* jubik_last_error(void) { return g_error.c_str(); }
@@ -710,6 +720,72 @@ int jubik_model_grad_batch(jubik_model* model, int64_t S, const void* amp_ref,
   DeviceGuard dg(model->lp->device);
   return model_grad(model, (int)S, 1, amp_ref, amp_alpha, xi_ref, xi_alpha, zm, mu, dnu, nll,
                     g_ref, g_alpha, as_stream(stream));
+}
+
+int jubik_model_grad_graph(jubik_graph** out, jubik_model* model, int64_t S,
+                           const void* amp_ref, const void* amp_alpha, const void* xi_ref,
+                           const void* xi_alpha, double zm, double mu, const void* dnu,
+                           double* nll, void* g_ref, void* g_alpha) {
+  if (!out || !model || !nll || !g_ref || !amp_ref || !xi_ref)
+    return fail(kInvalidArgument, "NULL argument");
+  *out = nullptr;
+  if (S < 0 || S > model->max_samples)
+    return fail(kInvalidArgument, "batch of " + std::to_string(S) +
+                                      " samples outside [0, max_samples = " +
+                                      std::to_string(model->max_samples) + "]");
+  if (timing_enabled())
+    return fail(kInvalidArgument, "disable kernel timing before capturing a graph");
+  int st = check_model(model);
+  if (!st) st = check_alpha(model, amp_alpha, xi_alpha, g_alpha, dnu);
+  if (st) return st;
+  DeviceGuard dg(model->lp->device);
+  hipStream_t cs = nullptr;
+  JUBIK_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+  auto* gr = new jubik_graph();
+  gr->model = model;
+  gr->single = (S == 0);
+  gr->device = model->lp->device;
+  hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+  if (e == hipSuccess) {
+    st = gr->single ? model_grad(model, 1, 0, amp_ref, amp_alpha, xi_ref, xi_alpha, zm, mu, dnu,
+                                 nll, g_ref, g_alpha, cs)
+                    : model_grad(model, (int)S, 1, amp_ref, amp_alpha, xi_ref, xi_alpha, zm, mu,
+                                 dnu, nll, g_ref, g_alpha, cs);
+    hipGraph_t g = nullptr;
+    const hipError_t e2 = hipStreamEndCapture(cs, &g);
+    gr->graph = g;
+    if (!st && e2 != hipSuccess) st = check_hip(e2, "hipStreamEndCapture");
+    if (!st) st = check_hip(hipGraphInstantiate(&gr->exec, gr->graph, nullptr, nullptr, 0),
+                            "hipGraphInstantiate");
+  } else {
+    st = check_hip(e, "hipStreamBeginCapture");
+  }
+  (void)hipStreamDestroy(cs);
+  if (st) {
+    if (gr->exec) (void)hipGraphExecDestroy(gr->exec);
+    if (gr->graph) (void)hipGraphDestroy(gr->graph);
+    delete gr;
+    return st;
+  }
+  *out = gr;
+  return kOk;
+}
+
+int jubik_graph_launch(jubik_graph* graph, void* stream) {
+  if (!graph || !graph->exec) return fail(kInvalidArgument, "NULL graph");
+  DeviceGuard dg(graph->device);
+  JUBIK_HIP(hipGraphLaunch(graph->exec, as_stream(stream)));
+  if (graph->single) graph->model->have_point = true;
+  return kOk;
+}
+
+int jubik_graph_destroy(jubik_graph* graph) {
+  if (!graph) return kOk;
+  DeviceGuard dg(graph->device);
+  if (graph->exec) (void)hipGraphExecDestroy(graph->exec);
+  if (graph->graph) (void)hipGraphDestroy(graph->graph);
+  delete graph;
+  return kOk;
 }
 
 int jubik_model_metric_batch(jubik_model* model, int64_t S, const void* amp_ref,

@@ -174,3 +174,49 @@ class XrayModel:
                   dev.ptr(o_alpha), dev.stream())
         return dev.like_input(o_ref, noise), (
             dev.like_input(o_alpha, noise) if o_alpha is not None else None)
+
+    def capture_grad(self, samples=0):
+        """Capture one gradient evaluation into an executable HIP graph.
+
+        samples = 0 captures the single evaluation (its launches set the
+        metric base point), samples = S >= 1 the batched one.  Write new
+        positions into ``graph.xi_ref`` / ``graph.xi_alpha`` and call
+        ``graph.launch()``; results appear in ``graph.nll``, ``graph.g_ref``
+        and ``graph.g_alpha``.
+        """
+        return GradGraph(self, int(samples))
+
+
+class GradGraph:
+    """A captured gradient evaluation with its fixed device buffers."""
+
+    def __init__(self, model: XrayModel, samples):
+        s = model.sky
+        if not 0 <= samples <= model.max_samples:
+            raise ValueError(f"samples must be in [0, max_samples={model.max_samples}]")
+        lead = (samples,) if samples else ()
+        shape = lead + (s.N, s.N)
+        self.model = model
+        self.xi_ref = torch.zeros(shape, dtype=model.dtype, device="cuda")
+        self.xi_alpha = torch.zeros(shape, dtype=model.dtype, device="cuda") if s.has_alpha else None
+        self.g_ref = torch.empty(shape, dtype=model.dtype, device="cuda")
+        self.g_alpha = torch.empty(shape, dtype=model.dtype, device="cuda") if s.has_alpha else None
+        self.nll = torch.empty(max(samples, 1), dtype=torch.float64, device="cuda")
+        self._graph = ctypes.c_void_p()
+        _lib.call("jubik_model_grad_graph", ctypes.byref(self._graph), model._model, samples,
+                  dev.ptr(s.amp_ref), dev.ptr(s.amp_alpha), dev.ptr(self.xi_ref),
+                  dev.ptr(self.xi_alpha), s.zm, s.mu, dev.ptr(s.dnu_dev), dev.ptr(self.nll),
+                  dev.ptr(self.g_ref), dev.ptr(self.g_alpha))
+
+    def launch(self):
+        """Enqueue one replay on the current stream (no synchronisation)."""
+        _lib.call("jubik_graph_launch", self._graph, dev.stream())
+
+    def __del__(self):
+        g = getattr(self, "_graph", None)
+        if g is not None and g.value:
+            try:
+                _lib.load().jubik_graph_destroy(g)
+            except Exception:
+                pass
+            self._graph = None
