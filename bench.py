@@ -47,6 +47,8 @@ def parse():
                     help="only run the mixed gradient steps (for rocprofv3)")
     ap.add_argument("--samples", type=int, default=8,
                     help="VI samples per call of the batched gradient leg (0: skip)")
+    ap.add_argument("--sweep", action="store_true",
+                    help="time every benchmark configuration (C1-C5, W*) instead of the W* line")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="with --profile-only: first copy a buffer of this many MiB "
                          "(a known-size reference for the HBM byte counters)")
@@ -199,6 +201,115 @@ def cpu_baseline(inp, counts, budget):
                        f"median {med:.2f} s"), result
 
 
+# Configurations C1-C5 and W*: grid, energies, modules, patches, margin,
+# PSF core sigma range (Gaussian, + Moffat beta), exposure range, mask
+# fraction, samples per batched call.
+CONFIGS = {
+    "C1": dict(H=128, E=1, M=1, n=4, m=16, sigma=(1.0, 3.0), moffat=None,
+               expo=(1e3, 1e3), mask=0.10, S=0, seed=1),
+    "C2": dict(H=1024, E=1, M=1, n=8, m=64, sigma=(1.0, 4.0), moffat=2.5,
+               expo=(5e2, 2e3), mask=0.05, S=0, seed=2),
+    "C3": dict(H=1024, E=3, M=7, n=10, m=60, sigma=(2.0, 6.0), moffat=None,
+               expo=(1e2, 3e3), mask=0.08, S=0, seed=3),
+    "C4a": dict(H=256, E=1, M=1, n=6, m=22, sigma=(1.0, 3.0), moffat=None,
+                expo=(5e2, 2e3), mask=0.05, S=32, seed=4),
+    "C4b": dict(H=512, E=1, M=1, n=8, m=32, sigma=(1.0, 3.0), moffat=None,
+                expo=(5e2, 2e3), mask=0.05, S=32, seed=4),
+    "C4c": dict(H=768, E=1, M=1, n=12, m=48, sigma=(1.0, 3.0), moffat=None,
+                expo=(5e2, 2e3), mask=0.05, S=32, seed=4),
+    "C5": dict(H=2048, E=4, M=1, n=16, m=64, sigma=(1.0, 4.0), moffat=2.5,
+               expo=(5e2, 2e3), mask=0.05, S=32, seed=5),
+    "W*": dict(H=1024, E=4, M=1, n=8, m=64, sigma=(1.0, 4.0), moffat=2.5,
+               expo=(5e2, 2e3), mask=0.05, S=8, seed=2),
+}
+
+
+def sweep(args):
+    """One JSON line per configuration: forward (sky + PSF + exposure),
+    gradient evaluation, and for S > 0 the batched metric products and
+    gradients, all with resident inputs; a one-evaluation CPU-oracle sample
+    where it takes under ~10 s."""
+    import torch
+    import __graft_entry__ as ge
+    ja = ge.load_package()
+    from tests.problems import powerlaw_amp, psf_windows
+    from oracle import model as om
+    for name, c in CONFIGS.items():
+        H, E, M, n, m = c["H"], c["E"], c["M"], c["n"], c["m"]
+        rng = np.random.default_rng(c["seed"])
+        spectral = E > 1
+        amp_ref = powerlaw_amp(H)
+        amp_alpha = 0.5 * amp_ref if spectral else None
+        dnu = np.log(np.geomspace(1.0, 4.0, E)) if spectral else np.zeros(1)
+        w = 2 * (H // n)
+        win = np.stack([psf_windows(rng, n, E, w, sigma=c["sigma"], moffat=c["moffat"],
+                                    rotation=k * np.pi / 7) for k in range(M)])
+        ramp = np.linspace(c["expo"][0], c["expo"][1], H)[None, None, None, :]
+        expo = np.broadcast_to(ramp, (M, E, H, H)).copy()
+        expo[rng.random(expo.shape) < c["mask"]] = 0.0
+        keep = expo > 0
+        xt = rng.normal(size=(H, H))
+        xat = rng.normal(size=(H, H)) if spectral else None
+
+        def model_for(dtype, S=1, counts=None):
+            sky = ja.CorrelatedFieldSky(H, (H, H), amp_ref, amp_alpha, zm=-2.0,
+                                        mu=-1.0 if spectral else 0.0, dnu=dnu, dtype=dtype)
+            psf = ja.LinpatchOperator((E, H, H), n, m, n_modules=M, dtype=dtype)
+            psf.set_kernel(win if M > 1 else win[0], layout="window")
+            cnt = counts if counts is not None else np.zeros(keep.shape, np.int32)
+            return sky, psf, ja.XrayModel(sky, psf, 1.0, expo, cnt, max_samples=max(S, 1))
+
+        # counts from the true position (FP64 where the GP grid allows it)
+        sky64, psf64, _ = model_for(torch.float64 if H <= 1024 else torch.float32)
+        s_true = sky64(torch.as_tensor(xt, device="cuda").to(sky64.dtype),
+                       None if xat is None else torch.as_tensor(xat, device="cuda").to(sky64.dtype))
+        lam = psf64.forward(s_true).reshape((M, E, H, H)) * torch.as_tensor(expo, device="cuda")
+        counts = np.where(keep, ja.poisson_sample(lam, seed=c["seed"]).cpu().numpy(), -1)
+        del sky64, psf64, s_true, lam
+        sky, psf, model = model_for(torch.float32, c["S"], counts.astype(np.int32))
+        xi = xt + 0.1 * rng.normal(size=(H, H))
+        xa = xat + 0.1 * rng.normal(size=(H, H)) if spectral else None
+        xr_t = torch.as_tensor(xi, dtype=torch.float32, device="cuda")
+        xa_t = torch.as_tensor(xa, dtype=torch.float32, device="cuda") if spectral else None
+        ex_t = torch.as_tensor(expo, dtype=torch.float32, device="cuda")
+        steps = max(5, args.steps // (4 if H >= 2048 else 1))
+
+        def fwd():
+            return psf.forward(sky(xr_t, xa_t)).reshape(ex_t.shape) * ex_t
+
+        out = model._grads()
+        rec = {"config": name, "H": H, "E": E, "M": M, "n": n, "margin": m,
+               "L": 2 * (H // n) + 2 * m,
+               "forward_ms": 1e3 * time_steps(torch, fwd, steps, 2),
+               "grad_eval_ms": 1e3 * time_steps(
+                   torch, lambda: model.nll_and_grad_device(xr_t, xa_t, out=out), steps, 2)}
+        if c["S"]:
+            S = c["S"]
+            model.nll_and_grad_device(xr_t, xa_t, out=out)
+            v = torch.randn((S // 2, H, H), device="cuda", dtype=torch.float32)
+            vr = torch.cat([v, -v])
+            va = torch.cat([v, -v]) if spectral else None
+            t_m = time_steps(torch, lambda: model.metric_batch(vr, va), max(3, steps // 8), 1)
+            xb = xr_t[None] + 0.05 * torch.randn((S, H, H), device="cuda")
+            xab = xa_t[None] + 0.05 * torch.randn((S, H, H), device="cuda") if spectral else None
+            t_g = time_steps(torch, lambda: model.nll_and_grad_batch(xb, xab), max(3, steps // 8), 1)
+            rec.update({"samples": S, "metric_batch_ms_per_product": 1e3 * t_m / S,
+                        "grad_batch_ms_per_eval": 1e3 * t_g / S})
+        if not args.no_cpu and H * H * E * M <= 1024 * 1024 * 4:
+            prob = om.Problem(shape=(E, H, H), distances=(1.0, 1.0, 1.0), n=n, margin=m,
+                              pixel_area=1.0, amp_ref=amp_ref, amp_alpha=amp_alpha, zm=-2.0,
+                              mu_alpha=-1.0 if spectral else 0.0, dnu=dnu)
+            prob.modules = [om.Module(window=win[k], exposure=expo[k], keep=keep[k],
+                                      data=counts[k][keep[k]].astype(np.float64))
+                            for k in range(M)]
+            t0 = time.perf_counter()
+            om.nll_and_grad(prob, xi, xa)
+            rec["cpu_oracle_grad_eval_ms"] = 1e3 * (time.perf_counter() - t0)
+        print(json.dumps(rec), flush=True)
+        del model, sky, psf
+        torch.cuda.empty_cache()
+
+
 def rel(a, b):
     return float(np.linalg.norm(a - b) / np.linalg.norm(b))
 
@@ -209,6 +320,9 @@ def nmax(a, b):
 
 def main():
     args = parse()
+    if args.sweep:
+        sweep(args)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
               "(the hot path is sharded by module/energy/sample across GPUs by design, "
