@@ -129,6 +129,24 @@ int jubik_poisson_sample(int dtype, const void* lam, int64_t n, uint64_t seed, i
 int jubik_poisson_nll_grad(int dtype, const void* lam, const void* data, int64_t n, double* nll,
                            void* grad, void* work, void* stream);
 
+/* ---------------------------------------------------------------- PSF preparation
+ * eROSITA patch PSFs of
+ * jubik/instruments/erosita/erosita_psf_utils.py:185-264 (psf_interpolator,
+ * with get_psf :90-164) for one energy: the nrad tabulated PSFs (device,
+ * nrad x ph x pw) at off-axis radii rs are weighted linearly by each patch
+ * centre's radius, rotated by its azimuth about pointing_center and sampled
+ * bilinearly (map_coordinates order 1, mode 'nearest').  rs,
+ * patch_center_ids (nrad x 2), patch_deltas (2) and pointing_center (2) are
+ * host arrays.  Writes slot e of out (P, E, ...): layout 0 the (H, W) frame
+ * rolled to the origin (the reference's output), layout 1 the centred
+ * 2H/npatch-wide window linpatch_convolve cuts from it.  Set-up routine: it
+ * allocates a small table and synchronises the stream. */
+int jubik_psf_interpolate(int dtype, const void* psfs, int64_t nrad, int64_t ph, int64_t pw,
+                          const double* rs, const double* patch_center_ids,
+                          const double* patch_deltas, const double* pointing_center, int64_t H,
+                          int64_t W, double dH, double dW, int64_t npatch, int64_t e, int64_t E,
+                          int layout, void* out, void* stream);
+
 /* ---------------------------------------------------------------- GP sky
  * Correlated-field sky of
  * jubik/sky_model/multifrequency/spectral_product_mf_sky.py:171-298 (and

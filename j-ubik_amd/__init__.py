@@ -15,6 +15,7 @@ from .sky import CorrelatedFieldSky, powerlaw_amplitude
 from .model import XrayModel, dense_counts
 from .sampling import conjugate_gradient, draw_linear_samples, metric_solve
 from .mock import create_mock_counts, poisson_sample
+from .psf import psf_interpolator
 
 __all__ = [
     "Domain", "Vector", "LinpatchOperator", "convolve", "integrate", "linpatch_convolve",
@@ -22,5 +23,5 @@ __all__ = [
     "linpatch_convolve_modules", "build_exposure_function", "build_readout_function",
     "Poissonian", "CorrelatedFieldSky", "powerlaw_amplitude", "XrayModel", "dense_counts",
     "conjugate_gradient", "draw_linear_samples", "metric_solve", "create_mock_counts",
-    "poisson_sample",
+    "poisson_sample", "psf_interpolator",
 ]

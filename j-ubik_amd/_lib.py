@@ -58,6 +58,8 @@ SIGNATURES = {
                                _vp, _vp, _vp],
     "jubik_graph_launch": [_vp, _vp],
     "jubik_graph_destroy": [_vp],
+    "jubik_psf_interpolate": [_int, _vp, _i64, _i64, _i64, _vp, _vp, _vp, _vp, _i64, _i64, _dbl,
+                              _dbl, _i64, _i64, _i64, _int, _vp, _vp],
 }
 
 _lock = threading.Lock()

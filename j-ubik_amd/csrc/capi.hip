@@ -12,6 +12,7 @@
 #include "gp.hpp"
 #include "linpatch.hpp"
 #include "primitives.hpp"
+#include "psf.hpp"
 #include "response.hpp"
 
 namespace jubik {
@@ -503,6 +504,82 @@ int jubik_poisson_nll_grad(int dtype, const void* lam, const void* data, int64_t
   if (n < 0) return fail(kInvalidArgument, "negative size");
   if (!nll || !work || (n && (!lam || !data))) return fail(kInvalidArgument, "NULL argument");
   return poisson_nll_grad(dtype, lam, data, n, nll, grad, (double*)work, as_stream(stream));
+}
+
+// ------------------------------------------------------------------ PSF
+int jubik_psf_interpolate(int dtype, const void* psfs, int64_t nrad, int64_t ph, int64_t pw,
+                          const double* rs, const double* patch_center_ids,
+                          const double* patch_deltas, const double* pointing_center, int64_t H,
+                          int64_t W, double dH, double dW, int64_t npatch, int64_t e, int64_t E,
+                          int layout, void* out, void* stream) {
+  if (dtype != 0 && dtype != 1) return fail(kInvalidArgument, "dtype must be 0 or 1");
+  if (!psfs || !rs || !patch_center_ids || !patch_deltas || !pointing_center || !out)
+    return fail(kInvalidArgument, "NULL argument");
+  if (nrad < 1 || ph < 1 || pw < 1 || npatch < 1 || H < 2 || W < 2 || E < 1 || e < 0 || e >= E)
+    return fail(kInvalidArgument, "invalid sizes");
+  if (layout != 0 && layout != 1) return fail(kInvalidArgument, "layout must be 0 or 1");
+  // erosita_psf_utils.py:196-201 and :143-147
+  if (H % npatch || W % npatch) return fail(kInvalidArgument, "grid not divisible by npatch");
+  if (H % 2 || W % 2) return fail(kInvalidArgument, "grid sides must be even");
+  for (int64_t k = 0; k < nrad; ++k) {
+    const double c0 = patch_center_ids[2 * k], c1 = patch_center_ids[2 * k + 1];
+    if (c0 > (double)ph || c1 > (double)pw || c0 < 0 || c1 < 0)
+      return fail(kInvalidArgument, "PSF centre index outside its grid");
+  }
+  if (!(patch_deltas[0] > 0) || !(patch_deltas[1] > 0))
+    return fail(kInvalidArgument, "PSF pixel sizes must be positive");
+  PsfGeom g{};
+  g.H = (int)H;
+  g.W = (int)W;
+  g.dH = dH;
+  g.dW = dW;
+  g.npatch = (int)npatch;
+  g.nrad = (int)nrad;
+  g.ph = (int)ph;
+  g.pw = (int)pw;
+  g.d0 = patch_deltas[0];
+  g.d1 = patch_deltas[1];
+  g.layout = layout;
+  g.w = (int)(2 * (H / npatch));
+  g.e = (int)e;
+  g.E = (int)E;
+  if (layout == 1 && (g.w > H || g.w > W))
+    return fail(kInvalidArgument, "window wider than the grid (npatch must be >= 2)");
+  std::vector<int> order;
+  std::vector<double> phi, wgt;
+  psf_patch_table(rs, (int)nrad, pointing_center, g.H, g.W, dH, dW, g.npatch, order, phi, wgt);
+  std::vector<double> cid(2 * nrad);
+  for (int64_t k = 0; k < nrad; ++k) {
+    cid[2 * k] = patch_center_ids[2 * order[k]];
+    cid[2 * k + 1] = patch_center_ids[2 * order[k] + 1];
+  }
+  // one set-up table on the device: order | cid | phi | wgt
+  const size_t b_order = (size_t)nrad * sizeof(int);
+  const size_t off_cid = (b_order + 15) / 16 * 16;
+  const size_t off_phi = off_cid + cid.size() * sizeof(double);
+  const size_t off_wgt = off_phi + phi.size() * sizeof(double);
+  const size_t bytes = off_wgt + wgt.size() * sizeof(double);
+  std::vector<unsigned char> host(bytes, 0);
+  std::memcpy(host.data(), order.data(), b_order);
+  std::memcpy(host.data() + off_cid, cid.data(), cid.size() * sizeof(double));
+  std::memcpy(host.data() + off_phi, phi.data(), phi.size() * sizeof(double));
+  std::memcpy(host.data() + off_wgt, wgt.data(), wgt.size() * sizeof(double));
+  void* table = nullptr;
+  int st = dev_alloc(&table, bytes, "PSF patch table");
+  if (st) return st;
+  hipStream_t s = as_stream(stream);
+  hipError_t he = hipMemcpyAsync(table, host.data(), bytes, hipMemcpyHostToDevice, s);
+  if (he == hipSuccess) {
+    char* t = reinterpret_cast<char*>(table);
+    st = psf_interpolate(dtype, g, psfs, reinterpret_cast<const int*>(t),
+                         reinterpret_cast<const double*>(t + off_cid),
+                         reinterpret_cast<const double*>(t + off_phi),
+                         reinterpret_cast<const double*>(t + off_wgt), out, s);
+    he = hipStreamSynchronize(s);
+  }
+  if (!st && he != hipSuccess) st = check_hip(he, "psf_interpolate");
+  dev_free(table);
+  return st;
 }
 
 // ------------------------------------------------------------------ GP
