@@ -308,6 +308,31 @@ def sweep(args):
         print(json.dumps(rec), flush=True)
         del model, sky, psf
         torch.cuda.empty_cache()
+    print(json.dumps(dft_compare(ja, torch, args)), flush=True)
+
+
+def dft_compare(ja, torch, args):
+    """The 128-point DFT comparison of the 2048^2 x 4 configuration: 2^20
+    vectors (1 GiB in, 1 GiB out) through the FP32 FFT and the split-BF16
+    matrix-core DFT; accuracy against numpy in float64 on 4096 of them."""
+    B = 1 << 20
+    g = torch.Generator(device="cuda").manual_seed(0)
+    x = torch.complex(torch.randn((B, 128), generator=g, device="cuda"),
+                      torch.randn((B, 128), generator=g, device="cuda"))
+    d = ja.DFT128()
+    y = torch.empty_like(x)
+    want = np.fft.fft(x[:4096].cpu().numpy().astype(np.complex128), axis=1)
+    rec = {"dft128": True, "vectors": B}
+    for method in ("fft", "mfma"):
+        t = time_steps(torch, lambda: d(x, method, out=y), max(5, args.steps // 5), 2)
+        got = y[:4096].cpu().numpy()
+        rec[method] = {"ms": 1e3 * t, "vectors_per_s": B / t,
+                       "hbm_GBps": 2 * x.numel() * 8 / t / 1e9,
+                       "rel_err_vs_fp64": float(np.linalg.norm(got - want) / np.linalg.norm(want))}
+    # 4 row tiles x 8 k-steps x 12 products of 32x32x16 per 32 vectors
+    rec["mfma"]["tflops_matrix_core"] = B / 32 * 4 * 8 * 12 * (32 * 32 * 16 * 2) / (
+        rec["mfma"]["ms"] * 1e-3) / 1e12
+    return rec
 
 
 def rel(a, b):

@@ -129,6 +129,18 @@ int jubik_poisson_sample(int dtype, const void* lam, int64_t n, uint64_t seed, i
 int jubik_poisson_nll_grad(int dtype, const void* lam, const void* data, int64_t n, double* nll,
                            void* grad, void* work, void* stream);
 
+/* ---------------------------------------------------------------- DFT comparison
+ * Batched 128-point complex DFTs (FP32 complex in and out, B vectors, B a
+ * multiple of 256), no counterpart in the reference: method 0 the FP32 FFT
+ * the PSF kernels use, method 1 a split-BF16 DFT as real GEMMs on the
+ * matrix cores (each product as hi*hi + hi*lo + lo*hi, FP32 accumulation),
+ * the comparison asked for the 2048^2 x 4 energy configuration. */
+typedef struct jubik_dft128 jubik_dft128;
+int jubik_dft128_create(jubik_dft128** plan);
+int jubik_dft128_run(jubik_dft128* plan, int method, const void* x, void* y, int64_t B,
+                     void* stream);
+int jubik_dft128_destroy(jubik_dft128* plan);
+
 /* ---------------------------------------------------------------- PSF preparation
  * eROSITA patch PSFs of
  * jubik/instruments/erosita/erosita_psf_utils.py:185-264 (psf_interpolator,

@@ -16,6 +16,7 @@ from .model import XrayModel, dense_counts
 from .sampling import conjugate_gradient, draw_linear_samples, metric_solve
 from .mock import create_mock_counts, poisson_sample
 from .psf import psf_interpolator
+from .dft import DFT128
 
 __all__ = [
     "Domain", "Vector", "LinpatchOperator", "convolve", "integrate", "linpatch_convolve",
@@ -23,5 +24,5 @@ __all__ = [
     "linpatch_convolve_modules", "build_exposure_function", "build_readout_function",
     "Poissonian", "CorrelatedFieldSky", "powerlaw_amplitude", "XrayModel", "dense_counts",
     "conjugate_gradient", "draw_linear_samples", "metric_solve", "create_mock_counts",
-    "poisson_sample", "psf_interpolator",
+    "poisson_sample", "psf_interpolator", "DFT128",
 ]

@@ -58,6 +58,9 @@ SIGNATURES = {
                                _vp, _vp, _vp],
     "jubik_graph_launch": [_vp, _vp],
     "jubik_graph_destroy": [_vp],
+    "jubik_dft128_create": [C.POINTER(_vp)],
+    "jubik_dft128_run": [_vp, _int, _vp, _vp, _i64, _vp],
+    "jubik_dft128_destroy": [_vp],
     "jubik_psf_interpolate": [_int, _vp, _i64, _i64, _i64, _vp, _vp, _vp, _vp, _i64, _i64, _dbl,
                               _dbl, _i64, _i64, _i64, _int, _vp, _vp],
 }

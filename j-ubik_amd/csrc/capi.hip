@@ -11,6 +11,7 @@
 #include "../../include/jubik_hip.h"
 #include "gp.hpp"
 #include "linpatch.hpp"
+#include "dft128.hpp"
 #include "primitives.hpp"
 #include "psf.hpp"
 #include "response.hpp"
@@ -504,6 +505,57 @@ int jubik_poisson_nll_grad(int dtype, const void* lam, const void* data, int64_t
   if (n < 0) return fail(kInvalidArgument, "negative size");
   if (!nll || !work || (n && (!lam || !data))) return fail(kInvalidArgument, "NULL argument");
   return poisson_nll_grad(dtype, lam, data, n, nll, grad, (double*)work, as_stream(stream));
+}
+
+// ------------------------------------------------------------------ DFT comparison
+struct jubik_dft128 {
+  void* frag = nullptr;
+  void* tw = nullptr;
+  int device = 0;
+  int nblocks = 0;
+};
+
+int jubik_dft128_create(jubik_dft128** plan) {
+  if (!plan) return fail(kInvalidArgument, "NULL argument");
+  *plan = nullptr;
+  auto* p = new jubik_dft128();
+  int st = hipGetDevice(&p->device) == hipSuccess ? kOk : fail(kRuntimeError, "no HIP device");
+  hipDeviceProp_t prop;
+  if (!st) st = check_hip(hipGetDeviceProperties(&prop, p->device), "hipGetDeviceProperties");
+  if (!st) p->nblocks = prop.multiProcessorCount;
+  std::vector<unsigned short> host;
+  dft128_fragments(host);
+  if (!st) st = dev_alloc(&p->frag, host.size() * 2, "DFT fragments");
+  if (!st) st = check_hip(hipMemcpy(p->frag, host.data(), host.size() * 2, hipMemcpyHostToDevice),
+                          "DFT fragments");
+  if (!st) st = upload_twiddles(&p->tw, 0, 128);
+  if (st) {
+    dev_free(p->frag);
+    dev_free(p->tw);
+    delete p;
+    return st;
+  }
+  *plan = p;
+  return kOk;
+}
+
+int jubik_dft128_run(jubik_dft128* plan, int method, const void* x, void* y, int64_t B,
+                     void* stream) {
+  if (!plan || !x || !y) return fail(kInvalidArgument, "NULL argument");
+  if (method != 0 && method != 1) return fail(kInvalidArgument, "method must be 0 or 1");
+  if (B <= 0 || B % kDft128TileVectors)
+    return fail(kInvalidArgument, "batch must be a positive multiple of 256 vectors");
+  DeviceGuard dg(plan->device);
+  return dft128_run(method, plan->frag, plan->tw, x, y, B, plan->nblocks, as_stream(stream));
+}
+
+int jubik_dft128_destroy(jubik_dft128* plan) {
+  if (!plan) return kOk;
+  DeviceGuard dg(plan->device);
+  dev_free(plan->frag);
+  dev_free(plan->tw);
+  delete plan;
+  return kOk;
 }
 
 // ------------------------------------------------------------------ PSF
