@@ -469,6 +469,13 @@ def main():
                           "fused_fp64_rel": rel(g64, go), "nll_mixed_rel": abs(nll32 - nll_o) /
                           abs(nll_o), "nll_fp64_rel": abs(nll64 - nll_o) / abs(nll_o)}
 
+    # practical HBM ceiling: a 1 GiB device-to-device copy (read + write)
+    src = torch.empty(1 << 28, dtype=torch.float32, device="cuda").fill_(1.0)
+    dst = torch.empty_like(src)
+    t_copy = time_steps(torch, lambda: dst.copy_(src), 20, 3)
+    copy_bw = 2 * src.numel() * 4 / t_copy
+    del src, dst
+
     value = 1.0 / t_mixed
     best_plain = max(plain.values(), key=lambda r: r["evals_per_s"]) if plain else None
     line = {
@@ -491,7 +498,8 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK,
                      "traffic": measured_traffic(dom, H, E, n, m), "kernel": dom,
-                     "kernel_avg_us": dom_avg * 1e6, "bytes_per_launch": kb.get(dom, 0)},
+                     "kernel_avg_us": dom_avg * 1e6, "bytes_per_launch": kb.get(dom, 0),
+                     "device_copy_GBps": copy_bw / 1e9, "frac_of_copy": achieved / copy_bw},
         "eval_roofline": {"B_alg": eval_bytes(H, E, n, m),
                           "achieved": eval_bytes(H, E, n, m) / t_mixed,
                           "frac_hbm": eval_bytes(H, E, n, m) / t_mixed / HBM_PEAK,
