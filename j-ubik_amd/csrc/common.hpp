@@ -111,6 +111,23 @@ __device__ __forceinline__ T tent_weight(int d, double step, int t) {
 }
 __device__ __forceinline__ double tent_step(int d) { return d > 1 ? 1.0 / (d - 1) : 0.0; }
 
+// Streaming access to workspace arrays that are written once and read once
+// by the next pass: non-temporal loads and stores.
+template <typename T>
+__device__ __forceinline__ cplx<T> ld_stream(const cplx<T>* p) {
+  typedef T v2 __attribute__((ext_vector_type(2)));
+  const v2 v = __builtin_nontemporal_load(reinterpret_cast<const v2*>(p));
+  return cplx<T>{v.x, v.y};
+}
+template <typename T>
+__device__ __forceinline__ void st_stream(cplx<T>* p, cplx<T> c) {
+  typedef T v2 __attribute__((ext_vector_type(2)));
+  v2 v;
+  v.x = c.re;
+  v.y = c.im;
+  __builtin_nontemporal_store(v, reinterpret_cast<v2*>(p));
+}
+
 // Tile sizes: vectors per work-group so a tile of V vectors of length L at
 // the odd LDS stride occupies at most 52 KiB (16 float vectors at L = 384).
 #ifndef JUBIK_COL_TILE_BYTES

@@ -170,8 +170,8 @@ __global__ __launch_bounds__(NT) void lp_rows(LinGeom g, const T* __restrict__ s
       A = {T(0.5) * (zk.re + zn.re), T(0.5) * (zk.im - zn.im)};
       B = {T(0.5) * (zk.im + zn.im), T(0.5) * (zn.re - zk.re)};
     }
-    if (st0) out[r0 * L2P + k] = A;
-    if (st1) out[(r0 + 1) * L2P + k] = B;
+    if (st0) st_stream(&out[r0 * L2P + k], A);
+    if (st1) st_stream(&out[(r0 + 1) * L2P + k], B);
   }
 }
 
@@ -281,10 +281,10 @@ __global__ __launch_bounds__(NT) void lp_cols(LinGeom g, const cplx<T>* __restri
     if (k >= LH) return cplx<T>{T(0), T(0)};
     if constexpr (MODE == COL_ADJ) {
       const int ir = row0 + pos;
-      return (ir >= 0 && ir < g.H) ? in[pos * L2P + k] : cplx<T>{T(0), T(0)};
+      return (ir >= 0 && ir < g.H) ? ld_stream(&in[pos * L2P + k]) : cplx<T>{T(0), T(0)};
     } else {
       const int rr = pos - m;
-      return (rr >= 0 && rr < w) ? in[rr * L2P + k] : cplx<T>{T(0), T(0)};
+      return (rr >= 0 && rr < w) ? ld_stream(&in[rr * L2P + k]) : cplx<T>{T(0), T(0)};
     }
   };
   if constexpr (NS == 1) {
@@ -357,9 +357,9 @@ __global__ __launch_bounds__(NT) void lp_cols(LinGeom g, const cplx<T>* __restri
       if (k >= LH) return;
       if constexpr (MODE == COL_ADJ) {
         const int r = pos - m;
-        if (r >= 0 && r < w) o[r * L2P + k] = val;
+        if (r >= 0 && r < w) st_stream(&o[r * L2P + k], val);
       } else {
-        o[pos * L2P + k] = val;
+        st_stream(&o[pos * L2P + k], val);
       }
     };
     if constexpr (NS == 1) {
@@ -476,8 +476,8 @@ __device__ __forceinline__ void sum_pair_spectra(cplx<T>* buf, const cplx<T>* __
         const cplx<T>* rowp = base + ((int64_t)ix * nrows + f) * L2P;
 #pragma unroll
         for (int u = 0; u < GPT; ++u) {
-          if (oa[u] >= 0) a[c][u] = rowp[oa[u]];
-          if (ob[u] >= 0) b[c][u] = rowp[ob[u]];
+          if (oa[u] >= 0) a[c][u] = ld_stream(&rowp[oa[u]]);
+          if (ob[u] >= 0) b[c][u] = ld_stream(&rowp[ob[u]]);
         }
       }
     }
