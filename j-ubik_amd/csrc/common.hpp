@@ -142,11 +142,12 @@ struct Tile {
   static constexpr int V = tile_vectors(JUBIK_COL_TILE_BYTES, L, (int)sizeof(cplx<T>));
 };
 
-// Row passes use half that LDS budget: their global traffic is contiguous
-// rows whatever V is, and the smaller footprint doubles residency.
+// Row passes use a quarter of that LDS budget: their global traffic is
+// contiguous rows whatever V is, and the smaller footprint raises residency
+// (at L = 384, 4 vectors per tile measured faster than 8 or 16).
 template <typename T, int L>
 struct RowTile {
-  static constexpr int V = tile_vectors(26624, L, (int)sizeof(cplx<T>));
+  static constexpr int V = tile_vectors(13312, L, (int)sizeof(cplx<T>));
 };
 
 // Tile of work-group b out of nb: the dispatcher places work-group b on XCD
