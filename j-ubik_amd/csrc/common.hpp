@@ -128,6 +128,39 @@ __device__ __forceinline__ void st_stream(cplx<T>* p, cplx<T> c) {
   __builtin_nontemporal_store(v, reinterpret_cast<v2*>(p));
 }
 
+// Bounds-checked gathers through a raw buffer resource (stride 0, range
+// checked on the byte offset): a load at an offset at or beyond ``bytes``
+// returns zero in hardware.  Callers pass that offset for elements outside
+// the array, so a gather needs neither a branch nor a select and all its
+// loads are in flight together.  Word 3 is the gfx9-family default format.
+// Offsets are 32-bit: gathered arrays stay below 2 GiB (linpatch images are
+// at most 4096 pixels wide; the GP adjoint checks its planes on the host).
+struct Gather {
+  __amdgpu_buffer_rsrc_t r;
+  int oob;  // an offset that reads zero
+};
+template <typename T>
+__device__ __forceinline__ Gather make_gather(const T* p, int64_t count) {
+  const int bytes = (int)(count * (int64_t)sizeof(T));
+  return Gather{__builtin_amdgcn_make_buffer_rsrc(const_cast<T*>(p), 0, bytes, 0x00020000),
+                bytes};
+}
+// Byte offset of element i, or the zero-reading offset when !ok.
+template <typename T>
+__device__ __forceinline__ int gather_off(const Gather& g, bool ok, int i) {
+  return ok ? i * (int)sizeof(T) : g.oob;
+}
+template <typename T>
+__device__ __forceinline__ T gather_load(const Gather& g, int off) {
+  static_assert(sizeof(T) == 4 || sizeof(T) == 8 || sizeof(T) == 16, "gather element size");
+  if constexpr (sizeof(T) == 4)
+    return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(g.r, off, 0, 0));
+  else if constexpr (sizeof(T) == 8)
+    return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(g.r, off, 0, 0));
+  else
+    return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b128(g.r, off, 0, 0));
+}
+
 // Tile sizes: vectors per work-group so a tile of V vectors of length L at
 // the odd LDS stride occupies at most 52 KiB (16 float vectors at L = 384).
 #ifndef JUBIK_COL_TILE_BYTES

@@ -62,23 +62,37 @@ __global__ __launch_bounds__(GNT) void gp_cols(GpGeom g, const T* __restrict__ a
   constexpr int EPT = (V * N + GNT - 1) / GNT;
   constexpr int CH = (MODE == 0 ? 16 : 8) / (sizeof(T) / 4);
   constexpr int CHK = EPT < CH ? EPT : CH;
+  // MODE 1 reads the planes through a zero-reading gather (elements outside
+  // the sky grid read zero), planes outer and the chunk unrolled inside, so
+  // a chunk's loads of one plane are in flight together.
+  const int64_t plane = (int64_t)g.H * g.W;
+  const Gather gu = make_gather(u, MODE == 1 ? g.uplanes * plane : 0);
   for (int u0 = 0; u0 < EPT; u0 += CHK) {
     cplx<T> val[CHK];
 #pragma unroll
-    for (int c = 0; c < CHK; ++c) {
-      const int e = tid + (u0 + c) * GNT;
-      const int x1 = e / V, v = e - (e / V) * V;
-      const int x2 = c0 + v;
-      val[c] = cplx<T>{T(0), T(0)};
-      if (e >= V * N) continue;
-      if constexpr (MODE == 0) {
+    for (int c = 0; c < CHK; ++c) val[c] = cplx<T>{T(0), T(0)};
+    if constexpr (MODE == 0) {
+#pragma unroll
+      for (int c = 0; c < CHK; ++c) {
+        const int e = tid + (u0 + c) * GNT;
+        const int x1 = e / V, v = e - (e / V) * V;
+        const int x2 = c0 + v;
+        if (e >= V * N) continue;
         const int64_t o = (int64_t)x1 * N + x2;
         val[c].re = amp_ref[o] * xi_ref[o];
         if (g.has_alpha) val[c].im = amp_alpha[o] * xi_alpha[o];
-      } else {
-        if (x1 < g.H && x2 < g.W)
-          for (int pl = 0; pl < g.uplanes; ++pl)
-            val[c] = cadd(val[c], u[((int64_t)pl * g.H + x1) * g.W + x2]);
+      }
+    } else {
+      for (int pl = 0; pl < g.uplanes; ++pl) {
+#pragma unroll
+        for (int c = 0; c < CHK; ++c) {
+          const int e = tid + (u0 + c) * GNT;
+          const int x1 = e / V, v = e - (e / V) * V;
+          const int x2 = c0 + v;
+          const bool ok = (V * N % GNT == 0 || e < V * N) && x1 < g.H && x2 < g.W;
+          val[c] = cadd(val[c], gather_load<cplx<T>>(
+                                    gu, gather_off<cplx<T>>(gu, ok, (int)(pl * plane) + x1 * g.W + x2)));
+        }
       }
     }
 #pragma unroll
@@ -329,6 +343,10 @@ int gp_tangent(int dtype, const GpGeom& g, const GpBuffers& b, const void* amp_r
 int gp_adjoint_u(int dtype, const GpGeom& g, const GpBuffers& b, const void* amp_ref,
                  const void* amp_alpha, const void* u, void* g_ref, void* g_alpha,
                  hipStream_t s) {
+  // The column pass gathers a sample's planes with 32-bit byte offsets.
+  const int64_t bytes = (int64_t)g.uplanes * g.H * g.W * (dtype == 0 ? 8 : 16);
+  if (bytes > INT32_MAX)
+    return fail(kUnsupported, "the cotangent planes of one sample exceed 2 GiB");
   return gp_dispatch(dtype, g.N, [&](auto ops) {
     return decltype(ops)::adjoint_u(g, b, amp_ref, amp_alpha, u, g_ref, g_alpha, s);
   });

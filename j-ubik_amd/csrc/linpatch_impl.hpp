@@ -89,64 +89,118 @@ __global__ __launch_bounds__(NT) void lp_rows(LinGeom g, const T* __restrict__ s
     const int e = tid + u * NT;
     if (e < L) twr[u] = twg[e];
   }
-  T av[EPT], bv[EPT];
   const double step = tent_step(g.d);
+  // Signal rows with L > NT / 2: thread-owned window columns f = tid + u NT
+  // and the row pairs of the tile uniform over the work-group, so each
+  // column's image index and tent weight is formed once and each row's once
+  // (instruction issue, not memory, bounds this prologue).  Shorter rows use
+  // the element-wise mapping, which keeps every lane busy.
+  constexpr bool kColumns = (MODE == ROW_SIGNAL || MODE == ROW_COTANGENT) && 2 * L > NT;
+  const Gather gi = make_gather(img, (int64_t)g.H * g.W);
+  if constexpr (kColumns) {
+    T av[TPT][V], bv[TPT][V];
+    T tr0[V], tr1[V];
 #pragma unroll
-  for (int u = 0; u < EPT; ++u) {
-    const int e = tid + u * NT;
-    av[u] = bv[u] = T(0);
-    if (V * L % NT != 0 && e >= V * L) break;
-    const int v = e / L, f = e - (e / L) * L;
-    const int r0 = 2 * (q0 + v);
-    if (r0 < nrows) {
+    for (int v = 0; v < V; ++v) {
+      const int r0 = 2 * (q0 + v);
+      tr0[v] = MODE == ROW_SIGNAL ? tent_weight<T>(g.d, step, r0) : T(1);
+      tr1[v] = MODE == ROW_SIGNAL ? tent_weight<T>(g.d, step, r0 + 1) : T(1);
+    }
+#pragma unroll
+    for (int u = 0; u < TPT; ++u) {
+      const int f = tid + u * NT;
+      int j;
+      bool cok;
+      T wc = T(1);
       if constexpr (MODE == ROW_SIGNAL) {
         const int c = f - g.m;
-        if (c >= 0 && c < g.w) {
-          const int j = sy + c - g.h;
-          const int i0 = sx + r0 - g.h;
-          if (j >= 0 && j < g.W) {
-            const T wc = tent_weight<T>(g.d, step, c) * scale;
-            if (i0 >= 0 && i0 < g.H) av[u] = tent_weight<T>(g.d, step, r0) * wc * img[i0 * g.W + j];
-            if (i0 + 1 >= 0 && i0 + 1 < g.H)
-              bv[u] = tent_weight<T>(g.d, step, r0 + 1) * wc * img[(i0 + 1) * g.W + j];
+        j = sy + c - g.h;
+        cok = c >= 0 && c < g.w && j >= 0 && j < g.W;
+        if (cok) wc = tent_weight<T>(g.d, step, c) * scale;
+      } else {
+        j = sy + f - g.h - g.m;
+        cok = j >= 0 && j < g.W;
+      }
+      if (L % NT != 0 && f >= L) cok = false;
+      const int ioff = MODE == ROW_SIGNAL ? sx - g.h : sx - g.h - g.m;
+      // Elements outside the image read zero through the gather, so the
+      // loads of all rows are in flight together.
+#pragma unroll
+      for (int v = 0; v < V; ++v) {
+        const int r0 = 2 * (q0 + v);
+        const int i0 = ioff + r0;
+        const bool ra = cok && r0 < nrows && i0 >= 0 && i0 < g.H;
+        const bool rb = cok && r0 < nrows && i0 + 1 >= 0 && i0 + 1 < g.H;
+        av[u][v] = tr0[v] * wc * gather_load<T>(gi, gather_off<T>(gi, ra, i0 * g.W + j));
+        bv[u][v] = tr1[v] * wc * gather_load<T>(gi, gather_off<T>(gi, rb, (i0 + 1) * g.W + j));
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < TPT; ++u) {
+      const int e = tid + u * NT;
+      if (e < L) tw[e] = twr[u];
+    }
+#pragma unroll
+    for (int u = 0; u < TPT; ++u) {
+      const int f = tid + u * NT;
+      if (L % NT != 0 && f >= L) break;
+#pragma unroll
+      for (int v = 0; v < V; ++v) buf[v * VS + f] = cplx<T>{av[u][v], bv[u][v]};
+    }
+  } else {
+    T av[EPT], bv[EPT];
+#pragma unroll
+    for (int u = 0; u < EPT; ++u) {
+      const int e = tid + u * NT;
+      av[u] = bv[u] = T(0);
+      if (V * L % NT != 0 && e >= V * L) break;
+      const int v = e / L, f = e - (e / L) * L;
+      const int r0 = 2 * (q0 + v);
+      if constexpr (MODE == ROW_SIGNAL || MODE == ROW_COTANGENT) {
+        // Zero-reading gathers, as in the column mapping above.
+        const int c = f - g.m;
+        const int j = MODE == ROW_SIGNAL ? sy + c - g.h : sy + f - g.h - g.m;
+        const int i0 = MODE == ROW_SIGNAL ? sx + r0 - g.h : sx + r0 - g.h - g.m;
+        const bool cok = r0 < nrows && (MODE != ROW_SIGNAL || (c >= 0 && c < g.w)) && j >= 0 &&
+                         j < g.W;
+        const bool ra = cok && i0 >= 0 && i0 < g.H;
+        const bool rb = cok && i0 + 1 >= 0 && i0 + 1 < g.H;
+        const T wc = MODE == ROW_SIGNAL && cok ? tent_weight<T>(g.d, step, c) * scale : T(1);
+        const T wa = MODE == ROW_SIGNAL ? tent_weight<T>(g.d, step, r0) * wc : T(1);
+        const T wb = MODE == ROW_SIGNAL ? tent_weight<T>(g.d, step, r0 + 1) * wc : T(1);
+        av[u] = wa * gather_load<T>(gi, gather_off<T>(gi, ra, i0 * g.W + j));
+        bv[u] = wb * gather_load<T>(gi, gather_off<T>(gi, rb, (i0 + 1) * g.W + j));
+      } else if (r0 < nrows) {
+        if constexpr (MODE == ROW_KERNEL_WIN) {
+          const int c = f - g.m;
+          if (c >= 0 && c < g.w) {
+            av[u] = img[r0 * g.w + c];
+            bv[u] = img[(r0 + 1) * g.w + c];
           }
-        }
-      } else if constexpr (MODE == ROW_COTANGENT) {
-        const int j = sy + f - g.h - g.m;
-        const int i0 = sx + r0 - g.h - g.m;
-        if (j >= 0 && j < g.W) {
-          if (i0 >= 0 && i0 < g.H) av[u] = img[i0 * g.W + j];
-          if (i0 + 1 >= 0 && i0 + 1 < g.H) bv[u] = img[(i0 + 1) * g.W + j];
-        }
-      } else if constexpr (MODE == ROW_KERNEL_WIN) {
-        const int c = f - g.m;
-        if (c >= 0 && c < g.w) {
-          av[u] = img[r0 * g.w + c];
-          bv[u] = img[(r0 + 1) * g.w + c];
-        }
-      } else {  // ROW_KERNEL_FULL: cut[r][c] = K[(r - d) mod H][(c - d) mod W]
-        const int c = f - g.m;
-        if (c >= 0 && c < g.w) {
-          const int j = (c - g.d + g.W) % g.W;
-          const int ia = (r0 - g.d + g.H) % g.H;
-          const int ib = (r0 + 1 - g.d + g.H) % g.H;
-          av[u] = img[ia * g.W + j];
-          bv[u] = img[ib * g.W + j];
+        } else {  // ROW_KERNEL_FULL: cut[r][c] = K[(r - d) mod H][(c - d) mod W]
+          const int c = f - g.m;
+          if (c >= 0 && c < g.w) {
+            const int j = (c - g.d + g.W) % g.W;
+            const int ia = (r0 - g.d + g.H) % g.H;
+            const int ib = (r0 + 1 - g.d + g.H) % g.H;
+            av[u] = img[ia * g.W + j];
+            bv[u] = img[ib * g.W + j];
+          }
         }
       }
     }
-  }
 #pragma unroll
-  for (int u = 0; u < TPT; ++u) {
-    const int e = tid + u * NT;
-    if (e < L) tw[e] = twr[u];
-  }
+    for (int u = 0; u < TPT; ++u) {
+      const int e = tid + u * NT;
+      if (e < L) tw[e] = twr[u];
+    }
 #pragma unroll
-  for (int u = 0; u < EPT; ++u) {
-    const int e = tid + u * NT;
-    if (V * L % NT != 0 && e >= V * L) break;
-    const int v = e / L, f = e - (e / L) * L;
-    buf[v * VS + f] = cplx<T>{av[u], bv[u]};
+    for (int u = 0; u < EPT; ++u) {
+      const int e = tid + u * NT;
+      if (V * L % NT != 0 && e >= V * L) break;
+      const int v = e / L, f = e - (e / L) * L;
+      buf[v * VS + f] = cplx<T>{av[u], bv[u]};
+    }
   }
   __syncthreads();
   fft_lds<T, L, V, VS, NT, false>(buf, tw, tid);
@@ -620,8 +674,11 @@ __global__ __launch_bounds__(NT) void lp_fwd_out(LinGeom g, const cplx<T>* __res
         const int d = cnt_v[t];
         T r = T(0);
         if (d >= 0) {
-          const double dl = (double)lam;
-          nll += dl - (double)d * log(dl);
+          // The log of an FP32 lam is taken in FP32 (<= 2 ulp; the gradient
+          // does not use it), the terms are summed in FP64.
+          const double lg = std::is_same<T, float>::value ? (double)logf((float)lam)
+                                                          : log((double)lam);
+          nll += (double)lam - (double)d * lg;
           r = ex * (T(1) - T(d) / lam);
         }
         resid[row + j] = r;
