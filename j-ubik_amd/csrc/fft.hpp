@@ -295,7 +295,25 @@ struct RegDft<T, N, INV, false> {
 // Forward radix sequence per length; inverse transforms use it reversed, so
 // the last forward stage and the first inverse stage share their register
 // layout (tasks (vec, j) holding positions j + t L/R).
-__host__ __device__ constexpr int plan_radix(int L, int s) {
+//
+// Plan kind 0 takes the largest radices (fewest LDS passes; the column pass
+// keeps its first and last stage in registers).  Plan kind 1 is for tiles of
+// few vectors (V = 2 or 4 with 256 threads): smaller radices give every stage
+// about one butterfly per thread, where large radices leave most waves idle
+// while one wave runs the long butterflies.  Lengths without an entry use
+// kind 0.
+__host__ __device__ constexpr int plan_radix(int L, int s, int pl = 0) {
+  if (pl == 1) {
+    int w0 = 0, w1 = 0, w2 = 0;
+    switch (L) {
+      case 256: w0 = 4; w1 = 8; w2 = 8; break;
+      case 384: w0 = 8; w1 = 8; w2 = 6; break;
+      case 512: w0 = 8; w1 = 8; w2 = 8; break;
+      case 1024: w0 = 8; w1 = 8; w2 = 16; break;
+      default: break;
+    }
+    if (w0 > 0) return s == 0 ? w0 : s == 1 ? w1 : s == 2 ? w2 : 0;
+  }
   int r0 = 0, r1 = 0, r2 = 0;
   switch (L) {
     case 2: r0 = 2; break;
@@ -321,22 +339,28 @@ __host__ __device__ constexpr int plan_radix(int L, int s) {
   }
   return s == 0 ? r0 : s == 1 ? r1 : s == 2 ? r2 : 0;
 }
-__host__ __device__ constexpr int plan_stages(int L) {
+__host__ __device__ constexpr int plan_stages(int L, int pl = 0) {
   int n = 0;
-  while (n < 3 && plan_radix(L, n) > 0) ++n;
+  while (n < 3 && plan_radix(L, n, pl) > 0) ++n;
   return n;
 }
 __host__ __device__ constexpr bool fft_length_ok(int L) { return plan_radix(L, 0) > 0; }
 // Radix of stage s in execution order (reversed for the inverse).
-__host__ __device__ constexpr int stage_radix(int L, bool inv, int s) {
-  const int n = plan_stages(L);
+__host__ __device__ constexpr int stage_radix(int L, bool inv, int s, int pl = 0) {
+  const int n = plan_stages(L, pl);
   if (s >= n) return 0;
-  return plan_radix(L, inv ? n - 1 - s : s);
+  return plan_radix(L, inv ? n - 1 - s : s, pl);
 }
-__host__ __device__ constexpr int stage_ns(int L, bool inv, int s) {
+__host__ __device__ constexpr int stage_ns(int L, bool inv, int s, int pl = 0) {
   int ns = 1;
-  for (int i = 0; i < s; ++i) ns *= stage_radix(L, inv, i);
+  for (int i = 0; i < s; ++i) ns *= stage_radix(L, inv, i, pl);
   return ns;
+}
+// Every plan multiplies out to its length.
+__host__ __device__ constexpr bool plan_ok(int L, int pl) {
+  int p = 1;
+  for (int s = 0; s < plan_stages(L, pl); ++s) p *= plan_radix(L, s, pl);
+  return p == L;
 }
 // Odd vector stride (in complex elements) for V vectors of length L.
 __host__ __device__ constexpr int odd_stride(int L) { return L | 1; }
@@ -350,10 +374,10 @@ __host__ __device__ constexpr int lds_stride(int L, int V) {
 }
 
 // ---------------------------------------------------------------- stages
-template <typename T, int L, bool INV, int S, int V, int VS, int NT>
+template <typename T, int L, bool INV, int S, int V, int VS, int NT, int PL = 0>
 struct Stage {
-  static constexpr int R = stage_radix(L, INV, S);
-  static constexpr int Ns = stage_ns(L, INV, S);
+  static constexpr int R = stage_radix(L, INV, S, PL);
+  static constexpr int Ns = stage_ns(L, INV, S, PL);
   static constexpr int NB = L / R;
   static constexpr int TOT = V * NB;
   static constexpr int BPT = (TOT + NT - 1) / NT;
@@ -449,27 +473,39 @@ struct Stage {
 };
 
 // Stages S..end, LDS -> LDS (a barrier after every scatter).
-template <typename T, int L, bool INV, int S, int V, int VS, int NT>
+template <typename T, int L, bool INV, int S, int V, int VS, int NT, int PL = 0>
 __device__ __forceinline__ void fft_stages_lds(cplx<T>* buf, const cplx<T>* tw, int tid) {
-  if constexpr (S < plan_stages(L)) {
-    using St = Stage<T, L, INV, S, V, VS, NT>;
+  if constexpr (S < plan_stages(L, PL)) {
+    using St = Stage<T, L, INV, S, V, VS, NT, PL>;
     typename St::Regs v;
     St::gather_lds(buf, v, tid);
     St::compute(v, tw, tid);
     __syncthreads();
     St::scatter_lds(buf, v, tid);
     __syncthreads();
-    fft_stages_lds<T, L, INV, S + 1, V, VS, NT>(buf, tw, tid);
+    fft_stages_lds<T, L, INV, S + 1, V, VS, NT, PL>(buf, tw, tid);
   }
 }
 
-// Whole transform of V vectors in LDS (stride VS).  Caller synchronises
-// after filling ``buf`` and ``tw``; the routine ends with a barrier.
-template <typename T, int L, int V, int VS, int NT, bool INV>
+// Whole transform of V vectors in LDS (stride VS) with plan kind PL.  Caller
+// synchronises after filling ``buf`` and ``tw``; the routine ends with a
+// barrier.
+template <typename T, int L, int V, int VS, int NT, bool INV, int PL = 0>
 __device__ __forceinline__ void fft_lds(cplx<T>* buf, const cplx<T>* tw, int tid) {
   static_assert(fft_length_ok(L), "FFT length has no plan");
-  fft_stages_lds<T, L, INV, 0, V, VS, NT>(buf, tw, tid);
+  static_assert(plan_ok(L, PL), "plan radices do not multiply to the length");
+  fft_stages_lds<T, L, INV, 0, V, VS, NT, PL>(buf, tw, tid);
 }
+
+// Plan kind for tiles of V vectors of length L on NT threads: kind 1 when
+// the stage of the largest radix would have fewer butterflies than threads.
+__host__ __device__ constexpr int plan_max_radix(int L, int pl) {
+  int r = 0;
+  for (int s = 0; s < plan_stages(L, pl); ++s) r = plan_radix(L, s, pl) > r ? plan_radix(L, s, pl) : r;
+  return r;
+}
+template <int L, int V, int NT>
+inline constexpr int kFewVectorPlan = (V * L / plan_max_radix(L, 0) < NT) ? 1 : 0;
 
 template <typename T>
 __device__ __forceinline__ void load_twiddles(cplx<T>* tw_lds, const cplx<T>* tw_glob, int L,

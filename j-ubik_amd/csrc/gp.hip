@@ -109,7 +109,7 @@ __global__ __launch_bounds__(GNT) void gp_cols(GpGeom g, const T* __restrict__ a
     if (e < N) tw[e] = twr[u2];
   }
   __syncthreads();
-  fft_lds<T, N, V, VS, GNT, false>(buf, tw, tid);
+  fft_lds<T, N, V, VS, GNT, false, kFewVectorPlan<N, V, GNT>>(buf, tw, tid);
   for (int e = tid; e < V * N; e += GNT) {
     const int x1 = e / V, v = e - (e / V) * V;
     Tw[(int64_t)x1 * N + c0 + v] = buf[v * VS + x1];
@@ -174,7 +174,7 @@ __global__ __launch_bounds__(GNT) void gp_rows(GpGeom g, const cplx<T>* __restri
     if (e < N) tw[e] = twr[u2];
   }
   __syncthreads();
-  fft_lds<T, N, 2, VS, GNT, false>(buf, tw, tid);
+  fft_lds<T, N, 2, VS, GNT, false, kFewVectorPlan<N, 2, GNT>>(buf, tw, tid);
   const int nrows = (r0 == r1) ? 1 : 2;
   for (int e = tid; e < nrows * N; e += GNT) {
     const int v = e / N, k = e - (e / N) * N;

@@ -203,7 +203,7 @@ __global__ __launch_bounds__(NT) void lp_rows(LinGeom g, const T* __restrict__ s
     }
   }
   __syncthreads();
-  fft_lds<T, L, V, VS, NT, false>(buf, tw, tid);
+  fft_lds<T, L, V, VS, NT, false, kFewVectorPlan<L, V, NT>>(buf, tw, tid);
 
   cplx<T>* out = R1 + r1base * L2P * nrows;
   for (int e = tid; e < V * LH; e += NT) {
@@ -624,7 +624,7 @@ __global__ __launch_bounds__(NT) void lp_fwd_out(LinGeom g, const cplx<T>* __res
       }
     }
     __syncthreads();
-    fft_lds<T, L, V, VS, NT, true>(buf, tw, tid);
+    fft_lds<T, L, V, VS, NT, true, kFewVectorPlan<L, V, NT>>(buf, tw, tid);
 #pragma unroll
     for (int t = 0; t < JPT; ++t) {
       const int j = tid + t * NT;
@@ -772,7 +772,7 @@ __global__ __launch_bounds__(NT) void lp_adj_out(LinGeom g, const cplx<T>* __res
         first = false;
       }
       __syncthreads();
-      fft_lds<T, L, V, VS, NT, true>(buf, tw, tid);
+      fft_lds<T, L, V, VS, NT, true, kFewVectorPlan<L, V, NT>>(buf, tw, tid);
 #pragma unroll
       for (int t = 0; t < JPT; ++t) {
         const int j = tid + t * NT;

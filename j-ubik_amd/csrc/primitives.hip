@@ -155,7 +155,7 @@ __global__ __launch_bounds__(PNT) void k_conv_cols(ConvGeom g, const T* __restri
     buf[v * VS + r] = PASS == 0 ? cplx<T>{x[kb * plane + o], y[b * plane + o]} : Tw[b * plane + o];
   }
   __syncthreads();
-  fft_lds<T, N, V, VS, PNT, PASS == 1>(buf, tw, tid);
+  fft_lds<T, N, V, VS, PNT, PASS == 1, kFewVectorPlan<N, V, PNT>>(buf, tw, tid);
   for (int e = tid; e < V * N; e += PNT) {
     const int r = e / V, v = e - (e / V) * V;
     const int64_t o = (int64_t)r * N + c0 + v;
@@ -183,7 +183,7 @@ __global__ __launch_bounds__(PNT) void k_conv_rows(ConvGeom g, const cplx<T>* __
     buf[v * VS + k] = base[(int64_t)(v ? r1 : r0) * N + k];
   }
   __syncthreads();
-  fft_lds<T, N, 2, VS, PNT, false>(buf, tw, tid);
+  fft_lds<T, N, 2, VS, PNT, false, kFewVectorPlan<N, 2, PNT>>(buf, tw, tid);
   constexpr int EPT = (2 * N + PNT - 1) / PNT;
   cplx<T> prod[EPT];
 #pragma unroll
@@ -205,7 +205,7 @@ __global__ __launch_bounds__(PNT) void k_conv_rows(ConvGeom g, const cplx<T>* __
     buf[(e / N) * VS + e - (e / N) * N] = prod[u];
   }
   __syncthreads();
-  fft_lds<T, N, 2, VS, PNT, true>(buf, tw, tid);
+  fft_lds<T, N, 2, VS, PNT, true, kFewVectorPlan<N, 2, PNT>>(buf, tw, tid);
   const int nrows = r0 == r1 ? 1 : 2;
   for (int e = tid; e < nrows * N; e += PNT) {
     const int v = e / N, k = e - (e / N) * N;

@@ -26,6 +26,7 @@ GEOMS = [
     (192, 3, 32),   # L = 192
     (128, 4, 48),   # L = 160? -> not compiled: skipped below
     (512, 8, 64),   # L = 256
+    (256, 2, 64),   # L = 384
     (384, 2, 0),    # placeholder, margin 0 handled separately
 ]
 
@@ -62,7 +63,7 @@ def test_forward_matches_oracle(ja, H, n, m, dtype):
     assert err <= tol
 
 
-@pytest.mark.parametrize("H,n,m", _cases()[:5])
+@pytest.mark.parametrize("H,n,m", _cases()[:5] + [(512, 8, 64), (256, 2, 64)])
 @pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
 def test_adjoint_matches_oracle(ja, H, n, m, dtype):
     E = 2

@@ -17,7 +17,8 @@ def _rel(a, b):
     return float(np.linalg.norm(np.ravel(a) - np.ravel(b)) / np.linalg.norm(np.ravel(b)))
 
 
-@pytest.mark.parametrize("N,H,E", [(64, 64, 1), (64, 48, 3), (256, 256, 4)])
+@pytest.mark.parametrize("N,H,E", [(64, 64, 1), (64, 48, 3), (256, 256, 4), (512, 512, 2),
+                                   (1024, 1000, 2)])
 @pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
 def test_gp_forward_and_vjp(ja, N, H, E, dtype):
     rng = np.random.default_rng(N + H + E)
@@ -87,6 +88,7 @@ def test_readout_matches_reference_fixture(ja):
     dict(H=64, n=4, margin=8, E=1, M=1, seed=1),
     dict(H=64, n=4, margin=8, E=3, M=2, seed=2),
     dict(H=128, n=4, margin=16, E=2, M=1, seed=3, N=128),
+    dict(H=512, n=4, margin=64, E=2, M=1, seed=4, N=512),  # L = 384, small-radix plans
 ])
 @pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
 def test_model_grad_matches_oracle(ja, cfg, dtype):
