@@ -67,20 +67,27 @@ __global__ __launch_bounds__(GNT) void gp_cols(GpGeom g, const T* __restrict__ a
   // a chunk's loads of one plane are in flight together.
   const int64_t plane = (int64_t)g.H * g.W;
   const Gather gu = make_gather(u, MODE == 1 ? g.uplanes * plane : 0);
+  const int64_t nn = MODE == 0 ? (int64_t)N * N : 0;
+  const Gather gar = make_gather(amp_ref, nn), gxr = make_gather(xi_ref, nn);
+  const Gather gaa = make_gather(amp_alpha, g.has_alpha ? nn : 0);
+  const Gather gxa = make_gather(xi_alpha, g.has_alpha ? nn : 0);
   for (int u0 = 0; u0 < EPT; u0 += CHK) {
     cplx<T> val[CHK];
 #pragma unroll
     for (int c = 0; c < CHK; ++c) val[c] = cplx<T>{T(0), T(0)};
     if constexpr (MODE == 0) {
+      // Zero-reading gathers (the alpha field reads zero when absent): the
+      // chunk's loads of all four arrays are in flight together.
 #pragma unroll
       for (int c = 0; c < CHK; ++c) {
         const int e = tid + (u0 + c) * GNT;
         const int x1 = e / V, v = e - (e / V) * V;
         const int x2 = c0 + v;
-        if (e >= V * N) continue;
-        const int64_t o = (int64_t)x1 * N + x2;
-        val[c].re = amp_ref[o] * xi_ref[o];
-        if (g.has_alpha) val[c].im = amp_alpha[o] * xi_alpha[o];
+        const bool ok = V * N % GNT == 0 || e < V * N;
+        const int o = x1 * N + x2;
+        const int orf = gather_off<T>(gar, ok, o), oal = gather_off<T>(gaa, ok && g.has_alpha, o);
+        val[c].re = gather_load<T>(gar, orf) * gather_load<T>(gxr, orf);
+        val[c].im = gather_load<T>(gaa, oal) * gather_load<T>(gxa, oal);
       }
     } else {
       for (int pl = 0; pl < g.uplanes; ++pl) {

@@ -303,16 +303,32 @@ __global__ __launch_bounds__(NT) void lp_cols(LinGeom g, const cplx<T>* __restri
   const cplx<T>* in = R1 + idx * L2P * nin;
   const int m = g.m, w = g.w;
 
-  load_twiddles(tw, twg, L, tid, NT);
+  // Twiddles are fetched into registers now and stored to LDS once the tile
+  // loads are issued: stage 0 needs none, and the barrier after it orders
+  // the stores before their first use.
+  constexpr int TPT = (L + NT - 1) / NT;
+  cplx<T> twr[TPT];
+#pragma unroll
+  for (int u = 0; u < TPT; ++u) {
+    const int e = tid + u * NT;
+    if (e < L) twr[u] = twg[e];
+  }
+  auto store_twiddles = [&]() {
+#pragma unroll
+    for (int u = 0; u < TPT; ++u) {
+      const int e = tid + u * NT;
+      if (e < L) tw[e] = twr[u];
+    }
+  };
   T factor = T(1);
   if constexpr (MODE == COL_KERNEL) {
     double s = 0.0;
     for (int rr = tid; rr < w; rr += NT) s += (double)in[rr * L2P].re;
     s = block_sum<NT>(s, red);
     if (tid == 0) total_s = s;
+    __syncthreads();
+    factor = T(1.0 / (total_s * (double)L * (double)L));
   }
-  __syncthreads();
-  if constexpr (MODE == COL_KERNEL) factor = T(1.0 / (total_s * (double)L * (double)L));
 
   // forward stage 0 straight from global memory
   using F0 = Stage<T, L, false, 0, V, VS, NT>;
@@ -341,12 +357,16 @@ __global__ __launch_bounds__(NT) void lp_cols(LinGeom g, const cplx<T>* __restri
       return (rr >= 0 && rr < w) ? ld_stream(&in[rr * L2P + k]) : cplx<T>{T(0), T(0)};
     }
   };
+  static_assert(F0::Ns == 1, "the first stage takes no twiddles");
   if constexpr (NS == 1) {
     F0::gather(last, tid, load_col);
+    store_twiddles();
+    __syncthreads();
     F0::compute(last, tw, tid);
   } else {
     typename F0::Regs v0;
     F0::gather(v0, tid, load_col);
+    store_twiddles();
     F0::compute(v0, tw, tid);
     F0::scatter_lds(buf, v0, tid);
     __syncthreads();
