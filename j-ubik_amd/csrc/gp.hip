@@ -167,6 +167,23 @@ __global__ __launch_bounds__(GNT) void gp_rows(GpGeom g, const cplx<T>* __restri
     const int e = tid + u2 * GNT;
     if (e < N) twr[u2] = twg[e];
   }
+  // MODE 1: the amplitudes of the epilogue are fetched with the rows (zero-
+  // reading gathers; alpha reads zero when absent), so their latency
+  // overlaps the transform.
+  T ampa[MODE == 1 ? EPT : 1], ampb[MODE == 1 ? EPT : 1];
+  if constexpr (MODE == 1) {
+    const Gather ga = make_gather(amp_ref, (int64_t)N * N);
+    const Gather gb = make_gather(amp_alpha, g.has_alpha ? (int64_t)N * N : 0);
+#pragma unroll
+    for (int u2 = 0; u2 < EPT; ++u2) {
+      const int e = tid + u2 * GNT;
+      const int v = e / N, k = e - (e / N) * N;
+      const bool ok = 2 * N % GNT == 0 || e < 2 * N;
+      const int o = (v ? r1 : r0) * N + k;
+      ampa[u2] = gather_load<T>(ga, gather_off<T>(ga, ok, o));
+      ampb[u2] = gather_load<T>(gb, gather_off<T>(gb, ok && g.has_alpha, o));
+    }
+  }
 #pragma unroll
   for (int u2 = 0; u2 < EPT; ++u2) {
     const int e = tid + u2 * GNT;
@@ -183,7 +200,10 @@ __global__ __launch_bounds__(GNT) void gp_rows(GpGeom g, const cplx<T>* __restri
   __syncthreads();
   fft_lds<T, N, 2, VS, GNT, false, kFewVectorPlan<N, 2, GNT>>(buf, tw, tid);
   const int nrows = (r0 == r1) ? 1 : 2;
-  for (int e = tid; e < nrows * N; e += GNT) {
+#pragma unroll
+  for (int u2 = 0; u2 < EPT; ++u2) {
+    const int e = tid + u2 * GNT;
+    if (e >= nrows * N) continue;
     const int v = e / N, k = e - (e / N) * N;
     const int x = v ? r1 : r0;
     const cplx<T> z = buf[v * VS + k];
@@ -192,8 +212,8 @@ __global__ __launch_bounds__(GNT) void gp_rows(GpGeom g, const cplx<T>* __restri
     const T htb = T(0.5) * (z.im + zp.im - z.re + zp.re);
     if constexpr (MODE == 1) {
       const int64_t o = (int64_t)x * N + k;
-      out_a[o] = amp_ref[o] * hta;
-      if (g.has_alpha) out_b[o] = amp_alpha[o] * htb;
+      out_a[o] = ampa[u2] * hta;
+      if (g.has_alpha) out_b[o] = ampb[u2] * htb;
     } else {
       if (x >= g.H || k >= g.W) continue;
       for (int en = 0; en < g.E; ++en) {
