@@ -481,9 +481,17 @@ struct OutTile {
   static constexpr int TPT = (L + NT - 1) / NT;       // twiddles per thread
 };
 
-// First patch column whose window [s, s + len) can contain canvas column C.
-__device__ __forceinline__ int first_cover(int C, int len, int d) {
-  return C >= len ? (C - len) / d + 1 : 0;
+// First patch column whose window [s, s + len) can contain canvas column C:
+// (C - len) / d + 1 for C >= len.  The quotient comes from the float
+// reciprocal rd = 1 / d with one correction step (exact for C < 2^22), in
+// place of a full integer division per column.
+__device__ __forceinline__ int first_cover(int C, int len, int d, float rd) {
+  if (C < len) return 0;
+  const int a = C - len;
+  int q = (int)((float)a * rd);
+  const int r = a - q * d;
+  q += (r >= d) - (r < 0);
+  return q + 1;
 }
 
 // Patch rows covering canvas row R: windows [start_x(ix), start_x(ix) + len)
@@ -519,7 +527,11 @@ __device__ __forceinline__ void sum_pair_spectra(cplx<T>* buf, const cplx<T>* __
   constexpr int V = OT::V, LH = OT::LH, GPT = OT::GPT;
   constexpr int VS = lds_stride(L, V);
   constexpr int L2P = half_stride(L);
-  constexpr int CH = 3;  // covering rows per load batch (L <= 3d in the common geometries)
+  // Covering rows per load batch: the forward's windows of L <= 3d rows meet
+  // a canvas row at most 3 times in the common geometries, the adjoint's
+  // windows of w = 2d rows exactly twice (a third slot would only hold
+  // registers).
+  constexpr int CH = WEIGHTED ? 2 : 3;
   const int npair = (g.n + 1) / 2;
   const int pstride = nrows * L2P;  // n <= 64, L <= 384: offsets fit in 31 bits
   int oa[GPT], ob[GPT];
@@ -620,27 +632,35 @@ __global__ __launch_bounds__(NT) void lp_fwd_out(LinGeom g, const cplx<T>* __res
   for (int t = 0; t < JPT; ++t) acc[t] = T(0);
   const int64_t row = ((int64_t)job * g.H + i) * g.W;
   const int64_t rowd = ((int64_t)(job % g.J) * g.H + i) * g.W;
+  const float rd = 1.0f / (float)g.d;
   T ex_v[JPT], lam_v[JPT];
   int cnt_v[JPT];
-  if constexpr (EPI != 0) {
-#pragma unroll
-    for (int t = 0; t < JPT; ++t) {
-      const int j = tid + t * NT;
-      ex_v[t] = j < g.W ? expo[rowd + j] : T(0);
-      cnt_v[t] = j < g.W ? counts[rowd + j] : -1;
-      if (EPI == 2) lam_v[t] = j < g.W ? y[rowd + j] : T(1);
-    }
-  }
   const int npair = (g.n + 1) / 2;
   const cplx<T>* base = R2 + (int64_t)job * g.P * L * L2P;
 
-  for (int q0 = 0; q0 < npair; q0 += V) {
+  // One batch of V patch-column pairs.  The first batch (all of them for
+  // n <= 2V) is peeled into straight-line code: inside the batch loop the
+  // loads pending at its head (twiddles, epilogue operands) would be waited
+  // for before the spectrum loads are issued.
+  auto pass = [&](int q0, auto first) {
     sum_pair_spectra<T, L, false>(buf, base, g, L, R, lo, nc, nullptr, q0, tid);
-    if (q0 == 0) {
+    if constexpr (decltype(first)::value) {
 #pragma unroll
       for (int u = 0; u < TPT; ++u) {
         const int e = tid + u * NT;
         if (e < L) tw[e] = twr[u];
+      }
+      // Epilogue operands are fetched once the spectrum loads are issued
+      // and consumed (fetched earlier, they hold those loads back); their
+      // latency overlaps the transform.
+      if constexpr (EPI != 0) {
+#pragma unroll
+        for (int t = 0; t < JPT; ++t) {
+          const int j = tid + t * NT;
+          ex_v[t] = j < g.W ? expo[rowd + j] : T(0);
+          cnt_v[t] = j < g.W ? counts[rowd + j] : -1;
+          if (EPI == 2) lam_v[t] = j < g.W ? y[rowd + j] : T(1);
+        }
       }
     }
     __syncthreads();
@@ -651,7 +671,7 @@ __global__ __launch_bounds__(NT) void lp_fwd_out(LinGeom g, const cplx<T>* __res
       if (j < g.W) {
         const int C = j + g.h + g.m;
         T s = T(0);
-        for (int iy = first_cover(C, L, g.d); iy < g.n; ++iy) {
+        for (int iy = first_cover(C, L, g.d, rd); iy < g.n; ++iy) {
           const int sy = start_y(g, iy);
           if (sy > C) break;
           const int c = C - sy;
@@ -665,7 +685,9 @@ __global__ __launch_bounds__(NT) void lp_fwd_out(LinGeom g, const cplx<T>* __res
       }
     }
     __syncthreads();
-  }
+  };
+  pass(0, std::true_type{});
+  for (int q0 = V; q0 < npair; q0 += V) pass(q0, std::false_type{});
 
   if constexpr (EPI == 0) {
 #pragma unroll
@@ -764,56 +786,64 @@ __global__ __launch_bounds__(NT) void lp_adj_out(LinGeom g, const cplx<T>* __res
     jstep = 1;
   }
 
-  // EPI 1: the sky row of this image, fetched before the spectrum loads.
+  // EPI 1: the sky row of this image, fetched once the first spectrum loads
+  // are issued and consumed (its latency then overlaps the transform).
   T sky_v[JPT];
-  if (EPI == 1) {
-    const int simg = g.sky_shared ? eb : blockIdx.y;
-    const T* srow = sky + ((int64_t)simg * g.H + i) * g.W;
+  const float rd = 1.0f / (float)g.d;
+  // One batch of V patch-column pairs of one job; the first batch of the
+  // first job is peeled into straight-line code (see lp_fwd_out).
+  auto pass = [&](const cplx<T>* base, int q0, auto first) {
+    sum_pair_spectra<T, L, true>(buf, base, g, g.w, Rp, lo, nc, tent, q0, tid);
+    if constexpr (decltype(first)::value) {
+#pragma unroll
+      for (int uu = 0; uu < TPT; ++uu) {
+        const int e = tid + uu * NT;
+        if (e < L) tw[e] = twr[uu];
+        if (e < g.w) tent_s[e] = tnr[uu];
+      }
+      if constexpr (EPI == 1) {
+        const int simg = g.sky_shared ? eb : blockIdx.y;
+        const T* srow = sky + ((int64_t)simg * g.H + i) * g.W;
+#pragma unroll
+        for (int t = 0; t < JPT; ++t) {
+          const int j = tid + t * NT;
+          sky_v[t] = j < g.W ? srow[j] : T(0);
+        }
+      }
+    }
+    __syncthreads();
+    fft_lds<T, L, V, VS, NT, true, kFewVectorPlan<L, V, NT>>(buf, tw, tid);
 #pragma unroll
     for (int t = 0; t < JPT; ++t) {
       const int j = tid + t * NT;
-      sky_v[t] = j < g.W ? srow[j] : T(0);
+      if (j < g.W) {
+        const int Cp = j + g.h;
+        T s = T(0);
+        for (int iy = first_cover(Cp, g.w, g.d, rd); iy < g.n; ++iy) {
+          const int sy = start_y(g, iy);
+          if (sy > Cp) break;
+          const int c = Cp - sy;
+          const int v = (iy >> 1) - q0;
+          if (c < g.w && v >= 0 && v < V) {
+            const cplx<T> z = buf[v * VS + g.m + c];
+            s += tent_s[c] * ((iy & 1) ? z.im : z.re);
+          }
+        }
+        tmp[t] += s;
+      }
     }
-  }
-  bool first = true;
+    __syncthreads();
+  };
   for (int job = jb; job < je; job += jstep) {
     const cplx<T>* base = R2 + (int64_t)job * g.P * g.w * L2P;
 #pragma unroll
     for (int t = 0; t < JPT; ++t) tmp[t] = T(0);
-    for (int q0 = 0; q0 < npair; q0 += V) {
-      sum_pair_spectra<T, L, true>(buf, base, g, g.w, Rp, lo, nc, tent, q0, tid);
-      if (first) {
-#pragma unroll
-        for (int uu = 0; uu < TPT; ++uu) {
-          const int e = tid + uu * NT;
-          if (e < L) tw[e] = twr[uu];
-          if (e < g.w) tent_s[e] = tnr[uu];
-        }
-        first = false;
-      }
-      __syncthreads();
-      fft_lds<T, L, V, VS, NT, true, kFewVectorPlan<L, V, NT>>(buf, tw, tid);
-#pragma unroll
-      for (int t = 0; t < JPT; ++t) {
-        const int j = tid + t * NT;
-        if (j < g.W) {
-          const int Cp = j + g.h;
-          T s = T(0);
-          for (int iy = first_cover(Cp, g.w, g.d); iy < g.n; ++iy) {
-            const int sy = start_y(g, iy);
-            if (sy > Cp) break;
-            const int c = Cp - sy;
-            const int v = (iy >> 1) - q0;
-            if (c < g.w && v >= 0 && v < V) {
-              const cplx<T> z = buf[v * VS + g.m + c];
-              s += tent_s[c] * ((iy & 1) ? z.im : z.re);
-            }
-          }
-          tmp[t] += s;
-        }
-      }
-      __syncthreads();
+    int q0 = 0;
+    if (job == jb) {
+      pass(base, 0, std::true_type{});
+      q0 = V;
     }
+    for (; q0 < npair; q0 += V) pass(base, q0, std::false_type{});
     if (EPI == 0) {
 #pragma unroll
       for (int t = 0; t < JPT; ++t) acc[t] += tmp[t];
