@@ -99,13 +99,20 @@ __global__ __launch_bounds__(NT) void lp_rows(LinGeom g, const T* __restrict__ s
   const Gather gi = make_gather(img, (int64_t)g.H * g.W);
   if constexpr (kColumns) {
     T av[TPT][V], bv[TPT][V];
+    // Tent weights from the plan's table (the same IEEE double values in T):
+    // the row weights are uniform (scalar loads), the column weights one
+    // zero-reading gather per column.
     T tr0[V], tr1[V];
 #pragma unroll
     for (int v = 0; v < V; ++v) {
       const int r0 = 2 * (q0 + v);
-      tr0[v] = MODE == ROW_SIGNAL ? tent_weight<T>(g.d, step, r0) : T(1);
-      tr1[v] = MODE == ROW_SIGNAL ? tent_weight<T>(g.d, step, r0 + 1) : T(1);
+      tr0[v] = tr1[v] = T(1);
+      if (MODE == ROW_SIGNAL && r0 < nrows) {
+        tr0[v] = tent[r0];
+        tr1[v] = tent[r0 + 1];
+      }
     }
+    const Gather gt = make_gather(tent, MODE == ROW_SIGNAL ? (int64_t)g.w : 0);
 #pragma unroll
     for (int u = 0; u < TPT; ++u) {
       const int f = tid + u * NT;
@@ -116,7 +123,7 @@ __global__ __launch_bounds__(NT) void lp_rows(LinGeom g, const T* __restrict__ s
         const int c = f - g.m;
         j = sy + c - g.h;
         cok = c >= 0 && c < g.w && j >= 0 && j < g.W;
-        if (cok) wc = tent_weight<T>(g.d, step, c) * scale;
+        wc = gather_load<T>(gt, gather_off<T>(gt, cok, c)) * scale;
       } else {
         j = sy + f - g.h - g.m;
         cok = j >= 0 && j < g.W;
@@ -206,8 +213,17 @@ __global__ __launch_bounds__(NT) void lp_rows(LinGeom g, const T* __restrict__ s
   fft_lds<T, L, V, VS, NT, false, kFewVectorPlan<L, V, NT>>(buf, tw, tid);
 
   cplx<T>* out = R1 + r1base * L2P * nrows;
-  for (int e = tid; e < V * LH; e += NT) {
-    const int v = e / LH, k = e - (e / LH) * LH;
+  // When L/2 is a multiple of the wave size the row pair v of a store is
+  // uniform over each wave, so its row base is scalar.
+  constexpr int SPT = (V * LH + NT - 1) / NT;
+  constexpr bool kWaveUniform = LH % 64 == 0 && NT % 64 == 0;
+#pragma unroll
+  for (int u = 0; u < SPT; ++u) {
+    const int e = tid + u * NT;
+    if (V * LH % NT != 0 && e >= V * LH) break;
+    int v = e / LH;
+    const int k = e - v * LH;
+    if constexpr (kWaveUniform) v = __builtin_amdgcn_readfirstlane(v);
     const int r0 = 2 * (q0 + v);
     if (r0 >= nrows) continue;
     const bool st0 = MODE != ROW_COTANGENT || (row0 + r0 >= 0 && row0 + r0 < g.H);
@@ -224,8 +240,11 @@ __global__ __launch_bounds__(NT) void lp_rows(LinGeom g, const T* __restrict__ s
       A = {T(0.5) * (zk.re + zn.re), T(0.5) * (zk.im - zn.im)};
       B = {T(0.5) * (zk.im + zn.im), T(0.5) * (zn.re - zk.re)};
     }
-    if (st0) st_stream(&out[r0 * L2P + k], A);
-    if (st1) st_stream(&out[(r0 + 1) * L2P + k], B);
+    // Row base (scalar when v is) plus a 32-bit lane byte offset.
+    char* orow = reinterpret_cast<char*>(out + (int64_t)r0 * L2P);
+    const uint32_t kb = (uint32_t)k * (uint32_t)sizeof(cplx<T>);
+    if (st0) st_stream(reinterpret_cast<cplx<T>*>(orow + kb), A);
+    if (st1) st_stream(reinterpret_cast<cplx<T>*>(orow + L2P * sizeof(cplx<T>) + kb), B);
   }
 }
 
