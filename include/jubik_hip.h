@@ -210,6 +210,20 @@ int jubik_model_set_data(jubik_model* model, const void* exposure, const int32_t
 int jubik_model_grad(jubik_model* model, const void* amp_ref, const void* amp_alpha,
                      const void* xi_ref, const void* xi_alpha, double zm, double mu,
                      const void* dnu, double* nll, void* g_ref, void* g_alpha, void* stream);
+/* jubik_model_grad plus the gradients with respect to the sky model's
+ * amplitude fields and offsets, the inputs of the reference's amplitude
+ * models and zero-mode priors (sky composition
+ * jubik/sky_model/multifrequency/spectral_product_mf_sky.py:171-298; the
+ * amplitude models of spectral_product_utils/normalized_amplitude_model.py:61-139
+ * produce amp_ref / amp_alpha): g_amp_ref = xi_ref * HT(u_ref) and
+ * g_amp_alpha = xi_alpha * HT(u_alpha) (N x N, model dtype), g_offsets =
+ * (d NLL / d zm, d NLL / d mu_alpha) (two values, model dtype), where u is
+ * the cotangent of log s.  Sets the metric base point like
+ * jubik_model_grad. */
+int jubik_model_grad_hyper(jubik_model* model, const void* amp_ref, const void* amp_alpha,
+                           const void* xi_ref, const void* xi_alpha, double zm, double mu,
+                           const void* dnu, double* nll, void* g_ref, void* g_alpha,
+                           void* g_amp_ref, void* g_amp_alpha, void* g_offsets, void* stream);
 /* MGVI metric product J^T diag(1/lam) J v at the position of the last
  * jubik_model_grad call (the metric nifty.re's sampling uses). */
 int jubik_model_metric(jubik_model* model, const void* amp_ref, const void* amp_alpha,

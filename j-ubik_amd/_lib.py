@@ -47,6 +47,8 @@ SIGNATURES = {
     "jubik_model_destroy": [_vp],
     "jubik_model_set_data": [_vp, _vp, _vp],
     "jubik_model_grad": [_vp, _vp, _vp, _vp, _vp, _dbl, _dbl, _vp, _vp, _vp, _vp, _vp],
+    "jubik_model_grad_hyper": [_vp, _vp, _vp, _vp, _vp, _dbl, _dbl, _vp, _vp, _vp, _vp, _vp, _vp,
+                               _vp, _vp],
     "jubik_model_metric": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
     "jubik_model_create_batched": [C.POINTER(_vp), _vp, _vp, _dbl, _i64],
     "jubik_model_grad_batch": [_vp, _i64, _vp, _vp, _vp, _vp, _dbl, _dbl, _vp, _vp, _vp, _vp,

@@ -770,7 +770,7 @@ namespace {
 // slot0 .. slot0 + S - 1, nll[s] is the NLL of position s.
 int model_grad(jubik_model* model, int S, int slot0, const void* amp_ref, const void* amp_alpha,
                const void* xi_ref, const void* xi_alpha, double zm, double mu, const void* dnu,
-               double* nll, void* g_ref, void* g_alpha, hipStream_t s) {
+               double* nll, void* g_ref, void* g_alpha, hipStream_t s, GpHyper hy = GpHyper{}) {
   jubik_linpatch* lp = model->lp;
   jubik_gp* gp = model->gp;
   GpGeom gg = gp->g;
@@ -799,7 +799,7 @@ int model_grad(jubik_model* model, int S, int slot0, const void* amp_ref, const 
   if (st) return st;
   gg.uplanes = g.E;
   return gp_adjoint_u(gp->dtype, gg, model->gbufs(), amp_ref, amp_alpha, model->u, g_ref, g_alpha,
-                      s);
+                      s, hy);
 }
 
 int check_model(jubik_model* model) {
@@ -816,6 +816,31 @@ int check_alpha(const jubik_model* model, const void* amp_alpha, const void* a, 
 }
 
 }  // namespace
+
+int jubik_model_grad_hyper(jubik_model* model, const void* amp_ref, const void* amp_alpha,
+                           const void* xi_ref, const void* xi_alpha, double zm, double mu,
+                           const void* dnu, double* nll, void* g_ref, void* g_alpha,
+                           void* g_amp_ref, void* g_amp_alpha, void* g_offsets, void* stream) {
+  if (!model || !nll || !g_ref || !amp_ref || !xi_ref || !g_amp_ref || !g_offsets)
+    return fail(kInvalidArgument, "NULL argument");
+  int st = check_model(model);
+  if (!st) st = check_alpha(model, amp_alpha, xi_alpha, g_alpha, dnu);
+  if (!st && model->gp->g.has_alpha && !g_amp_alpha)
+    st = fail(kInvalidArgument, "spectral-index inputs are NULL");
+  if (st) return st;
+  DeviceGuard dg(model->lp->device);
+  model->have_point = false;
+  GpHyper hy;
+  hy.xi_ref = xi_ref;
+  hy.xi_alpha = xi_alpha;
+  hy.g_amp_ref = g_amp_ref;
+  hy.g_amp_alpha = g_amp_alpha;
+  hy.g_off = g_offsets;
+  st = model_grad(model, 1, 0, amp_ref, amp_alpha, xi_ref, xi_alpha, zm, mu, dnu, nll, g_ref,
+                  g_alpha, as_stream(stream), hy);
+  if (st == kOk) model->have_point = true;
+  return st;
+}
 
 int jubik_model_grad(jubik_model* model, const void* amp_ref, const void* amp_alpha,
                      const void* xi_ref, const void* xi_alpha, double zm, double mu,

@@ -125,7 +125,10 @@ __global__ __launch_bounds__(GNT) void gp_cols(GpGeom g, const T* __restrict__ a
 
 // grid (N / 2 + 1, S).  MODE 0: sky = exp(log s) cropped to (E, H, W);
 // MODE 1: g_ref = amp_ref HT(u_ref), g_alpha = amp_alpha HT(u_alpha) on N x N;
-// MODE 2: tangent ds_e = sky_e (HT(a) + HT(b) dnu_e) on (E, H, W).
+// MODE 2: tangent ds_e = sky_e (HT(a) + HT(b) dnu_e) on (E, H, W);
+// MODE 3: MODE 1 plus the amplitude-field and offset gradients (GpHyper):
+// g_amp_ref = xi_ref HT(u_ref), g_amp_alpha = xi_alpha HT(u_alpha), and
+// (d zm, d mu) = (HT(u_ref), HT(u_alpha)) at k = 0, i.e. the sums of u.
 // Sample blockIdx.y reads work[s] and writes output s; the tangent's sky is
 // the same base-point sky for every sample when g.sky_shared.
 template <typename T, int N, int MODE>
@@ -135,7 +138,9 @@ __global__ __launch_bounds__(GNT) void gp_rows(GpGeom g, const cplx<T>* __restri
                                                const T* __restrict__ amp_ref,
                                                const T* __restrict__ amp_alpha,
                                                const T* __restrict__ sky_in,
-                                               T* __restrict__ out_a, T* __restrict__ out_b) {
+                                               T* __restrict__ out_a, T* __restrict__ out_b,
+                                               GpHyper hy) {
+  constexpr bool ADJ = MODE == 1 || MODE == 3;
   constexpr int VS = lds_stride(N, 2);
   __shared__ cplx<T> buf[2 * VS];
   __shared__ cplx<T> tw[N];
@@ -144,9 +149,23 @@ __global__ __launch_bounds__(GNT) void gp_rows(GpGeom g, const cplx<T>* __restri
   const int r1 = (N - r0) % N;
   const int64_t smp = blockIdx.y;
   Tw += smp * N * N;
-  if constexpr (MODE == 1) {
+  const T* xi_a = (const T*)hy.xi_ref;
+  const T* xi_b = (const T*)hy.xi_alpha;
+  T* ga_a = (T*)hy.g_amp_ref;
+  T* ga_b = (T*)hy.g_amp_alpha;
+  T* goff = (T*)hy.g_off;
+  if constexpr (ADJ) {
     out_a += smp * N * N;
     if (g.has_alpha) out_b += smp * N * N;
+    if constexpr (MODE == 3) {
+      xi_a += smp * N * N;
+      ga_a += smp * N * N;
+      goff += smp * 2;
+      if (g.has_alpha) {
+        xi_b += smp * N * N;
+        ga_b += smp * N * N;
+      }
+    }
   } else {
     out_a += smp * g.E * g.H * g.W;
     if (MODE == 2 && !g.sky_shared) sky_in += smp * g.E * g.H * g.W;
@@ -170,10 +189,13 @@ __global__ __launch_bounds__(GNT) void gp_rows(GpGeom g, const cplx<T>* __restri
   // MODE 1: the amplitudes of the epilogue are fetched with the rows (zero-
   // reading gathers; alpha reads zero when absent), so their latency
   // overlaps the transform.
-  T ampa[MODE == 1 ? EPT : 1], ampb[MODE == 1 ? EPT : 1];
-  if constexpr (MODE == 1) {
+  T ampa[ADJ ? EPT : 1], ampb[ADJ ? EPT : 1];
+  T xia[MODE == 3 ? EPT : 1], xib[MODE == 3 ? EPT : 1];
+  if constexpr (ADJ) {
     const Gather ga = make_gather(amp_ref, (int64_t)N * N);
     const Gather gb = make_gather(amp_alpha, g.has_alpha ? (int64_t)N * N : 0);
+    const Gather gxa = make_gather(xi_a, MODE == 3 ? (int64_t)N * N : 0);
+    const Gather gxb = make_gather(xi_b, MODE == 3 && g.has_alpha ? (int64_t)N * N : 0);
 #pragma unroll
     for (int u2 = 0; u2 < EPT; ++u2) {
       const int e = tid + u2 * GNT;
@@ -182,6 +204,10 @@ __global__ __launch_bounds__(GNT) void gp_rows(GpGeom g, const cplx<T>* __restri
       const int o = (v ? r1 : r0) * N + k;
       ampa[u2] = gather_load<T>(ga, gather_off<T>(ga, ok, o));
       ampb[u2] = gather_load<T>(gb, gather_off<T>(gb, ok && g.has_alpha, o));
+      if constexpr (MODE == 3) {
+        xia[u2] = gather_load<T>(gxa, gather_off<T>(gxa, ok, o));
+        xib[u2] = gather_load<T>(gxb, gather_off<T>(gxb, ok && g.has_alpha, o));
+      }
     }
   }
 #pragma unroll
@@ -210,10 +236,18 @@ __global__ __launch_bounds__(GNT) void gp_rows(GpGeom g, const cplx<T>* __restri
     const cplx<T> zp = buf[(1 - v) * VS + (k == 0 ? 0 : N - k)];
     const T hta = T(0.5) * (z.re + zp.re + z.im - zp.im);
     const T htb = T(0.5) * (z.im + zp.im - z.re + zp.re);
-    if constexpr (MODE == 1) {
+    if constexpr (ADJ) {
       const int64_t o = (int64_t)x * N + k;
       out_a[o] = ampa[u2] * hta;
       if (g.has_alpha) out_b[o] = ampb[u2] * htb;
+      if constexpr (MODE == 3) {
+        ga_a[o] = xia[u2] * hta;
+        if (g.has_alpha) ga_b[o] = xib[u2] * htb;
+        if (x == 0 && k == 0) {
+          goff[0] = hta;
+          goff[1] = g.has_alpha ? htb : T(0);
+        }
+      }
     } else {
       if (x >= g.H || k >= g.W) continue;
       for (int en = 0; en < g.E; ++en) {
@@ -282,7 +316,7 @@ struct GpOps {
       TimedLaunch _t("gp_rows_fwd", s);
       gp_rows<T, N, 0><<<dim3(N / 2 + 1, g.S), GNT, 0, s>>>(g, (const C*)b.work, (const C*)b.tw,
                                                  (const T*)dnu, nullptr, nullptr, nullptr,
-                                                 (T*)sky, nullptr);
+                                                 (T*)sky, nullptr, GpHyper{});
     }
     return check_hip(hipGetLastError(), "gp_forward");
   }
@@ -299,23 +333,28 @@ struct GpOps {
       TimedLaunch _t("gp_rows_tangent", s);
       gp_rows<T, N, 2><<<dim3(N / 2 + 1, g.S), GNT, 0, s>>>(g, (const C*)b.work, (const C*)b.tw,
                                                  (const T*)dnu, nullptr, nullptr, (const T*)sky,
-                                                 (T*)dsky, nullptr);
+                                                 (T*)dsky, nullptr, GpHyper{});
     }
     return check_hip(hipGetLastError(), "gp_tangent");
   }
   static int adjoint_u(const GpGeom& g, const GpBuffers& b, const void* amp_ref,
                        const void* amp_alpha, const void* u, void* g_ref, void* g_alpha,
-                       hipStream_t s) {
+                       hipStream_t s, GpHyper hy) {
     {
       TimedLaunch _t("gp_cols_adj", s);
       gp_cols<T, N, 1><<<dim3(N / GpTile<T, N, 1>::V, g.S), GNT, 0, s>>>(g, nullptr, nullptr, nullptr, nullptr, (const C*)u,
                                              (const C*)b.tw, (C*)b.work);
     }
-    {
+    if (hy.g_amp_ref) {
+      TimedLaunch _t("gp_rows_adj_hyper", s);
+      gp_rows<T, N, 3><<<dim3(N / 2 + 1, g.S), GNT, 0, s>>>(g, (const C*)b.work, (const C*)b.tw, nullptr,
+                                                 (const T*)amp_ref, (const T*)amp_alpha, nullptr,
+                                                 (T*)g_ref, (T*)g_alpha, hy);
+    } else {
       TimedLaunch _t("gp_rows_adj", s);
       gp_rows<T, N, 1><<<dim3(N / 2 + 1, g.S), GNT, 0, s>>>(g, (const C*)b.work, (const C*)b.tw, nullptr,
                                                  (const T*)amp_ref, (const T*)amp_alpha, nullptr,
-                                                 (T*)g_ref, (T*)g_alpha);
+                                                 (T*)g_ref, (T*)g_alpha, GpHyper{});
     }
     return check_hip(hipGetLastError(), "gp_adjoint");
   }
@@ -369,13 +408,13 @@ int gp_tangent(int dtype, const GpGeom& g, const GpBuffers& b, const void* amp_r
 }
 int gp_adjoint_u(int dtype, const GpGeom& g, const GpBuffers& b, const void* amp_ref,
                  const void* amp_alpha, const void* u, void* g_ref, void* g_alpha,
-                 hipStream_t s) {
+                 hipStream_t s, GpHyper hy) {
   // The column pass gathers a sample's planes with 32-bit byte offsets.
   const int64_t bytes = (int64_t)g.uplanes * g.H * g.W * (dtype == 0 ? 8 : 16);
   if (bytes > INT32_MAX)
     return fail(kUnsupported, "the cotangent planes of one sample exceed 2 GiB");
   return gp_dispatch(dtype, g.N, [&](auto ops) {
-    return decltype(ops)::adjoint_u(g, b, amp_ref, amp_alpha, u, g_ref, g_alpha, s);
+    return decltype(ops)::adjoint_u(g, b, amp_ref, amp_alpha, u, g_ref, g_alpha, s, hy);
   });
 }
 int gp_fold(int dtype, const GpGeom& g, const void* sky, const void* cot, const void* dnu,

@@ -16,6 +16,18 @@ struct GpGeom {
   int sky_shared; // tangent: 1 = one base-point sky for every sample
 };
 
+// Optional outputs of the adjoint: the gradients with respect to the
+// amplitude fields (g_amp_*, N x N per sample, needing the positions xi_*)
+// and to the offsets zm and mu (g_off, two values per sample).  Unused when
+// g_amp_ref is NULL.
+struct GpHyper {
+  const void* xi_ref = nullptr;
+  const void* xi_alpha = nullptr;
+  void* g_amp_ref = nullptr;
+  void* g_amp_alpha = nullptr;
+  void* g_off = nullptr;
+};
+
 struct GpBuffers {
   const void* tw;  // N twiddles
   void* work;      // S x N x N complex
@@ -29,7 +41,8 @@ int gp_tangent(int dtype, const GpGeom& g, const GpBuffers& b, const void* amp_r
                const void* amp_alpha, const void* v_ref, const void* v_alpha, const void* dnu,
                const void* sky, void* dsky, hipStream_t s);
 int gp_adjoint_u(int dtype, const GpGeom& g, const GpBuffers& b, const void* amp_ref,
-                 const void* amp_alpha, const void* u, void* g_ref, void* g_alpha, hipStream_t s);
+                 const void* amp_alpha, const void* u, void* g_ref, void* g_alpha, hipStream_t s,
+                 GpHyper hy = GpHyper{});
 int gp_fold(int dtype, const GpGeom& g, const void* sky, const void* cot, const void* dnu,
             void* u, hipStream_t s);
 

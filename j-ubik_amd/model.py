@@ -100,6 +100,29 @@ class XrayModel:
         return val, dev.like_input(g_ref, xi_ref), (
             dev.like_input(g_alpha, xi_ref) if g_alpha is not None else None)
 
+    def nll_and_grad_hyper(self, xi_ref, xi_alpha=None):
+        """One gradient evaluation that also returns the gradients with
+        respect to the sky's amplitude fields and offsets:
+        (NLL, grad_xi_ref, grad_xi_alpha, grad_amp_ref, grad_amp_alpha,
+        grad_zm, grad_mu); the alpha entries are None for a single-energy
+        sky.  These are what the reference's amplitude models and zero-mode
+        priors chain their hyperparameter gradients through."""
+        s = self.sky
+        xr = dev.to_device(xi_ref, self.dtype)
+        xa = dev.to_device(xi_alpha, self.dtype) if s.has_alpha else None
+        g_ref, g_alpha = self._grads()
+        ga_ref, ga_alpha = self._grads()
+        off = torch.empty(2, dtype=self.dtype, device="cuda")
+        _lib.call("jubik_model_grad_hyper", self._model, dev.ptr(s.amp_ref),
+                  dev.ptr(s.amp_alpha), dev.ptr(xr), dev.ptr(xa), s.zm, s.mu,
+                  dev.ptr(s.dnu_dev), dev.ptr(self._nll), dev.ptr(g_ref), dev.ptr(g_alpha),
+                  dev.ptr(ga_ref), dev.ptr(ga_alpha), dev.ptr(off), dev.stream())
+        val = float(self._nll.item())
+        offs = off.double().cpu().numpy()
+        like = lambda t: None if t is None else dev.like_input(t, xi_ref)
+        return (val, like(g_ref), like(g_alpha), like(ga_ref), like(ga_alpha), float(offs[0]),
+                float(offs[1]) if s.has_alpha else None)
+
     def metric(self, v_ref, v_alpha=None):
         """J^T diag(1/lam) J v at the position of the last gradient call."""
         s = self.sky

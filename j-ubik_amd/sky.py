@@ -67,6 +67,24 @@ class CorrelatedFieldSky:
                 pass
             self._plan = None
 
+    def set_parameters(self, amp_ref=None, amp_alpha=None, zm=None, mu=None):
+        """Replace the amplitude fields and/or offsets (the outputs of the
+        reference's amplitude models and zero-mode priors, which change as
+        their hyperparameters are inferred); shapes and dtype are fixed."""
+        for name, a in (("amp_ref", amp_ref), ("amp_alpha", amp_alpha)):
+            if a is None:
+                continue
+            if name == "amp_alpha" and not self.has_alpha:
+                raise ValueError("this sky has no spectral-index field")
+            t = dev.to_device(a, self.dtype)
+            if tuple(t.shape) != (self.N, self.N):
+                raise ValueError(f"amplitude fields must be ({self.N}, {self.N})")
+            setattr(self, name, t.contiguous())
+        if zm is not None:
+            self.zm = float(zm)
+        if mu is not None:
+            self.mu = float(mu)
+
     @property
     def shape(self):
         return (self.E, self.H, self.W)

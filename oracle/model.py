@@ -71,22 +71,24 @@ def nll(prob, xi_ref, xi_alpha=None):
     return total
 
 
-def _backward(prob, s, cot):
-    """J^T applied to the dense per-module cotangents ``cot`` of lam."""
+def _log_sky_cotangent(prob, s, cot):
+    """Cotangent of log s for the dense per-module cotangents ``cot`` of lam."""
     xbar = np.zeros(s.shape)
     for mod, c in zip(prob.modules, cot):
         xbar += cv.linpatch_adjoint_window(mod.exposure * c, prob.domain,
                                            mod.window, prob.n, prob.margin)
-    u_log = s * (prob.pixel_area * xbar)
+    return s * (prob.pixel_area * xbar)
+
+
+def _backward(prob, s, cot):
+    """J^T applied to the dense per-module cotangents ``cot`` of lam."""
     g_ref, g_alpha, _, _ = gp.gp_adjoint(
-        u_log, prob.amp_ref, prob.amp_alpha, prob.dnu,
+        _log_sky_cotangent(prob, s, cot), prob.amp_ref, prob.amp_alpha, prob.dnu,
         grid_shape=prob.amp_ref.shape)
     return g_ref, g_alpha
 
 
-def nll_and_grad(prob, xi_ref, xi_alpha=None):
-    """NLL and its gradient with respect to (xi_ref, xi_alpha)."""
-    s = sky(prob, xi_ref, xi_alpha)
+def _nll_and_cotangents(prob, s):
     total = 0.0
     cot = []
     for mod, lam in zip(prob.modules, response(prob, s)):
@@ -95,7 +97,45 @@ def nll_and_grad(prob, xi_ref, xi_alpha=None):
         r = np.zeros(lam.shape)
         r[mod.keep] = 1.0 - mod.data / v
         cot.append(r)
+    return total, cot
+
+
+def nll_and_grad(prob, xi_ref, xi_alpha=None):
+    """NLL and its gradient with respect to (xi_ref, xi_alpha)."""
+    s = sky(prob, xi_ref, xi_alpha)
+    total, cot = _nll_and_cotangents(prob, s)
     return total, _backward(prob, s, cot)
+
+
+def nll_and_grad_hyper(prob, xi_ref, xi_alpha=None):
+    """NLL, its gradient with respect to (xi_ref, xi_alpha), and with respect
+    to the sky model's amplitude fields and offsets.
+
+    With s_e = exp(zm + HT(A_ref xi_ref) + (HT(A_alpha xi_alpha) + mu) dnu_e)
+    (spectral_product_mf_sky.py:171-298 with the amplitude fields given) and
+    u_ref = sum_e u_e, u_alpha = sum_e dnu_e u_e for the cotangent u of log s:
+    d/dA_ref = xi_ref * HT(u_ref), d/dA_alpha = xi_alpha * HT(u_alpha),
+    d/dzm = sum u_ref, d/dmu = sum u_alpha (HT is symmetric).  Returns
+    (nll, g_ref, g_alpha, g_amp_ref, g_amp_alpha, g_zm, g_mu); the alpha
+    entries are None for a single-energy sky.
+    """
+    s = sky(prob, xi_ref, xi_alpha)
+    total, cot = _nll_and_cotangents(prob, s)
+    u_log = _log_sky_cotangent(prob, s, cot)
+    grid = prob.amp_ref.shape
+    full = np.zeros(u_log.shape[:-2] + tuple(grid))
+    full[..., :u_log.shape[-2], :u_log.shape[-1]] = u_log
+    u_ref = full.sum(axis=0)
+    h_ref = gp.hartley(u_ref)
+    out = [total, prob.amp_ref * h_ref, None, np.asarray(xi_ref) * h_ref, None,
+           float(u_ref.sum()), None]
+    if prob.amp_alpha is not None:
+        u_alpha = np.tensordot(np.asarray(prob.dnu, dtype=np.float64), full, axes=(0, 0))
+        h_alpha = gp.hartley(u_alpha)
+        out[2] = prob.amp_alpha * h_alpha
+        out[4] = np.asarray(xi_alpha) * h_alpha
+        out[6] = float(u_alpha.sum())
+    return tuple(out)
 
 
 def jvp(prob, xi_ref, xi_alpha, v_ref, v_alpha):

@@ -209,6 +209,36 @@ def test_model_gradient_finite_differences(cfg):
     assert fd == pytest.approx(an, rel=1e-6)
 
 
+@pytest.mark.parametrize("cfg", [dict(H=32, n=4, margin=4, E=1, M=1, seed=5, N=40),
+                                 dict(H=32, n=4, margin=4, E=3, M=2, seed=6)])
+def test_hyper_gradients_finite_differences(cfg):
+    """Amplitude-field and offset gradients against central differences of
+    the NLL in the amplitude fields, zm and mu_alpha."""
+    from dataclasses import replace
+    prob, xi = make_problem(**cfg)
+    nll, gr, ga, gar, gaa, gzm, gmu = om.nll_and_grad_hyper(prob, *xi)
+    want_nll, (wr, wa) = om.nll_and_grad(prob, *xi)
+    assert nll == want_nll
+    np.testing.assert_array_equal(gr, wr)
+    h = 1e-6
+    f = lambda **kw: om.nll(replace(prob, **kw), *xi)
+    rng = np.random.default_rng(2)
+    v = rng.normal(size=prob.amp_ref.shape)
+    fd = (f(amp_ref=prob.amp_ref + h * v) - f(amp_ref=prob.amp_ref - h * v)) / (2 * h)
+    assert fd == pytest.approx(np.vdot(gar, v), rel=1e-6)
+    fd = (f(zm=prob.zm + h) - f(zm=prob.zm - h)) / (2 * h)
+    assert fd == pytest.approx(gzm, rel=1e-6)
+    if prob.amp_alpha is None:
+        assert ga is None and gaa is None and gmu is None
+        return
+    np.testing.assert_array_equal(ga, wa)
+    v = rng.normal(size=prob.amp_alpha.shape)
+    fd = (f(amp_alpha=prob.amp_alpha + h * v) - f(amp_alpha=prob.amp_alpha - h * v)) / (2 * h)
+    assert fd == pytest.approx(np.vdot(gaa, v), rel=1e-6)
+    fd = (f(mu_alpha=prob.mu_alpha + h) - f(mu_alpha=prob.mu_alpha - h)) / (2 * h)
+    assert fd == pytest.approx(gmu, rel=1e-6)
+
+
 def test_metric_is_gauss_newton_of_the_response():
     """v^T M v = sum (J v)^2 / lam with J v from central differences."""
     prob, xi = make_problem(H=32, n=4, margin=4, E=2, M=1, seed=4)
