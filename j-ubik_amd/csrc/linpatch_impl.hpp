@@ -490,7 +490,7 @@ __global__ __launch_bounds__(NT) void lp_cols(LinGeom g, const cplx<T>* __restri
 // columns iy = 2q, 2q+1 share one complex vector as real and imaginary
 // part).  Output columns are owned by threads (j = tid + t * NT) and summed
 // in registers in a fixed order, so results are deterministic.
-// Output columns per thread (JPT): W <= JPT * NT; instantiated for 4 and 16.
+// Output columns per thread (JPT): W <= JPT * NT; instantiated for 4, 8 and 16.
 
 template <typename T, int L>
 struct OutTile {
@@ -971,11 +971,13 @@ struct LinOps {
     }
   }
 
-  // Output columns per thread: 4 covers W <= 1024, 16 covers W <= 4096.
+  // Output columns per thread: 4 covers W <= 1024, 8 W <= 2048, 16 W <= 4096.
   template <class F>
   static void with_jpt(const LinGeom& g, F&& f) {
     if (g.W <= 4 * NT)
       f(std::integral_constant<int, 4>{});
+    else if (g.W <= 8 * NT)
+      f(std::integral_constant<int, 8>{});
     else
       f(std::integral_constant<int, 16>{});
   }

@@ -27,6 +27,7 @@ GEOMS = [
     (128, 4, 48),   # L = 160? -> not compiled: skipped below
     (512, 8, 64),   # L = 256
     (256, 2, 64),   # L = 384
+    (1280, 10, 64), # L = 384, W > 1024: 8 output columns per thread
     (384, 2, 0),    # placeholder, margin 0 handled separately
 ]
 
@@ -63,7 +64,7 @@ def test_forward_matches_oracle(ja, H, n, m, dtype):
     assert err <= tol
 
 
-@pytest.mark.parametrize("H,n,m", _cases()[:5] + [(512, 8, 64), (256, 2, 64)])
+@pytest.mark.parametrize("H,n,m", _cases()[:5] + [(512, 8, 64), (256, 2, 64), (1280, 10, 64)])
 @pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
 def test_adjoint_matches_oracle(ja, H, n, m, dtype):
     E = 2

@@ -56,6 +56,22 @@ def test_grad_and_metric_vs_oracle(ja, cfg, dtype, tol):
     assert nb[1] == n1 and np.array_equal(gb[1], g1)
 
 
+def test_wide_image_fp32_vs_oracle(ja):
+    """W = 1280 (8 output columns per thread in the overlap kernels) on a
+    2048 GP grid, FP32 (the FP64 GP is compiled up to 1024): NLL, gradient
+    and a metric product against the oracle."""
+    prob, xi = make_problem(H=1280, n=10, margin=64, E=1, M=1, seed=66, N=2048)
+    want_nll, (want_r, _) = om.nll_and_grad(prob, *xi)
+    model = to_product(ja, prob, dtype=torch.float32)
+    nll, g_r, _ = model.nll_and_grad(torch.as_tensor(xi[0], device="cuda"))
+    assert abs(nll - want_nll) <= 1e-5 * abs(want_nll)
+    assert _rel(g_r.double().cpu().numpy(), want_r) <= 1e-5
+    v = np.random.default_rng(5).normal(size=xi[0].shape)
+    got, _ = model.metric(torch.as_tensor(v, device="cuda"))
+    want, _ = om.metric_product(prob, xi[0], None, v, None)
+    assert _rel(got.double().cpu().numpy(), want) <= 1e-5
+
+
 @pytest.mark.parametrize("dtype,tol", [(torch.float64, 1e-10), (torch.float32, 1e-5)])
 def test_kernel_shared_by_energies(ja, dtype, tol):
     """kernel_per_energy=False (convolve.py:167-171 broadcast): the oracle
