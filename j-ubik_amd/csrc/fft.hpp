@@ -306,7 +306,14 @@ __host__ __device__ constexpr int plan_radix(int L, int s, int pl = 0) {
   if (pl == 1) {
     int w0 = 0, w1 = 0, w2 = 0;
     switch (L) {
+      case 48: w0 = 4; w1 = 4; w2 = 3; break;
+      case 64: w0 = 4; w1 = 4; w2 = 4; break;
+      case 96: w0 = 4; w1 = 4; w2 = 6; break;
+      case 128: w0 = 4; w1 = 4; w2 = 8; break;
+      case 192: w0 = 8; w1 = 4; w2 = 6; break;
+      case 224: w0 = 8; w1 = 4; w2 = 7; break;
       case 256: w0 = 4; w1 = 8; w2 = 8; break;
+      case 324: w0 = 6; w1 = 6; w2 = 9; break;
       case 384: w0 = 8; w1 = 8; w2 = 6; break;
       case 512: w0 = 8; w1 = 8; w2 = 8; break;
       case 1024: w0 = 8; w1 = 8; w2 = 16; break;
