@@ -482,22 +482,25 @@ __global__ __launch_bounds__(NT) void lp_cols(LinGeom g, const cplx<T>* __restri
 }
 
 // ------------------------------------------------------------------ overlap
+// Threads per work-group of the overlap kernels.
+constexpr int NTO = 256;
 // The overlap kernels own one output image row per work-group.  Patch rows
 // of one patch column (same iy, hence the same column offset) enter the
 // row with plain (forward) or tent-weighted (adjoint) sums, which commute
 // with the inverse row FFT: their half-spectrum rows are summed first, so
 // each work-group runs one inverse FFT per pair of patch columns (patch
 // columns iy = 2q, 2q+1 share one complex vector as real and imaginary
-// part).  Output columns are owned by threads (j = tid + t * NT) and summed
+// part).  Output columns are owned by threads (j = tid + t * NTO) and summed
 // in registers in a fixed order, so results are deterministic.
-// Output columns per thread (JPT): W <= JPT * NT; instantiated for 4, 8 and 16.
+// Output columns per thread (JPT): W <= JPT * NTO; the host picks 1024, 2048 or
+// 4096 columns over NTO threads.
 
 template <typename T, int L>
 struct OutTile {
   static constexpr int V = 4;  // iy pairs per pass: n <= 8 in one pass
   static constexpr int LH = L / 2;
-  static constexpr int GPT = (V * LH + NT - 1) / NT;  // gather tasks (v, kk) per thread
-  static constexpr int TPT = (L + NT - 1) / NT;       // twiddles per thread
+  static constexpr int GPT = (V * LH + NTO - 1) / NTO;  // gather tasks (v, kk) per thread
+  static constexpr int TPT = (L + NTO - 1) / NTO;       // twiddles per thread
 };
 
 // First patch column whose window [s, s + len) can contain canvas column C:
@@ -556,10 +559,10 @@ __device__ __forceinline__ void sum_pair_spectra(cplx<T>* buf, const cplx<T>* __
   int oa[GPT], ob[GPT];
 #pragma unroll
   for (int u = 0; u < GPT; ++u) {
-    const int e = tid + u * NT;
+    const int e = tid + u * NTO;
     const int v = e / LH, kk = e - (e / LH) * LH;
     const int q = q0 + v;
-    const bool ok = (V * LH % NT == 0 || e < V * LH) && q < npair;
+    const bool ok = (V * LH % NTO == 0 || e < V * LH) && q < npair;
     oa[u] = ok ? 2 * q * g.n * pstride + kk : -1;
     ob[u] = ok && 2 * q + 1 < g.n ? (2 * q + 1) * g.n * pstride + kk : -1;
   }
@@ -601,8 +604,8 @@ __device__ __forceinline__ void sum_pair_spectra(cplx<T>* buf, const cplx<T>* __
   }
 #pragma unroll
   for (int u = 0; u < GPT; ++u) {
-    const int e = tid + u * NT;
-    if (V * LH % NT != 0 && e >= V * LH) break;
+    const int e = tid + u * NTO;
+    if (V * LH % NTO != 0 && e >= V * LH) break;
     const int v = e / LH, kk = e - (e / LH) * LH;
     cplx<T>* dst = buf + v * VS;
     if (kk == 0) {
@@ -618,7 +621,7 @@ __device__ __forceinline__ void sum_pair_spectra(cplx<T>* buf, const cplx<T>* __
 // grid (H, S * J).  Exposure, counts and (metric) the base-point lam are
 // per (mod, e) and shared by the samples; y, resid and partial are per job.
 template <typename T, int L, int EPI, int JPT>
-__global__ __launch_bounds__(NT) void lp_fwd_out(LinGeom g, const cplx<T>* __restrict__ R2,
+__global__ __launch_bounds__(NTO) void lp_fwd_out(LinGeom g, const cplx<T>* __restrict__ R2,
                                                  const cplx<T>* __restrict__ twg,
                                                  T* __restrict__ y, const T* __restrict__ expo,
                                                  const int* __restrict__ counts,
@@ -630,7 +633,7 @@ __global__ __launch_bounds__(NT) void lp_fwd_out(LinGeom g, const cplx<T>* __res
   constexpr int L2P = half_stride(L);
   __shared__ cplx<T> buf[V * VS];
   __shared__ cplx<T> tw[L];
-  __shared__ double red[NT / 64];
+  __shared__ double red[NTO / 64];
   const int tid = threadIdx.x;
   const int i = blockIdx.x;
   const int job = blockIdx.y;
@@ -643,7 +646,7 @@ __global__ __launch_bounds__(NT) void lp_fwd_out(LinGeom g, const cplx<T>* __res
   cplx<T> twr[TPT];
 #pragma unroll
   for (int u = 0; u < TPT; ++u) {
-    const int e = tid + u * NT;
+    const int e = tid + u * NTO;
     if (e < L) twr[u] = twg[e];
   }
   T acc[JPT];
@@ -666,7 +669,7 @@ __global__ __launch_bounds__(NT) void lp_fwd_out(LinGeom g, const cplx<T>* __res
     if constexpr (decltype(first)::value) {
 #pragma unroll
       for (int u = 0; u < TPT; ++u) {
-        const int e = tid + u * NT;
+        const int e = tid + u * NTO;
         if (e < L) tw[e] = twr[u];
       }
       // Epilogue operands are fetched once the spectrum loads are issued
@@ -675,7 +678,7 @@ __global__ __launch_bounds__(NT) void lp_fwd_out(LinGeom g, const cplx<T>* __res
       if constexpr (EPI != 0) {
 #pragma unroll
         for (int t = 0; t < JPT; ++t) {
-          const int j = tid + t * NT;
+          const int j = tid + t * NTO;
           ex_v[t] = j < g.W ? expo[rowd + j] : T(0);
           cnt_v[t] = j < g.W ? counts[rowd + j] : -1;
           if (EPI == 2) lam_v[t] = j < g.W ? y[rowd + j] : T(1);
@@ -683,10 +686,10 @@ __global__ __launch_bounds__(NT) void lp_fwd_out(LinGeom g, const cplx<T>* __res
       }
     }
     __syncthreads();
-    fft_lds<T, L, V, VS, NT, true, kFewVectorPlan<L, V, NT>>(buf, tw, tid);
+    fft_lds<T, L, V, VS, NTO, true, kFewVectorPlan<L, V, NTO>>(buf, tw, tid);
 #pragma unroll
     for (int t = 0; t < JPT; ++t) {
-      const int j = tid + t * NT;
+      const int j = tid + t * NTO;
       if (j < g.W) {
         const int C = j + g.h + g.m;
         T s = T(0);
@@ -711,14 +714,14 @@ __global__ __launch_bounds__(NT) void lp_fwd_out(LinGeom g, const cplx<T>* __res
   if constexpr (EPI == 0) {
 #pragma unroll
     for (int t = 0; t < JPT; ++t) {
-      const int j = tid + t * NT;
+      const int j = tid + t * NTO;
       if (j < g.W) y[row + j] = acc[t];
     }
   } else if constexpr (EPI == 2) {
     // Metric: cotangent expo * (d lam / lam) on read-out pixels, y = lam.
 #pragma unroll
     for (int t = 0; t < JPT; ++t) {
-      const int j = tid + t * NT;
+      const int j = tid + t * NTO;
       if (j < g.W) {
         const T ex = ex_v[t];
         resid[row + j] = cnt_v[t] >= 0 ? ex * (ex * acc[t]) / lam_v[t] : T(0);
@@ -728,7 +731,7 @@ __global__ __launch_bounds__(NT) void lp_fwd_out(LinGeom g, const cplx<T>* __res
     double nll = 0.0;
 #pragma unroll
     for (int t = 0; t < JPT; ++t) {
-      const int j = tid + t * NT;
+      const int j = tid + t * NTO;
       if (j < g.W) {
         const T ex = ex_v[t];
         const T lam = ex * acc[t];
@@ -746,7 +749,7 @@ __global__ __launch_bounds__(NT) void lp_fwd_out(LinGeom g, const cplx<T>* __res
         if (y) y[row + j] = lam;
       }
     }
-    nll = block_sum<NT>(nll, red);
+    nll = block_sum<NTO>(nll, red);
     if (tid == 0) partial[(int64_t)job * g.H + i] = nll;
   }
 }
@@ -757,7 +760,7 @@ __global__ __launch_bounds__(NT) void lp_fwd_out(LinGeom g, const cplx<T>* __res
 //        energy (the GP adjoint sums a sample's planes while loading them);
 //        sky[s, e] is sky[e] when g.sky_shared.
 template <typename T, int L, int EPI, int JPT>
-__global__ __launch_bounds__(NT) void lp_adj_out(LinGeom g, const cplx<T>* __restrict__ R2,
+__global__ __launch_bounds__(NTO) void lp_adj_out(LinGeom g, const cplx<T>* __restrict__ R2,
                                                  const cplx<T>* __restrict__ twg,
                                                  const T* __restrict__ tent,
                                                  T* __restrict__ gx, T scale,
@@ -780,13 +783,13 @@ __global__ __launch_bounds__(NT) void lp_adj_out(LinGeom g, const cplx<T>* __res
   cplx<T> twr[TPT];
 #pragma unroll
   for (int uu = 0; uu < TPT; ++uu) {
-    const int e = tid + uu * NT;
+    const int e = tid + uu * NTO;
     if (e < L) twr[uu] = twg[e];
   }
   T tnr[TPT];
 #pragma unroll
   for (int uu = 0; uu < TPT; ++uu) {
-    const int e = tid + uu * NT;
+    const int e = tid + uu * NTO;
     if (e < g.w) tnr[uu] = tent[e];
   }
   T acc[JPT], acc2[JPT], tmp[JPT];
@@ -816,7 +819,7 @@ __global__ __launch_bounds__(NT) void lp_adj_out(LinGeom g, const cplx<T>* __res
     if constexpr (decltype(first)::value) {
 #pragma unroll
       for (int uu = 0; uu < TPT; ++uu) {
-        const int e = tid + uu * NT;
+        const int e = tid + uu * NTO;
         if (e < L) tw[e] = twr[uu];
         if (e < g.w) tent_s[e] = tnr[uu];
       }
@@ -825,16 +828,16 @@ __global__ __launch_bounds__(NT) void lp_adj_out(LinGeom g, const cplx<T>* __res
         const T* srow = sky + ((int64_t)simg * g.H + i) * g.W;
 #pragma unroll
         for (int t = 0; t < JPT; ++t) {
-          const int j = tid + t * NT;
+          const int j = tid + t * NTO;
           sky_v[t] = j < g.W ? srow[j] : T(0);
         }
       }
     }
     __syncthreads();
-    fft_lds<T, L, V, VS, NT, true, kFewVectorPlan<L, V, NT>>(buf, tw, tid);
+    fft_lds<T, L, V, VS, NTO, true, kFewVectorPlan<L, V, NTO>>(buf, tw, tid);
 #pragma unroll
     for (int t = 0; t < JPT; ++t) {
-      const int j = tid + t * NT;
+      const int j = tid + t * NTO;
       if (j < g.W) {
         const int Cp = j + g.h;
         T s = T(0);
@@ -871,7 +874,7 @@ __global__ __launch_bounds__(NT) void lp_adj_out(LinGeom g, const cplx<T>* __res
       const T dn = dnu ? dnu[e] : T(0);
 #pragma unroll
       for (int t = 0; t < JPT; ++t) {
-        const int j = tid + t * NT;
+        const int j = tid + t * NTO;
         if (j < g.W) {
           const T val = scale * sky_v[t] * tmp[t];
           acc[t] += val;
@@ -884,14 +887,14 @@ __global__ __launch_bounds__(NT) void lp_adj_out(LinGeom g, const cplx<T>* __res
     const int64_t row = ((int64_t)blockIdx.y * g.H + i) * g.W;
 #pragma unroll
     for (int t = 0; t < JPT; ++t) {
-      const int j = tid + t * NT;
+      const int j = tid + t * NTO;
       if (j < g.W) gx[row + j] = scale * acc[t];
     }
   } else {
     const int64_t row = ((int64_t)blockIdx.y * g.H + i) * g.W;
 #pragma unroll
     for (int t = 0; t < JPT; ++t) {
-      const int j = tid + t * NT;
+      const int j = tid + t * NTO;
       if (j < g.W) u[row + j] = cplx<T>{acc[t], acc2[t]};
     }
   }
@@ -971,15 +974,15 @@ struct LinOps {
     }
   }
 
-  // Output columns per thread: 4 covers W <= 1024, 8 W <= 2048, 16 W <= 4096.
+  // Output columns per thread: W <= 1024, 2048 or 4096 over NTO threads.
   template <class F>
   static void with_jpt(const LinGeom& g, F&& f) {
-    if (g.W <= 4 * NT)
-      f(std::integral_constant<int, 4>{});
-    else if (g.W <= 8 * NT)
-      f(std::integral_constant<int, 8>{});
+    if (g.W <= 1024)
+      f(std::integral_constant<int, 1024 / NTO>{});
+    else if (g.W <= 2048)
+      f(std::integral_constant<int, 2048 / NTO>{});
     else
-      f(std::integral_constant<int, 16>{});
+      f(std::integral_constant<int, 4096 / NTO>{});
   }
 
   template <int EPI>
@@ -988,7 +991,7 @@ struct LinOps {
                       const char* name) {
     TimedLaunch _t(name, s);
     with_jpt(g, [&](auto jpt) {
-      lp_fwd_out<T, L, EPI, decltype(jpt)::value><<<dim3(g.H, g.S * g.J), NT, 0, s>>>(
+      lp_fwd_out<T, L, EPI, decltype(jpt)::value><<<dim3(g.H, g.S * g.J), NTO, 0, s>>>(
           g, (const C*)b.R2, (const C*)b.tw, y, expo, counts, resid, partial);
     });
   }
@@ -998,7 +1001,7 @@ struct LinOps {
                       const T* sky, const T* dnu, C* u, hipStream_t s, const char* name) {
     TimedLaunch _t(name, s);
     with_jpt(g, [&](auto jpt) {
-      lp_adj_out<T, L, EPI, decltype(jpt)::value><<<dim3(g.H, nimg), NT, 0, s>>>(
+      lp_adj_out<T, L, EPI, decltype(jpt)::value><<<dim3(g.H, nimg), NTO, 0, s>>>(
           g, (const C*)b.R2, (const C*)b.tw, (const T*)b.tent, gx, T(scale), sky, dnu, u);
     });
   }
