@@ -363,7 +363,9 @@ __global__ __launch_bounds__(NT) void lp_cols(LinGeom g, const cplx<T>* __restri
       return k < LH ? ks[pos * L2P + k] : cplx<T>{T(0), T(0)};
     });
   }
-  // adjoint: window rows outside the image are zero and are never stored
+  // Image row of window row 0: the adjoint's input rows outside the image
+  // are zero and were never stored; the forward's output rows there are
+  // never read.
   const int row0 = start_x(g, p % g.n) - g.h - g.m;
   auto load_col = [&](int vec, int pos) -> cplx<T> {
     const int k = k0 + vec;
@@ -452,7 +454,10 @@ __global__ __launch_bounds__(NT) void lp_cols(LinGeom g, const cplx<T>* __restri
         const int r = pos - m;
         if (r >= 0 && r < w) st_stream(&o[r * L2P + k], val);
       } else {
-        st_stream(&o[pos * L2P + k], val);
+        // Output rows that fall outside the image are never read by the
+        // overlap pass (it gathers image rows only): not stored.
+        const int ir = row0 + pos;
+        if (ir >= 0 && ir < g.H) st_stream(&o[pos * L2P + k], val);
       }
     };
     if constexpr (NS == 1) {
