@@ -111,18 +111,34 @@ def time_steps(torch, fn, steps, warmup):
     return (time.perf_counter() - t0) / steps
 
 
+def window_rows_in_image(H, n, m):
+    """Rows of the L-row windows (all n x n patches) that fall inside the
+    image: the only ones the forward output and the adjoint input need."""
+    dx = H // n
+    h, w = dx // 2, 2 * dx
+    L, lim = w + 2 * m, H + 2 * (dx // 2) - w
+    tot = 0
+    for ix in range(n):
+        s = min(ix * dx, lim) - h - m
+        tot += max(0, min(s + L, H) - max(s, 0))
+    return tot * n
+
+
 def kernel_bytes(H, E, n, m, M=1):
     """Bytes each kernel must move per launch given the three-pass design
-    (inputs read once, outputs written once; FP32, complex = 8 B)."""
+    (inputs read once, outputs written once; FP32, complex = 8 B).  The
+    forward output and the adjoint input of the column passes are the window
+    rows inside the image (the others are never read, or zero)."""
     dx = H // n
     w, L = 2 * dx, 2 * dx + 2 * m
     L2 = L // 2  # stored half-spectrum columns (DC and Nyquist packed)
     J, P = M * E, n * n
     img = J * H * H * 4
     c = 8
+    rows_in = window_rows_in_image(H, n, m)
     r1_fwd = J * P * w * L2 * c
-    r2_fwd = J * P * L * L2 * c
-    r1_adj = J * P * L * L2 * c
+    r2_fwd = J * rows_in * L2 * c
+    r1_adj = J * rows_in * L2 * c
     r2_adj = J * P * w * L2 * c
     kt = M * P * E * L2 * L * c
     gp = H * H * c
