@@ -48,7 +48,7 @@ def parse():
     ap.add_argument("--samples", type=int, default=8,
                     help="VI samples per call of the batched gradient leg (0: skip)")
     ap.add_argument("--sweep", action="store_true",
-                    help="time every benchmark configuration (C1-C5, W*) instead of the W* line")
+                    help="time every benchmark configuration instead of the 1024^2 x 4E line")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="with --profile-only: first copy a buffer of this many MiB "
                          "(a known-size reference for the HBM byte counters)")
@@ -217,26 +217,26 @@ def cpu_baseline(inp, counts, budget):
                        f"median {med:.2f} s"), result
 
 
-# Configurations C1-C5 and W*: grid, energies, modules, patches, margin,
+# Benchmark configurations: grid, energies, modules, patches, margin,
 # PSF core sigma range (Gaussian, + Moffat beta), exposure range, mask
 # fraction, samples per batched call.
 CONFIGS = {
-    "C1": dict(H=128, E=1, M=1, n=4, m=16, sigma=(1.0, 3.0), moffat=None,
-               expo=(1e3, 1e3), mask=0.10, S=0, seed=1),
-    "C2": dict(H=1024, E=1, M=1, n=8, m=64, sigma=(1.0, 4.0), moffat=2.5,
-               expo=(5e2, 2e3), mask=0.05, S=0, seed=2),
-    "C3": dict(H=1024, E=3, M=7, n=10, m=60, sigma=(2.0, 6.0), moffat=None,
-               expo=(1e2, 3e3), mask=0.08, S=0, seed=3),
-    "C4a": dict(H=256, E=1, M=1, n=6, m=22, sigma=(1.0, 3.0), moffat=None,
-                expo=(5e2, 2e3), mask=0.05, S=32, seed=4),
-    "C4b": dict(H=512, E=1, M=1, n=8, m=32, sigma=(1.0, 3.0), moffat=None,
-                expo=(5e2, 2e3), mask=0.05, S=32, seed=4),
-    "C4c": dict(H=768, E=1, M=1, n=12, m=48, sigma=(1.0, 3.0), moffat=None,
-                expo=(5e2, 2e3), mask=0.05, S=32, seed=4),
-    "C5": dict(H=2048, E=4, M=1, n=16, m=64, sigma=(1.0, 4.0), moffat=2.5,
-               expo=(5e2, 2e3), mask=0.05, S=32, seed=5),
-    "W*": dict(H=1024, E=4, M=1, n=8, m=64, sigma=(1.0, 4.0), moffat=2.5,
-               expo=(5e2, 2e3), mask=0.05, S=8, seed=2),
+    "small_128x1E": dict(H=128, E=1, M=1, n=4, m=16, sigma=(1.0, 3.0), moffat=None,
+                         expo=(1e3, 1e3), mask=0.10, S=0, seed=1),
+    "adjoint_1024x1E": dict(H=1024, E=1, M=1, n=8, m=64, sigma=(1.0, 4.0), moffat=2.5,
+                            expo=(5e2, 2e3), mask=0.05, S=0, seed=2),
+    "modules_1024x3Ex7M": dict(H=1024, E=3, M=7, n=10, m=60, sigma=(2.0, 6.0), moffat=None,
+                               expo=(1e2, 3e3), mask=0.08, S=0, seed=3),
+    "grid_256x1E_S32": dict(H=256, E=1, M=1, n=6, m=22, sigma=(1.0, 3.0), moffat=None,
+                            expo=(5e2, 2e3), mask=0.05, S=32, seed=4),
+    "grid_512x1E_S32": dict(H=512, E=1, M=1, n=8, m=32, sigma=(1.0, 3.0), moffat=None,
+                            expo=(5e2, 2e3), mask=0.05, S=32, seed=4),
+    "grid_768x1E_S32": dict(H=768, E=1, M=1, n=12, m=48, sigma=(1.0, 3.0), moffat=None,
+                            expo=(5e2, 2e3), mask=0.05, S=32, seed=4),
+    "wide_2048x4E_S32": dict(H=2048, E=4, M=1, n=16, m=64, sigma=(1.0, 4.0), moffat=2.5,
+                             expo=(5e2, 2e3), mask=0.05, S=32, seed=5),
+    "grad_1024x4E_S8": dict(H=1024, E=4, M=1, n=8, m=64, sigma=(1.0, 4.0), moffat=2.5,
+                            expo=(5e2, 2e3), mask=0.05, S=8, seed=2),
 }
 
 
@@ -507,7 +507,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": f"W*: {H}x{H}x{E}E spectral GP sky, 1 module, linpatch n={n} "
+        "config": {"workload": f"{H}x{H}x{E}E spectral GP sky, 1 module, linpatch n={n} "
                                f"margin={m} (L={2 * (H // n) + 2 * m}), Gaussian+Moffat PSFs, "
                                "exposure 5e2-2e3 s, 5% mask, Poisson NLL (FP64 reduction)",
                    "precision": "FP32 FFTs/PSF products/exp, FP64 NLL reductions"},

@@ -73,7 +73,7 @@ def test_grad_batch_vs_oracle_fp64(ja):
         assert _rel(ga_b[s], ga) <= 1e-11
 
 
-def test_c4_antithetic_metric_batch(ja):
+def test_antithetic_metric_batch(ja):
     """32 metric products (16 antithetic pairs +-v) on the 256^2 observation
     (n = 6, margin 22, L = 128) in one pass: the pairs cancel exactly and a
     pair checks against the oracle."""

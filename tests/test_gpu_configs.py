@@ -1,9 +1,10 @@
 """The geometries of the benchmark configurations on the GPU.
 
-C3: 7 modules x 3 energies, n = 10, margin 60 (L = 324 = 2^2 3^4).
-C4: three observations on 256^2 / 512^2 / 768^2 grids with n = 6 / 8 / 12
-    and margins 22 / 32 / 48 (L = 128 / 192 / 224) and metric products.
-C5: 2048^2 x 4 energies, n = 16, margin 64 (L = 384).
+Multi-module: 7 modules x 3 energies, n = 10, margin 60 (L = 324 = 2^2 3^4).
+Uneven grids: three observations on 256^2 / 512^2 / 768^2 grids with
+    n = 6 / 8 / 12 and margins 22 / 32 / 48 (L = 128 / 192 / 224) and
+    metric products.
+Wide: 2048^2 x 4 energies, n = 16, margin 64 (L = 384).
 Small instances are checked against the float64 oracle; the full sizes
 through FP32 vs FP64 kernels, adjointness and linearity.
 """
@@ -24,7 +25,7 @@ def _rel(a, b):
 
 
 @pytest.mark.parametrize("dtype,tol", [(torch.float64, 1e-12), (torch.float32, 2e-6)])
-def test_c3_geometry_L324_modules_vs_oracle(ja, dtype, tol):
+def test_modules_geometry_L324_vs_oracle(ja, dtype, tol):
     """L = 324 on a 303^2 grid (n = 3, stride 101, margin 61): the last
     window start 202 is clamped to H + 2h - w = 201; 2 modules x 3 energies."""
     H, n, m, E, M = 303, 3, 61, 3, 2
@@ -45,7 +46,7 @@ def test_c3_geometry_L324_modules_vs_oracle(ja, dtype, tol):
     assert _rel(got, want) <= tol
 
 
-def test_c3_full_size_adjointness(ja):
+def test_modules_1024x3E_full_size_adjointness(ja):
     """1024^2 x 3 energies, 7 modules, n = 10, margin 60 (FP32)."""
     H, n, m, E, M = 1024, 10, 60, 3, 7
     rng = np.random.default_rng(3)
@@ -61,7 +62,7 @@ def test_c3_full_size_adjointness(ja):
 
 
 @pytest.mark.parametrize("H,n,m,L", [(256, 6, 22, 128), (512, 8, 32, 192), (768, 12, 48, 224)])
-def test_c4_grids_vs_oracle(ja, H, n, m, L):
+def test_uneven_grids_vs_oracle(ja, H, n, m, L):
     rng = np.random.default_rng(H)
     x = rng.random((1, H, H))
     win = psf_windows(rng, n, 1, 2 * (H // n))
@@ -75,7 +76,7 @@ def test_c4_grids_vs_oracle(ja, H, n, m, L):
         assert _rel(got, want) <= tol
 
 
-def test_c4_metric_products_vs_oracle(ja):
+def test_uneven_grids_metric_products_vs_oracle(ja):
     """Metric products J^T diag(1/lam) J v for several antithetic samples on
     the 256^2 observation (n = 6, margin 22)."""
     prob, xi = make_problem(H=256, n=6, margin=22, E=1, M=1, seed=41, N=256)
@@ -90,7 +91,7 @@ def test_c4_metric_products_vs_oracle(ja):
             assert _rel(got.double().cpu().numpy(), want) <= 1e-5
 
 
-def test_c5_full_size_fp32_vs_fp64(ja):
+def test_wide_2048x4E_fp32_vs_fp64(ja):
     """2048^2 x 4 energies, n = 16, margin 64 (1024 patch-energies)."""
     H, n, m, E = 2048, 16, 64, 4
     rng = np.random.default_rng(5)
@@ -110,7 +111,7 @@ def test_c5_full_size_fp32_vs_fp64(ja):
     assert _rel(out[torch.float32][1], out[torch.float64][1]) <= 2e-6
 
 
-def test_c5_gp_2048_vs_oracle(ja):
+def test_gp_2048_vs_oracle(ja):
     """The 2048^2 spectral GP (FP32) against the oracle."""
     from oracle import sky as osky
     from tests.problems import powerlaw_amp
