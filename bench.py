@@ -299,8 +299,15 @@ def sweep(args):
                "forward_ms": 1e3 * time_steps(torch, fwd, steps, 2),
                "grad_eval_ms": 1e3 * time_steps(
                    torch, lambda: model.nll_and_grad_device(xr_t, xa_t, out=out), steps, 2)}
+        # column tile picked by the autotuner for single evaluations
+        tune = model.autotune(samples=1, reps=10)
+        rec.update({"tile_ms": tune["ms"], "col_div": tune["col_div"],
+                    "grad_eval_ms_tuned": 1e3 * time_steps(
+                        torch, lambda: model.nll_and_grad_device(xr_t, xa_t, out=out), steps, 2)})
         if c["S"]:
             S = c["S"]
+            tune = model.autotune(samples=S, reps=3)
+            rec.update({"tile_ms_batch": tune["ms"], "col_div_batch": tune["col_div"]})
             model.nll_and_grad_device(xr_t, xa_t, out=out)
             v = torch.randn((S // 2, H, H), device="cuda", dtype=torch.float32)
             vr = torch.cat([v, -v])
@@ -378,6 +385,8 @@ def main():
     inp = build_inputs(H, E, n, m)
     counts = draw_counts(ja, torch, inp)
     mixed = make_model(ja, torch, inp, torch.float32, counts)
+    # column tile of the column passes picked by the autotuner (set-up)
+    tune = mixed.autotune(samples=1, reps=10)
     xi32 = [torch.as_tensor(v, dtype=torch.float32, device="cuda") for v in inp["xi"]]
     out32 = mixed._grads()
     step = lambda: mixed.nll_and_grad_device(*xi32, out=out32)
@@ -412,11 +421,13 @@ def main():
     if args.samples > 1:
         S = args.samples
         bm = make_model(ja, torch, inp, torch.float32, counts, max_samples=S)
+        tune_b = bm.autotune(samples=S, reps=3)
         brng = np.random.default_rng(17)
         xb = [torch.as_tensor(v[None] + 0.05 * brng.normal(size=(S,) + v.shape),
                               dtype=torch.float32, device="cuda") for v in inp["xi"]]
         t_b = time_steps(torch, lambda: bm.nll_and_grad_batch(*xb), max(3, args.steps // S), 2)
-        batched = {"samples": S, "ms_per_call": t_b * 1e3, "evals_per_s": S / t_b}
+        batched = {"samples": S, "ms_per_call": t_b * 1e3, "evals_per_s": S / t_b,
+                   "column_tile_divisor": tune_b["col_div"]}
         del bm, xb
         torch.cuda.empty_cache()
 
@@ -511,6 +522,8 @@ def main():
                                f"margin={m} (L={2 * (H // n) + 2 * m}), Gaussian+Moffat PSFs, "
                                "exposure 5e2-2e3 s, 5% mask, Poisson NLL (FP64 reduction)",
                    "precision": "FP32 FFTs/PSF products/exp, FP64 NLL reductions"},
+        "autotune": {"column_tile_divisor": tune["col_div"],
+                     "ms_per_forward_adjoint": tune["ms"]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK,
                      "traffic": measured_traffic(dom, H, E, n, m), "kernel": dom,

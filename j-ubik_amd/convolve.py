@@ -8,7 +8,9 @@ plan (device workspace + prepared kernel spectra) behind it, with the
 adjoint used for gradients.
 """
 import ctypes
+import json
 import math
+from pathlib import Path
 
 import numpy as np
 import torch
@@ -58,6 +60,7 @@ class LinpatchOperator:
         self.dx, self.window, self.L, self.P, self.pad, self.J = (int(v) for v in info[:6])
         self.workspace_bytes, self.spectra_bytes = int(info[6]), int(info[7])
         self._kernel_ref = None
+        self.col_div = 1
 
     def __del__(self):
         plan = getattr(self, "_plan", None)
@@ -115,6 +118,67 @@ class LinpatchOperator:
         _lib.call("jubik_linpatch_adjoint", self._plan, dev.ptr(gt), dev.ptr(gx),
                   float(scale), dev.stream())
         return gx
+
+    # -------------------------------------------------------------- tuning
+    def tune_key(self, samples=1):
+        """Key of this geometry in a tile cache: device architecture, dtype,
+        FFT length, grid, energies, modules, patches, samples per pass."""
+        arch = getattr(torch.cuda.get_device_properties(torch.cuda.current_device()),
+                       "gcnArchName", "gpu")
+        dt = "f32" if self.dtype == torch.float32 else "f64"
+        return (f"{arch}/{dt}/L{self.L}/{self.H}x{self.W}/E{self.E}/M{self.M}/P{self.P}"
+                f"/S{int(samples)}/xs{int(self.x_shared)}/ke{int(self.kernel_per_energy)}")
+
+    def set_tile(self, col_div):
+        """Column-pass tile divisor: 1, 2 or 4 (results agree to rounding)."""
+        _lib.call("jubik_linpatch_set_tile", self._plan, int(col_div))
+        self.col_div = int(col_div)
+        return self
+
+    def autotune(self, reps=20, cache=None):
+        """Pick the fastest column-pass tile for this geometry by timing one
+        forward + adjoint per candidate (the kernel must be set).  ``cache``:
+        a JSON file of earlier choices, read first and updated."""
+        key = self.tune_key()
+        hit = tile_cache_get(cache, key)
+        if hit is not None:
+            self.set_tile(hit)
+            return {"col_div": hit, "ms": None, "cached": True}
+        best = ctypes.c_int()
+        ms = (ctypes.c_double * 3)()
+        _lib.call("jubik_linpatch_autotune", self._plan, int(reps), ctypes.byref(best), ms,
+                  dev.stream())
+        self.col_div = int(best.value)
+        tile_cache_put(cache, key, self.col_div)
+        return {"col_div": self.col_div, "cached": False,
+                "ms": {d: t for d, t in zip((1, 2, 4), ms) if t >= 0}}
+
+
+def tile_cache_get(cache, key):
+    """Tile divisor stored under ``key`` in the JSON file ``cache``, or None."""
+    if cache is None:
+        return None
+    try:
+        table = json.loads(Path(cache).read_text())
+    except FileNotFoundError:
+        return None
+    hit = table.get(key)
+    return None if hit is None else int(hit)
+
+
+def tile_cache_put(cache, key, col_div):
+    """Store a tile divisor under ``key`` in the JSON file ``cache``."""
+    if cache is None:
+        return
+    path = Path(cache)
+    try:
+        table = json.loads(path.read_text())
+    except FileNotFoundError:
+        table = {}
+    table[key] = int(col_div)
+    tmp = path.with_suffix(path.suffix + ".tmp")
+    tmp.write_text(json.dumps(table, indent=1, sort_keys=True) + "\n")
+    tmp.replace(path)
 
 
 _PLANS = {}

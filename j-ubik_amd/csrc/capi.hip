@@ -146,7 +146,8 @@ struct jubik_linpatch {
   void* KB = nullptr;
   size_t ws_bytes = 0, spec_bytes = 0;
   bool ready = false;
-  LinBuffers bufs() const { return LinBuffers{tw, tent, KT, R1, R2, KB}; }
+  int col_div = 1;  // column-pass tile divisor (set by the autotuner)
+  LinBuffers bufs() const { return LinBuffers{tw, tent, KT, R1, R2, KB, col_div}; }
   void release() {
     dev_free(tw);
     dev_free(tent);
@@ -193,6 +194,7 @@ struct jubik_model {
   void* R2 = nullptr;
   void* gwork = nullptr;    // own GP workspace (max_samples > 1)
   bool have_point = false;
+  int col_div = 0;  // column-pass tile divisor; 0 = the plan's
   void release() {
     dev_free(sky);
     dev_free(dsky);
@@ -208,6 +210,7 @@ struct jubik_model {
     LinBuffers b = lp->bufs();
     if (R1) b.R1 = R1;
     if (R2) b.R2 = R2;
+    if (col_div) b.col_div = col_div;
     return b;
   }
   GpBuffers gbufs() const {
@@ -417,6 +420,84 @@ int jubik_linpatch_adjoint(jubik_linpatch* plan, const void* gy, void* gx, doubl
   if (!plan->ready) return fail(kNotReady, "no kernel set (call jubik_linpatch_set_kernel)");
   DeviceGuard dg(plan->device);
   return linpatch_adjoint(plan->dtype, plan->g, plan->bufs(), gy, gx, scale, as_stream(stream));
+}
+
+// Times one linpatch forward + adjoint (S samples, the given workspaces) for
+// every column-tile divisor compiled in for L, on zero inputs, and returns
+// the fastest divisor; ms[0..2] receive the times of divisors 1, 2, 4
+// (-1 where not compiled in).  Set-up work: allocates and synchronises.
+static int autotune_cols(int dtype, const LinGeom& g0, LinBuffers b, int S, int reps,
+                         int* best, double* ms, hipStream_t s) {
+  LinGeom g = g0;
+  g.S = S;
+  const size_t bytes = (size_t)S * g.J * g.H * g.W * elem_size(dtype);
+  void *x = nullptr, *y = nullptr;
+  int st = dev_alloc(&x, bytes, "autotune input");
+  if (!st) st = dev_alloc(&y, bytes, "autotune output");
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (!st && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess))
+    st = fail(kRuntimeError, "creating timing events");
+  if (!st && (hipMemsetAsync(x, 0, bytes, s) != hipSuccess ||
+              hipMemsetAsync(y, 0, bytes, s) != hipSuccess))
+    st = fail(kRuntimeError, "clearing autotune buffers");
+  double best_ms = -1.0;
+  *best = 1;
+  const int divs[3] = {1, 2, 4};
+  for (int k = 0; k < 3 && !st; ++k) {
+    ms[k] = -1.0;
+    if (!linpatch_col_tile(dtype, g.L, divs[k])) continue;
+    b.col_div = divs[k];
+    auto once = [&]() {
+      int r = linpatch_forward(dtype, g, b, x, y, 1.0, s);
+      if (!r) r = linpatch_adjoint(dtype, g, b, y, x, 1.0, s);
+      return r;
+    };
+    for (int w = 0; w < 2 && !st; ++w) st = once();
+    if (st || hipEventRecord(e0, s) != hipSuccess) break;
+    for (int r = 0; r < reps && !st; ++r) st = once();
+    if (st) break;
+    float t = 0.f;
+    if (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+        hipEventElapsedTime(&t, e0, e1) != hipSuccess) {
+      st = fail(kRuntimeError, "timing a column tile");
+      break;
+    }
+    ms[k] = (double)t / reps;
+    if (best_ms < 0.0 || ms[k] < best_ms) {
+      best_ms = ms[k];
+      *best = divs[k];
+    }
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (x || y) (void)hipStreamSynchronize(s);
+  dev_free(x);
+  dev_free(y);
+  return st;
+}
+
+int jubik_linpatch_set_tile(jubik_linpatch* plan, int col_div) {
+  if (!plan) return fail(kInvalidArgument, "NULL argument");
+  if (!linpatch_col_tile(plan->dtype, plan->g.L, col_div))
+    return fail(kInvalidArgument, "column tile divisor " + std::to_string(col_div) +
+                                      " is not compiled in for L = " + std::to_string(plan->g.L));
+  plan->col_div = col_div;
+  return kOk;
+}
+
+int jubik_linpatch_autotune(jubik_linpatch* plan, int reps, int* col_div, double* ms,
+                            void* stream) {
+  if (!plan || !col_div || !ms) return fail(kInvalidArgument, "NULL argument");
+  if (reps < 1) return fail(kInvalidArgument, "reps must be positive");
+  if (!plan->ready) return fail(kNotReady, "no kernel set (call jubik_linpatch_set_kernel)");
+  DeviceGuard dg(plan->device);
+  int best = 1;
+  const int st = autotune_cols(plan->dtype, plan->g, plan->bufs(), 1, reps, &best, ms,
+                               as_stream(stream));
+  if (st) return st;
+  plan->col_div = best;
+  *col_div = best;
+  return kOk;
 }
 
 // ------------------------------------------------------------------ primitives
@@ -874,6 +955,37 @@ int jubik_model_grad_batch(jubik_model* model, int64_t S, const void* amp_ref,
   DeviceGuard dg(model->lp->device);
   return model_grad(model, (int)S, 1, amp_ref, amp_alpha, xi_ref, xi_alpha, zm, mu, dnu, nll,
                     g_ref, g_alpha, as_stream(stream));
+}
+
+int jubik_model_set_tile(jubik_model* model, int col_div) {
+  if (!model || !model->lp) return fail(kInvalidArgument, "NULL argument");
+  if (col_div != 0 && !linpatch_col_tile(model->lp->dtype, model->lp->g.L, col_div))
+    return fail(kInvalidArgument, "column tile divisor " + std::to_string(col_div) +
+                                      " is not compiled in for L = " +
+                                      std::to_string(model->lp->g.L));
+  model->col_div = col_div;
+  return kOk;
+}
+
+int jubik_model_autotune(jubik_model* model, int64_t S, int reps, int* col_div, double* ms,
+                         void* stream) {
+  if (!model || !col_div || !ms) return fail(kInvalidArgument, "NULL argument");
+  if (reps < 1) return fail(kInvalidArgument, "reps must be positive");
+  if (S < 1 || S > model->max_samples)
+    return fail(kInvalidArgument, "batch of " + std::to_string(S) +
+                                      " samples outside [1, max_samples = " +
+                                      std::to_string(model->max_samples) + "]");
+  int st = check_model(model);
+  if (st) return st;
+  DeviceGuard dg(model->lp->device);
+  LinBuffers b = model->lbufs();
+  int best = 1;
+  st = autotune_cols(model->lp->dtype, model->lp->g, b, (int)S, reps, &best, ms,
+                     as_stream(stream));
+  if (st) return st;
+  model->col_div = best;
+  *col_div = best;
+  return kOk;
 }
 
 int jubik_model_grad_graph(jubik_graph** out, jubik_model* model, int64_t S,

@@ -32,6 +32,13 @@ bool linpatch_length_supported(int L) {
   }
 }
 
+int linpatch_col_tile(int dtype, int L, int col_div) {
+  if ((dtype != 0 && dtype != 1) || !linpatch_length_supported(L)) return 0;
+  if (col_div != 1 && col_div != 2 && col_div != 4) return 0;
+  const int v = tile_vectors(JUBIK_COL_TILE_BYTES, L, dtype == 0 ? 8 : 16) / col_div;
+  return (col_div == 1 || v >= 2) ? v : 0;
+}
+
 int linpatch_prepare(int dtype, const LinGeom& g, const LinBuffers& b, const void* kernel,
                      int layout, hipStream_t s) {
   if (dtype == 0) return lp_prepare_t<float>(g, b, kernel, layout, s);

@@ -12,6 +12,7 @@ struct LinBuffers {
   void* R1;          // row-pass workspace, S x J x P x L x L2p
   void* R2;          // column-pass workspace, S x J x P x L x L2p
   void* KB;          // beta of the packed DC/Nyquist column, (M, P, Ek) x L
+  int col_div;       // column-pass tile: 1, 2 or 4 (a full, half or quarter tile)
 };
 
 // FFT lengths L = 2 (H / n) + 2 margin compiled into the library.
@@ -19,6 +20,10 @@ struct LinBuffers {
   X(16) X(24) X(32) X(48) X(64) X(96) X(128) X(192) X(224) X(256) X(324) X(384)
 
 bool linpatch_length_supported(int L);
+// Columns per work-group of the forward and adjoint column passes for a
+// column-tile divisor (1, 2 or 4), or 0 when that divisor is not compiled in
+// for L (fewer than 2 columns).
+int linpatch_col_tile(int dtype, int L, int col_div);
 
 // dtype: 0 = float32, 1 = float64.  layout: 0 = full frame (H, W), 1 = cut
 // window (w, w).  Kernels are laid out (M, P, Ek, ...).

@@ -1,0 +1,97 @@
+"""Column-tile autotuning of the linpatch column passes: every tile gives
+the same results to rounding (the same butterflies run per column, only
+their grouping into work-groups and the compiler's contraction of
+multiply-adds change: a few ulp of the largest output), the tuner returns
+a compiled tile with its timings, and a tile cache round-trips."""
+import json
+
+import numpy as np
+import pytest
+import torch
+
+from tests.problems import make_problem, psf_windows, to_product
+
+pytestmark = pytest.mark.gpu
+
+
+def _operator(ja, H, n, m, E, dtype, seed):
+    rng = np.random.default_rng(seed)
+    op = ja.LinpatchOperator((E, H, H), n, m, dtype=dtype)
+    op.set_kernel(psf_windows(rng, n, E, 2 * (H // n)), layout="window")
+    x = torch.as_tensor(rng.normal(size=(E, H, H)), dtype=dtype, device="cuda")
+    gy = torch.as_tensor(rng.normal(size=(E, H, H)), dtype=dtype, device="cuda")
+    return op, x, gy
+
+
+# Largest difference allowed, relative to the largest output: 4 ulp.
+ULPS = {torch.float32: 4 * 2.0 ** -23, torch.float64: 4 * 2.0 ** -52}
+
+
+def _close(a, b, dtype):
+    return float((a - b).abs().max()) <= ULPS[dtype] * float(b.abs().max())
+
+
+@pytest.mark.parametrize("H,n,m,E", [(128, 4, 16, 1), (256, 2, 64, 2), (303, 3, 61, 1)])
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+def test_every_tile_agrees_to_rounding(ja, H, n, m, E, dtype):
+    op, x, gy = _operator(ja, H, n, m, E, dtype, seed=H + n)
+    ref_y, ref_g = op.forward(x).clone(), op.adjoint(gy).clone()
+    for d in (2, 4):
+        op.set_tile(d)
+        assert _close(op.forward(x), ref_y, dtype), f"forward, col_div {d}"
+        assert _close(op.adjoint(gy), ref_g, dtype), f"adjoint, col_div {d}"
+    # back to the default tile: bit for bit the first result again
+    op.set_tile(1)
+    assert torch.equal(op.forward(x), ref_y) and torch.equal(op.adjoint(gy), ref_g)
+
+
+def test_autotune_picks_a_compiled_tile(ja, tmp_path):
+    op, x, gy = _operator(ja, 128, 4, 16, 1, torch.float32, seed=5)
+    ref_y = op.forward(x).clone()
+    res = op.autotune(reps=5)
+    assert res["col_div"] in (1, 2, 4) and not res["cached"]
+    assert set(res["ms"]) == {1, 2, 4} and all(t > 0 for t in res["ms"].values())
+    assert res["ms"][res["col_div"]] == min(res["ms"].values())
+    assert op.col_div == res["col_div"]
+    assert _close(op.forward(x), ref_y, torch.float32)
+    # the choice is stored and read back by a plan of the same geometry
+    cache = tmp_path / "tiles.json"
+    first = op.autotune(reps=5, cache=cache)
+    table = json.loads(cache.read_text())
+    assert table[op.tune_key()] == first["col_div"]
+    table[op.tune_key()] = 4
+    cache.write_text(json.dumps(table))
+    op2, _, _ = _operator(ja, 128, 4, 16, 1, torch.float32, seed=5)
+    again = op2.autotune(cache=cache)
+    assert again == {"col_div": 4, "ms": None, "cached": True} and op2.col_div == 4
+
+
+def test_invalid_tile_is_rejected(ja):
+    op, _, _ = _operator(ja, 128, 4, 16, 1, torch.float32, seed=6)
+    for d in (0, 3, 8, -1):
+        with pytest.raises(ValueError):
+            op.set_tile(d)
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+def test_model_tiles_agree(ja, dtype):
+    prob, _ = make_problem(H=256, n=4, margin=32, E=2, M=1, seed=81)
+    model = to_product(ja, prob, dtype=dtype, max_samples=4)
+    rng = np.random.default_rng(8)
+    N = model.sky.N
+    xr = torch.as_tensor(rng.normal(size=(4, N, N)), dtype=dtype, device="cuda")
+    xa = torch.as_tensor(rng.normal(size=(4, N, N)), dtype=dtype, device="cuda")
+    want = model.nll_and_grad_batch(xr, xa)
+    res = model.autotune(samples=4, reps=3)
+    assert res["col_div"] in (1, 2, 4) and set(res["ms"]) == {1, 2, 4}
+    # gradients to the bar of the FP32 kernels against FP64 (1e-5), tighter
+    # in FP64; the default tile again gives bit-identical results
+    tol = 1e-6 if dtype == torch.float32 else 1e-13
+    for d in (2, 4, 1):
+        model.set_tile(d)
+        got = model.nll_and_grad_batch(xr, xa)
+        assert np.allclose(got[0], want[0], rtol=tol, atol=0)
+        for g, w in zip(got[1:], want[1:]):
+            assert float((g - w).norm() / w.norm()) <= tol
+    assert np.array_equal(got[0], want[0]) and torch.equal(got[1], want[1])
+    assert torch.equal(got[2], want[2])
