@@ -47,6 +47,10 @@ def parse():
                     help="only run the mixed gradient steps (for rocprofv3)")
     ap.add_argument("--samples", type=int, default=8,
                     help="VI samples per call of the batched gradient leg (0: skip)")
+    ap.add_argument("--tile", default="auto", choices=["auto", "1", "2", "4"],
+                    help="column-tile divisor of the column passes: the autotuner's pick "
+                         "(auto) or a fixed one (profiling runs repeat the pick without "
+                         "the tuner's own launches)")
     ap.add_argument("--sweep", action="store_true",
                     help="time every benchmark configuration instead of the 1024^2 x 4E line")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
@@ -386,7 +390,11 @@ def main():
     counts = draw_counts(ja, torch, inp)
     mixed = make_model(ja, torch, inp, torch.float32, counts)
     # column tile of the column passes picked by the autotuner (set-up)
-    tune = mixed.autotune(samples=1, reps=10)
+    if args.tile == "auto":
+        tune = mixed.autotune(samples=1, reps=10)
+    else:
+        mixed.set_tile(int(args.tile))
+        tune = {"col_div": int(args.tile), "ms": None}
     xi32 = [torch.as_tensor(v, dtype=torch.float32, device="cuda") for v in inp["xi"]]
     out32 = mixed._grads()
     step = lambda: mixed.nll_and_grad_device(*xi32, out=out32)
@@ -421,7 +429,11 @@ def main():
     if args.samples > 1:
         S = args.samples
         bm = make_model(ja, torch, inp, torch.float32, counts, max_samples=S)
-        tune_b = bm.autotune(samples=S, reps=3)
+        if args.tile == "auto":
+            tune_b = bm.autotune(samples=S, reps=3)
+        else:
+            bm.set_tile(int(args.tile))
+            tune_b = {"col_div": int(args.tile)}
         brng = np.random.default_rng(17)
         xb = [torch.as_tensor(v[None] + 0.05 * brng.normal(size=(S,) + v.shape),
                               dtype=torch.float32, device="cuda") for v in inp["xi"]]
