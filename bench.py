@@ -254,6 +254,7 @@ def sweep(args):
     ja = ge.load_package()
     from tests.problems import powerlaw_amp, psf_windows
     from oracle import model as om
+    uneven = []
     for name, c in CONFIGS.items():
         H, E, M, n, m = c["H"], c["E"], c["M"], c["n"], c["m"]
         rng = np.random.default_rng(c["seed"])
@@ -333,9 +334,49 @@ def sweep(args):
             om.nll_and_grad(prob, xi, xa)
             rec["cpu_oracle_grad_eval_ms"] = 1e3 * (time.perf_counter() - t0)
         print(json.dumps(rec), flush=True)
+        if name.startswith("grid_"):
+            # kept for the uneven-grid graph below, with its single-evaluation tile
+            model.set_tile(rec["col_div"])
+            uneven.append((name, model, xr_t, xa_t))
         del model, sky, psf
         torch.cuda.empty_cache()
+    if len(uneven) > 1:
+        print(json.dumps(uneven_graph(ja, torch, uneven, args.steps)), flush=True)
+    del uneven
     print(json.dumps(dft_compare(ja, torch, args)), flush=True)
+
+
+def uneven_graph(ja, torch, uneven, steps):
+    """One gradient evaluation of each observation of an uneven-grid set:
+    eager calls one after another, one captured graph per observation
+    launched one after another, and one graph holding all of them as
+    parallel branches (jubik_graph_capture_batch)."""
+    outs = [mdl._grads() for _, mdl, _, _ in uneven]
+
+    def eager():
+        for (_, mdl, xr, xa), o in zip(uneven, outs):
+            mdl.nll_and_grad_device(xr, xa, out=o)
+
+    singles = [mdl.capture_grad(0) for _, mdl, _, _ in uneven]
+    batch = ja.capture_grad_batch([(mdl, 0) for _, mdl, _, _ in uneven])
+    for gs, (_, _, xr, xa) in zip(singles, uneven):
+        gs.xi_ref.copy_(xr)
+        if xa is not None:
+            gs.xi_alpha.copy_(xa)
+    for sl, (_, _, xr, xa) in zip(batch.slots, uneven):
+        sl.xi_ref.copy_(xr)
+        if xa is not None:
+            sl.xi_alpha.copy_(xa)
+
+    def graphs():
+        for gs in singles:
+            gs.launch()
+
+    t_e = time_steps(torch, eager, steps, 2)
+    t_s = time_steps(torch, graphs, steps, 2)
+    t_b = time_steps(torch, batch.launch, steps, 2)
+    return {"uneven_grids": [nm for nm, _, _, _ in uneven], "eager_ms": 1e3 * t_e,
+            "graph_per_grid_ms": 1e3 * t_s, "one_graph_ms": 1e3 * t_b}
 
 
 def dft_compare(ja, torch, args):

@@ -273,6 +273,29 @@ int jubik_model_grad_graph(jubik_graph** graph, jubik_model* model, int64_t S,
                            double* nll, void* g_ref, void* g_alpha);
 int jubik_graph_launch(jubik_graph* graph, void* stream);
 int jubik_graph_destroy(jubik_graph* graph);
+/* Several gradient evaluations in one graph: the modules, energies and
+ * samples of observations on uneven grids (the instruments' per-module
+ * vmaps, chandra_response.py:127-130 and erosita_response.py:209-212, with
+ * one model per grid).  Each descriptor is one jubik_model_grad (samples =
+ * 0, sets that model's base point at launch) or jubik_model_grad_batch
+ * (samples = S) call; the evaluations become parallel branches of the
+ * graph, so small grids that cannot fill the device alone run side by side.
+ * Descriptors may not share a model or a plan; at most 64. */
+typedef struct jubik_grad_desc {
+  jubik_model* model;
+  int64_t samples;
+  const void* amp_ref;
+  const void* amp_alpha;
+  const void* xi_ref;
+  const void* xi_alpha;
+  double zm;
+  double mu;
+  const void* dnu;
+  double* nll;
+  void* g_ref;
+  void* g_alpha;
+} jubik_grad_desc;
+int jubik_graph_capture_batch(jubik_graph** graph, const jubik_grad_desc* descs, int64_t count);
 /* Left square root of the likelihood metric at the base point, for S noise
  * vectors: out = J^T diag(lam^-1/2) noise, noise (S, M, E, H, W) in data
  * space (ignored where counts < 0).  With noise ~ N(0, 1) this draws from

@@ -63,6 +63,7 @@ SIGNATURES = {
     "jubik_model_grad_graph": [C.POINTER(_vp), _vp, _i64, _vp, _vp, _vp, _vp, _dbl, _dbl, _vp,
                                _vp, _vp, _vp],
     "jubik_graph_launch": [_vp, _vp],
+    "jubik_graph_capture_batch": [C.POINTER(_vp), _vp, _i64],
     "jubik_graph_destroy": [_vp],
     "jubik_dft128_create": [C.POINTER(_vp)],
     "jubik_dft128_run": [_vp, _int, _vp, _vp, _i64, _vp],
@@ -70,6 +71,15 @@ SIGNATURES = {
     "jubik_psf_interpolate": [_int, _vp, _i64, _i64, _i64, _vp, _vp, _vp, _vp, _i64, _i64, _dbl,
                               _dbl, _i64, _i64, _i64, _int, _vp, _vp],
 }
+
+
+
+class GradDesc(C.Structure):
+    """jubik_grad_desc: one evaluation of a multi-model graph."""
+    _fields_ = [("model", _vp), ("samples", _i64), ("amp_ref", _vp), ("amp_alpha", _vp),
+                ("xi_ref", _vp), ("xi_alpha", _vp), ("zm", _dbl), ("mu", _dbl), ("dnu", _vp),
+                ("nll", _vp), ("g_ref", _vp), ("g_alpha", _vp)]
+
 
 _lock = threading.Lock()
 _lib = None

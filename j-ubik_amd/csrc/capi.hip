@@ -232,8 +232,7 @@ static hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s);
 struct jubik_graph {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
-  jubik_model* model = nullptr;
-  bool single = false;
+  std::vector<jubik_model*> points;  // models whose single evaluation sets the base point
   int device = 0;
 };
 
@@ -1008,12 +1007,11 @@ int jubik_model_grad_graph(jubik_graph** out, jubik_model* model, int64_t S,
   hipStream_t cs = nullptr;
   JUBIK_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
   auto* gr = new jubik_graph();
-  gr->model = model;
-  gr->single = (S == 0);
+  if (S == 0) gr->points.push_back(model);
   gr->device = model->lp->device;
   hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
   if (e == hipSuccess) {
-    st = gr->single ? model_grad(model, 1, 0, amp_ref, amp_alpha, xi_ref, xi_alpha, zm, mu, dnu,
+    st = S == 0 ? model_grad(model, 1, 0, amp_ref, amp_alpha, xi_ref, xi_alpha, zm, mu, dnu,
                                  nll, g_ref, g_alpha, cs)
                     : model_grad(model, (int)S, 1, amp_ref, amp_alpha, xi_ref, xi_alpha, zm, mu,
                                  dnu, nll, g_ref, g_alpha, cs);
@@ -1041,7 +1039,91 @@ int jubik_graph_launch(jubik_graph* graph, void* stream) {
   if (!graph || !graph->exec) return fail(kInvalidArgument, "NULL graph");
   DeviceGuard dg(graph->device);
   JUBIK_HIP(hipGraphLaunch(graph->exec, as_stream(stream)));
-  if (graph->single) graph->model->have_point = true;
+  for (jubik_model* m : graph->points) m->have_point = true;
+  return kOk;
+}
+
+int jubik_graph_capture_batch(jubik_graph** out, const jubik_grad_desc* descs, int64_t count) {
+  if (!out || !descs) return fail(kInvalidArgument, "NULL argument");
+  *out = nullptr;
+  if (count < 1 || count > 64) return fail(kInvalidArgument, "count must be in [1, 64]");
+  if (timing_enabled())
+    return fail(kInvalidArgument, "disable kernel timing before capturing a graph");
+  for (int64_t i = 0; i < count; ++i) {
+    const jubik_grad_desc& d = descs[i];
+    const std::string at = "descriptor " + std::to_string(i) + ": ";
+    if (!d.model || !d.nll || !d.g_ref || !d.amp_ref || !d.xi_ref)
+      return fail(kInvalidArgument, at + "NULL argument");
+    if (d.samples < 0 || d.samples > d.model->max_samples)
+      return fail(kInvalidArgument, at + "batch of " + std::to_string(d.samples) +
+                                        " samples outside [0, max_samples = " +
+                                        std::to_string(d.model->max_samples) + "]");
+    int st = check_model(d.model);
+    if (!st) st = check_alpha(d.model, d.amp_alpha, d.xi_alpha, d.g_alpha, d.dnu);
+    if (st) return st;
+    if (d.model->lp->device != descs[0].model->lp->device)
+      return fail(kInvalidArgument, at + "models on different devices");
+    // the evaluations run as parallel branches: no two may share workspaces
+    for (int64_t k = 0; k < i; ++k) {
+      const jubik_model* o = descs[k].model;
+      if (o == d.model || o->lp == d.model->lp || o->gp == d.model->gp)
+        return fail(kInvalidArgument, at + "shares its model or a plan with descriptor " +
+                                          std::to_string(k));
+    }
+  }
+  DeviceGuard dg(descs[0].model->lp->device);
+  const int n = (int)count;
+  hipStream_t cs = nullptr;
+  std::vector<hipStream_t> br(n, nullptr);
+  std::vector<hipEvent_t> ev(n + 1, nullptr);
+  int st = check_hip(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking), "hipStreamCreate");
+  for (int i = 0; i < n && !st; ++i)
+    st = check_hip(hipStreamCreateWithFlags(&br[i], hipStreamNonBlocking), "hipStreamCreate");
+  for (int i = 0; i <= n && !st; ++i)
+    st = check_hip(hipEventCreateWithFlags(&ev[i], hipEventDisableTiming), "hipEventCreate");
+  auto* gr = new jubik_graph();
+  gr->device = descs[0].model->lp->device;
+  if (!st) st = check_hip(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal),
+                          "hipStreamBeginCapture");
+  if (!st) {
+    // fork: every branch waits on the origin stream, runs its evaluation
+    // and is joined back before the capture ends
+    int cst = check_hip(hipEventRecord(ev[0], cs), "hipEventRecord");
+    for (int i = 0; i < n && !cst; ++i) {
+      const jubik_grad_desc& d = descs[i];
+      cst = check_hip(hipStreamWaitEvent(br[i], ev[0], 0), "hipStreamWaitEvent");
+      if (cst) break;
+      cst = d.samples == 0
+                ? model_grad(d.model, 1, 0, d.amp_ref, d.amp_alpha, d.xi_ref, d.xi_alpha, d.zm,
+                             d.mu, d.dnu, d.nll, d.g_ref, d.g_alpha, br[i])
+                : model_grad(d.model, (int)d.samples, 1, d.amp_ref, d.amp_alpha, d.xi_ref,
+                             d.xi_alpha, d.zm, d.mu, d.dnu, d.nll, d.g_ref, d.g_alpha, br[i]);
+      const int jst = check_hip(hipEventRecord(ev[i + 1], br[i]), "hipEventRecord");
+      const int wst = jst ? jst : check_hip(hipStreamWaitEvent(cs, ev[i + 1], 0),
+                                            "hipStreamWaitEvent");
+      if (!cst) cst = wst;
+      if (d.samples == 0) gr->points.push_back(d.model);
+    }
+    hipGraph_t g = nullptr;
+    const hipError_t e2 = hipStreamEndCapture(cs, &g);
+    gr->graph = g;
+    st = cst;
+    if (!st && e2 != hipSuccess) st = check_hip(e2, "hipStreamEndCapture");
+    if (!st) st = check_hip(hipGraphInstantiate(&gr->exec, gr->graph, nullptr, nullptr, 0),
+                            "hipGraphInstantiate");
+  }
+  for (hipEvent_t e : ev)
+    if (e) (void)hipEventDestroy(e);
+  for (hipStream_t b : br)
+    if (b) (void)hipStreamDestroy(b);
+  if (cs) (void)hipStreamDestroy(cs);
+  if (st) {
+    if (gr->exec) (void)hipGraphExecDestroy(gr->exec);
+    if (gr->graph) (void)hipGraphDestroy(gr->graph);
+    delete gr;
+    return st;
+  }
+  *out = gr;
   return kOk;
 }
 

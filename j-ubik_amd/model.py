@@ -239,6 +239,65 @@ class XrayModel:
         return GradGraph(self, int(samples))
 
 
+class _GradSlot:
+    """Fixed device buffers of one captured evaluation: positions in,
+    NLL and gradients out."""
+
+    def __init__(self, model, samples):
+        s = model.sky
+        if not 0 <= samples <= model.max_samples:
+            raise ValueError(f"samples must be in [0, max_samples={model.max_samples}]")
+        shape = ((samples,) if samples else ()) + (s.N, s.N)
+        self.model, self.samples = model, samples
+        self.xi_ref = torch.zeros(shape, dtype=model.dtype, device="cuda")
+        self.xi_alpha = torch.zeros(shape, dtype=model.dtype, device="cuda") if s.has_alpha else None
+        self.g_ref = torch.empty(shape, dtype=model.dtype, device="cuda")
+        self.g_alpha = torch.empty(shape, dtype=model.dtype, device="cuda") if s.has_alpha else None
+        self.nll = torch.empty(max(samples, 1), dtype=torch.float64, device="cuda")
+
+    def desc(self):
+        s = self.model.sky
+        return _lib.GradDesc(self.model._model.value, self.samples, dev.ptr(s.amp_ref),
+                             dev.ptr(s.amp_alpha), dev.ptr(self.xi_ref), dev.ptr(self.xi_alpha),
+                             s.zm, s.mu, dev.ptr(s.dnu_dev), dev.ptr(self.nll),
+                             dev.ptr(self.g_ref), dev.ptr(self.g_alpha))
+
+
+def capture_grad_batch(entries):
+    """Capture the gradient evaluations of several models (observations on
+    uneven grids, one model each) into one HIP graph whose evaluations run
+    as parallel branches.  ``entries``: (model, samples) pairs, samples = 0
+    for a single evaluation (its launches set that model's metric base
+    point) or S >= 1 for a batch.  Returns a ``BatchGradGraph``: write
+    positions into ``graph.slots[i].xi_ref`` / ``xi_alpha``, call
+    ``graph.launch()``, read ``slots[i].nll`` / ``g_ref`` / ``g_alpha``."""
+    return BatchGradGraph([_GradSlot(m, int(S)) for m, S in entries])
+
+
+class BatchGradGraph:
+    """Several captured gradient evaluations in one executable graph."""
+
+    def __init__(self, slots):
+        self.slots = slots
+        descs = (_lib.GradDesc * len(slots))(*[sl.desc() for sl in slots])
+        self._graph = ctypes.c_void_p()
+        _lib.call("jubik_graph_capture_batch", ctypes.byref(self._graph),
+                  ctypes.cast(descs, ctypes.c_void_p), len(slots))
+
+    def launch(self):
+        """Enqueue one replay on the current stream (no synchronisation)."""
+        _lib.call("jubik_graph_launch", self._graph, dev.stream())
+
+    def __del__(self):
+        g = getattr(self, "_graph", None)
+        if g is not None and g.value:
+            try:
+                _lib.load().jubik_graph_destroy(g)
+            except Exception:
+                pass
+            self._graph = None
+
+
 class GradGraph:
     """A captured gradient evaluation with its fixed device buffers."""
 

@@ -83,3 +83,79 @@ def test_library_graph_matches_eager(ja, samples):
         model2.nll_and_grad(xi[0], xi[1])
         m_eager = model2.metric(v, v)
         assert np.array_equal(m_graph[0], m_eager[0])
+
+
+def _uneven_models(ja, dtype):
+    """Three observations on uneven grids: 64^2 x 2E (spectral), 128^2 with
+    two modules (one energy, batched 2 samples), 96^2 x 2E on a 128^2 GP."""
+    cfgs = [(dict(H=64, n=4, margin=8, E=2, M=1, seed=31), 1),
+            (dict(H=128, n=4, margin=16, E=1, M=2, seed=32), 2),
+            (dict(H=96, n=4, margin=24, E=2, M=1, seed=33, N=128), 1)]
+    out = []
+    for cfg, ms in cfgs:
+        prob, xi = make_problem(**cfg)
+        out.append((prob, xi, to_product(ja, prob, dtype, max_samples=ms)))
+    return out
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+def test_batch_graph_of_uneven_grids_matches_eager(ja, dtype):
+    """jubik_graph_capture_batch: the evaluations of three models on uneven
+    grids as parallel branches of one graph; every replay equals eager
+    evaluations bit for bit, and launches set the single evaluations'
+    metric base points."""
+    models = _uneven_models(ja, dtype)
+    samples = [0, 2, 0]
+    graph = ja.capture_grad_batch([(m, S) for (_, _, m), S in zip(models, samples)])
+    rng = np.random.default_rng(11)
+    for _ in range(3):
+        pos = []
+        for (prob, xi, m), S, slot in zip(models, samples, graph.slots):
+            lead = (S,) if S else ()
+            pr = xi[0] + 0.05 * rng.normal(size=lead + xi[0].shape)
+            pa = xi[1] + 0.05 * rng.normal(size=lead + xi[1].shape) if xi[1] is not None else None
+            slot.xi_ref.copy_(torch.as_tensor(pr))
+            if pa is not None:
+                slot.xi_alpha.copy_(torch.as_tensor(pa))
+            pos.append((pr, pa))
+        graph.launch()
+        torch.cuda.synchronize()
+        for (prob, xi, m), S, slot, (pr, pa) in zip(models, samples, graph.slots, pos):
+            if S:
+                nll_e, gr_e, ga_e = m.nll_and_grad_batch(pr, pa)
+                assert np.array_equal(slot.nll.cpu().numpy(), nll_e)
+            else:
+                nll_e, gr_e, ga_e = m.nll_and_grad(pr, pa)
+                assert float(slot.nll[0].item()) == nll_e
+            assert np.array_equal(slot.g_ref.cpu().numpy(), gr_e)
+            if ga_e is not None:
+                assert np.array_equal(slot.g_alpha.cpu().numpy(), ga_e)
+            else:
+                assert slot.g_alpha is None
+    # base points: fresh models, one launch, then the metric of each single
+    # evaluation equals that of an eager evaluation at the same position
+    fresh = _uneven_models(ja, dtype)
+    g2 = ja.capture_grad_batch([(fresh[0][2], 0), (fresh[2][2], 0)])
+    for (prob, xi, m), slot in zip((fresh[0], fresh[2]), g2.slots):
+        slot.xi_ref.copy_(torch.as_tensor(xi[0]))
+        slot.xi_alpha.copy_(torch.as_tensor(xi[1]))
+    g2.launch()
+    torch.cuda.synchronize()
+    for (prob, xi, m), (_, _, m_eager) in zip((fresh[0], fresh[2]), (models[0], models[2])):
+        v = np.random.default_rng(3).normal(size=(2,) + xi[0].shape)
+        got = m.metric(v[0], v[1])
+        m_eager.nll_and_grad(xi[0], xi[1])
+        want = m_eager.metric(v[0], v[1])
+        assert np.array_equal(got[0], want[0]) and np.array_equal(got[1], want[1])
+
+
+def test_batch_graph_rejects_shared_workspaces(ja):
+    models = _uneven_models(ja, torch.float32)
+    m0 = models[0][2]
+    with pytest.raises(ValueError):
+        ja.capture_grad_batch([(m0, 0), (m0, 0)])
+    twin = ja.XrayModel(m0.sky, m0.psf, 1.0, m0.exposure, m0.counts)
+    with pytest.raises(ValueError):
+        ja.capture_grad_batch([(m0, 0), (twin, 0)])
+    with pytest.raises(ValueError):
+        ja.capture_grad_batch([(models[1][2], 3)])  # above max_samples
