@@ -62,6 +62,66 @@ def test_invalid_arguments_fail_before_device_work(ja):
                                 None) == _lib.EINVAL
 
 
+def test_tuning_and_batch_graph_entry_points_validate_arguments(ja):
+    from jubik_amd import _lib
+    lib = _lib.load()
+    best = ctypes.c_int()
+    ms = (ctypes.c_double * 3)()
+    assert lib.jubik_linpatch_set_tile(None, 1) == _lib.EINVAL
+    assert lib.jubik_linpatch_autotune(None, 5, ctypes.byref(best), ms, None) == _lib.EINVAL
+    assert lib.jubik_model_set_tile(None, 0) == _lib.EINVAL
+    assert lib.jubik_model_autotune(None, 1, 5, ctypes.byref(best), ms, None) == _lib.EINVAL
+    graph = ctypes.c_void_p()
+    assert lib.jubik_graph_capture_batch(ctypes.byref(graph), None, 1) == _lib.EINVAL
+    descs = (_lib.GradDesc * 1)()
+    st = lib.jubik_graph_capture_batch(ctypes.byref(graph), ctypes.cast(descs, ctypes.c_void_p), 0)
+    assert st == _lib.EINVAL and b"count" in lib.jubik_last_error()
+    st = lib.jubik_graph_capture_batch(ctypes.byref(graph), ctypes.cast(descs, ctypes.c_void_p), 1)
+    assert st == _lib.EINVAL and b"descriptor 0" in lib.jubik_last_error()
+
+
+def test_grad_desc_layout_matches_the_header(ja, tmp_path):
+    """The ctypes mirror of jubik_grad_desc has the C struct's size and
+    field offsets (checked with the host C compiler)."""
+    import shutil
+    import subprocess
+    from jubik_amd import _lib
+    cc = shutil.which("gcc") or shutil.which("cc")
+    if cc is None:
+        pytest.skip("no host C compiler")
+    fields = [f for f, _ in _lib.GradDesc._fields_]
+    src = tmp_path / "layout.c"
+    src.write_text("#include <stdio.h>\n#include <stddef.h>\n#include \"jubik_hip.h\"\n"
+                   "int main(void) {\n  printf(\"%zu\\n\", sizeof(jubik_grad_desc));\n"
+                   + "".join(f'  printf("%zu\\n", offsetof(jubik_grad_desc, {f}));\n'
+                             for f in fields)
+                   + "  return 0;\n}\n")
+    exe = tmp_path / "layout"
+    subprocess.run([cc, "-std=c99", "-I", str(HEADER.parent), str(src), "-o", str(exe)],
+                   check=True)
+    got = [int(v) for v in subprocess.run([str(exe)], check=True, capture_output=True,
+                                          text=True).stdout.split()]
+    assert got[0] == ctypes.sizeof(_lib.GradDesc)
+    assert got[1:] == [getattr(_lib.GradDesc, f).offset for f in fields]
+
+
+def test_tile_cache_round_trip(ja, tmp_path):
+    import json
+    from jubik_amd.convolve import tile_cache_get, tile_cache_put
+    cache = tmp_path / "tiles.json"
+    assert tile_cache_get(None, "k") is None
+    assert tile_cache_get(cache, "k") is None
+    tile_cache_put(None, "k", 2)  # no cache: nothing written
+    assert not cache.exists()
+    tile_cache_put(cache, "gfx950/f32/L384", 2)
+    tile_cache_put(cache, "gfx950/f64/L96", 4)
+    tile_cache_put(cache, "gfx950/f32/L384", 1)
+    assert tile_cache_get(cache, "gfx950/f32/L384") == 1
+    assert tile_cache_get(cache, "gfx950/f64/L96") == 4
+    assert tile_cache_get(cache, "other") is None
+    assert json.loads(cache.read_text()) == {"gfx950/f32/L384": 1, "gfx950/f64/L96": 4}
+
+
 def test_python_api_errors_match_reference(ja):
     x = np.ones((1, 16, 16))
     with pytest.raises(ValueError, match="domain has to be an instance of jubik.Domain"):
