@@ -31,7 +31,8 @@ def _close(a, b, dtype):
     return float((a - b).abs().max()) <= ULPS[dtype] * float(b.abs().max())
 
 
-@pytest.mark.parametrize("H,n,m,E", [(128, 4, 16, 1), (256, 2, 64, 2), (303, 3, 61, 1)])
+@pytest.mark.parametrize("H,n,m,E", [(128, 4, 16, 1), (256, 2, 64, 2), (303, 3, 61, 1),
+                                     (8, 2, 4, 1), (16, 2, 8, 2), (128, 2, 48, 1)])
 @pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
 def test_every_tile_agrees_to_rounding(ja, H, n, m, E, dtype):
     op, x, gy = _operator(ja, H, n, m, E, dtype, seed=H + n)

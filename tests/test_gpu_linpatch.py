@@ -28,6 +28,10 @@ GEOMS = [
     (512, 8, 64),   # L = 256
     (256, 2, 64),   # L = 384
     (1280, 10, 64), # L = 384, W > 1024: 8 output columns per thread
+    (8, 2, 4),      # L = 16, the shortest compiled length
+    (16, 2, 8),     # L = 32
+    (128, 2, 48),   # L = 224 = 2^5 7
+    (162, 2, 81),   # L = 324 = 2^2 3^4, odd stride 81
     (384, 2, 0),    # placeholder, margin 0 handled separately
 ]
 
@@ -64,7 +68,9 @@ def test_forward_matches_oracle(ja, H, n, m, dtype):
     assert err <= tol
 
 
-@pytest.mark.parametrize("H,n,m", _cases()[:5] + [(512, 8, 64), (256, 2, 64), (1280, 10, 64)])
+@pytest.mark.parametrize("H,n,m", _cases()[:5] + [(512, 8, 64), (256, 2, 64), (1280, 10, 64),
+                                                  (8, 2, 4), (16, 2, 8), (128, 2, 48),
+                                                  (162, 2, 81)])
 @pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
 def test_adjoint_matches_oracle(ja, H, n, m, dtype):
     E = 2
