@@ -11,7 +11,7 @@ from .convolve import (LinpatchOperator, convolve, integrate, linpatch_convolve,
                        linpatch_convolve_modules, slice_patches, slice_patches_transpose)
 from .response import build_exposure_function, build_readout_function
 from .likelihood import Poissonian
-from .sky import CorrelatedFieldSky, powerlaw_amplitude
+from .sky import CorrelatedFieldSky, powerlaw_amplitude, powerlaw_amplitude_grad
 from .model import XrayModel, capture_grad_batch, dense_counts
 from .sampling import conjugate_gradient, draw_linear_samples, metric_solve
 from .mock import create_mock_counts, poisson_sample

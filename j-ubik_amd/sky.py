@@ -35,6 +35,32 @@ def powerlaw_amplitude(N, distance=1.0, slope=-4.0, fluctuations=1.0):
     return amp
 
 
+def powerlaw_amplitude_grad(N, g_amp, distance=1.0, slope=-4.0, fluctuations=1.0):
+    """Gradients of a scalar with respect to the power-law amplitude's
+    hyperparameters (slope, fluctuations), given its gradient g_amp with
+    respect to the amplitude field ``powerlaw_amplitude`` builds (for
+    instance g_amp_ref of ``XrayModel.nll_and_grad_hyper``).
+
+    With amp = f u / |u|, u_k = |k|^(slope / 2):
+    d amp_k / d f = amp_k / f and
+    d amp_k / d slope = amp_k (ln|k_k| / 2 - sum_j amp_j^2 ln|k_j| / (2 f^2)).
+    Returns (g_slope, g_fluctuations) as floats (float64 sums on the host).
+    """
+    g = g_amp.detach().double().cpu().numpy() if isinstance(g_amp, torch.Tensor) else \
+        np.asarray(g_amp, dtype=np.float64)
+    if g.shape != (N, N):
+        raise ValueError(f"g_amp has shape {g.shape}, expected {(N, N)}")
+    amp = powerlaw_amplitude(N, distance, slope, fluctuations)
+    k = np.fft.fftfreq(N, d=distance)
+    kk = np.sqrt(k[:, None] ** 2 + k[None, :] ** 2)
+    half_log = np.zeros((N, N))
+    nz = kk > 0
+    half_log[nz] = 0.5 * np.log(kk[nz])
+    c = float(np.sum(amp ** 2 * half_log)) / fluctuations ** 2
+    ga = g * amp
+    return float(np.sum(ga * half_log) - c * np.sum(ga)), float(np.sum(ga)) / fluctuations
+
+
 class CorrelatedFieldSky:
     """GP sky plan: N x N GP grid cropped at the origin to (H, W)."""
 

@@ -64,3 +64,18 @@ def test_set_parameters_moves_the_model(ja):
     assert abs(got[5] - want[5]) <= 1e-10 * abs(want[5])
     with pytest.raises(ValueError):
         model.sky.set_parameters(amp_ref=np.ones((3, 3)))
+
+
+def test_powerlaw_hyperparameters_from_the_gpu_gradient(ja):
+    """The power-law slope and fluctuation gradients chained from the GPU's
+    amplitude-field gradient equal those chained from the oracle's."""
+    prob, xi = make_problem(H=64, n=4, margin=8, E=1, M=1, seed=95)
+    amp = ja.powerlaw_amplitude(64, slope=-3.5, fluctuations=0.8)
+    prob = replace(prob, amp_ref=amp)
+    want = om.nll_and_grad_hyper(prob, *xi)
+    model = to_product(ja, prob, dtype=torch.float64)
+    got = model.nll_and_grad_hyper(torch.as_tensor(xi[0], device="cuda"), None)
+    w = ja.powerlaw_amplitude_grad(64, want[3], slope=-3.5, fluctuations=0.8)
+    g = ja.powerlaw_amplitude_grad(64, got[3], slope=-3.5, fluctuations=0.8)
+    assert g[0] == pytest.approx(w[0], rel=1e-10)
+    assert g[1] == pytest.approx(w[1], rel=1e-10)

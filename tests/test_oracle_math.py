@@ -259,3 +259,35 @@ def test_metric_is_gauss_newton_of_the_response():
                for p, q, l0, mod in zip(lam_p, lam_m, lam0, prob.modules))
     vMv = np.vdot(Mv[0], v[0]) + np.vdot(Mv[1], v[1])
     assert vMv == pytest.approx(quad, rel=1e-6)
+
+
+@pytest.mark.parametrize("slope,fluct,E", [(-4.0, 1.0, 1), (-3.0, 0.7, 3)])
+def test_powerlaw_hyperparameter_gradients(ja, slope, fluct, E):
+    """powerlaw_amplitude_grad chained to the oracle's amplitude-field
+    gradient against central differences of the NLL in the power-law
+    slope and fluctuations (the amplitude field rebuilt each time)."""
+    from dataclasses import replace
+    N = 32
+    prob, xi = make_problem(H=32, n=4, margin=4, E=E, M=1, seed=41)
+
+    def with_hyper(s, f):
+        a = ja.powerlaw_amplitude(N, slope=s, fluctuations=f)
+        return replace(prob, amp_ref=a, amp_alpha=None if prob.amp_alpha is None else 0.5 * a)
+
+    p0 = with_hyper(slope, fluct)
+    _, _, _, gar, gaa, _, _ = om.nll_and_grad_hyper(p0, *xi)
+    # amp_alpha = amp_ref / 2 shares the hyperparameters: its gradient adds at half weight
+    g_amp = gar if gaa is None else gar + 0.5 * gaa
+    g_s, g_f = ja.powerlaw_amplitude_grad(N, g_amp, slope=slope, fluctuations=fluct)
+    h = 1e-5
+    fd_s = (om.nll(with_hyper(slope + h, fluct), *xi) -
+            om.nll(with_hyper(slope - h, fluct), *xi)) / (2 * h)
+    fd_f = (om.nll(with_hyper(slope, fluct + h), *xi) -
+            om.nll(with_hyper(slope, fluct - h), *xi)) / (2 * h)
+    assert g_s == pytest.approx(fd_s, rel=1e-6)
+    assert g_f == pytest.approx(fd_f, rel=1e-6)
+    # closed form of the fluctuations gradient: d amp / d f = amp / f
+    amp = ja.powerlaw_amplitude(N, slope=slope, fluctuations=fluct)
+    assert g_f == pytest.approx(float(np.sum(g_amp * amp)) / fluct, rel=1e-12)
+    with pytest.raises(ValueError):
+        ja.powerlaw_amplitude_grad(N, np.zeros((N, N + 1)))
