@@ -105,6 +105,35 @@ def test_grad_desc_layout_matches_the_header(ja, tmp_path):
     assert got[1:] == [getattr(_lib.GradDesc, f).offset for f in fields]
 
 
+def test_last_error_is_per_thread(ja):
+    """jubik_last_error reports the calling thread's failure even when
+    another thread failed in between."""
+    import threading
+    from jubik_amd import _lib
+    lib = _lib.load()
+    barrier = threading.Barrier(2)
+    seen = {}
+
+    def worker(name, margin):
+        plan = ctypes.c_void_p()
+        if name == "b":
+            barrier.wait()  # fail after thread a
+        st = lib.jubik_linpatch_create(ctypes.byref(plan), 0, 64, 64, 1, 1, 4, margin, 1, 1)
+        if name == "a":
+            barrier.wait()
+        barrier.wait()  # both have failed
+        seen[name] = (st, lib.jubik_last_error())
+
+    ts = [threading.Thread(target=worker, args=("a", 0)),
+          threading.Thread(target=worker, args=("b", 9))]
+    for t in ts:
+        t.start()
+    for t in ts:
+        t.join(timeout=30)
+    assert seen["a"][0] == _lib.EINVAL and b"margin" in seen["a"][1]
+    assert seen["b"][0] == _lib.EUNSUPPORTED and b"FFT length 50" in seen["b"][1]
+
+
 def test_tile_cache_round_trip(ja, tmp_path):
     import json
     from jubik_amd.convolve import tile_cache_get, tile_cache_put
