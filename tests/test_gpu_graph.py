@@ -159,3 +159,30 @@ def test_batch_graph_rejects_shared_workspaces(ja):
         ja.capture_grad_batch([(m0, 0), (twin, 0)])
     with pytest.raises(ValueError):
         ja.capture_grad_batch([(models[1][2], 3)])  # above max_samples
+
+
+def test_two_models_on_two_streams_match_serial_results(ja):
+    """Entry points run on the caller's stream: two models evaluated on two
+    side streams at once give the results of serial evaluation, bit for bit."""
+    models = _uneven_models(ja, torch.float32)
+    (pa, xa, ma), (pb, xb, mb) = models[0], models[2]
+    want_a = ma.nll_and_grad(xa[0], xa[1])
+    want_b = mb.nll_and_grad(xb[0], xb[1])
+    torch.cuda.synchronize()
+    dev = lambda a: torch.as_tensor(a, dtype=torch.float32, device="cuda")
+    ins_a, ins_b = (dev(xa[0]), dev(xa[1])), (dev(xb[0]), dev(xb[1]))
+    outs_a, outs_b = ma._grads(), mb._grads()
+    s1, s2 = torch.cuda.Stream(), torch.cuda.Stream()
+    s1.wait_stream(torch.cuda.current_stream())
+    s2.wait_stream(torch.cuda.current_stream())
+    for _ in range(3):
+        with torch.cuda.stream(s1):
+            na, ga_r, ga_a = ma.nll_and_grad_device(*ins_a, out=outs_a)
+        with torch.cuda.stream(s2):
+            nb, gb_r, gb_a = mb.nll_and_grad_device(*ins_b, out=outs_b)
+        torch.cuda.synchronize()
+        assert float(na.item()) == want_a[0] and float(nb.item()) == want_b[0]
+        assert np.array_equal(ga_r.cpu().numpy(), want_a[1])
+        assert np.array_equal(ga_a.cpu().numpy(), want_a[2])
+        assert np.array_equal(gb_r.cpu().numpy(), want_b[1])
+        assert np.array_equal(gb_a.cpu().numpy(), want_b[2])
