@@ -47,10 +47,10 @@ def parse():
                     help="only run the mixed gradient steps (for rocprofv3)")
     ap.add_argument("--samples", type=int, default=8,
                     help="VI samples per call of the batched gradient leg (0: skip)")
-    ap.add_argument("--tile", default="auto", choices=["auto", "1", "2", "4"],
-                    help="column-tile divisor of the column passes: the autotuner's pick "
-                         "(auto) or a fixed one (profiling runs repeat the pick without "
-                         "the tuner's own launches)")
+    ap.add_argument("--tile", default="auto",
+                    help="tile divisors of the column and row passes: the autotuner's pick "
+                         "(auto) or fixed ones, 'C' or 'C,R' with C, R in {1, 2, 4} "
+                         "(profiling runs repeat the pick without the tuner's own launches)")
     ap.add_argument("--sweep", action="store_true",
                     help="time every benchmark configuration instead of the 1024^2 x 4E line")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
@@ -306,13 +306,14 @@ def sweep(args):
                    torch, lambda: model.nll_and_grad_device(xr_t, xa_t, out=out), steps, 2)}
         # column tile picked by the autotuner for single evaluations
         tune = model.autotune(samples=1, reps=10)
-        rec.update({"tile_ms": tune["ms"], "col_div": tune["col_div"],
+        rec.update({"tile_ms": tune["ms"], "col_div": tune["col_div"], "row_div": tune["row_div"],
                     "grad_eval_ms_tuned": 1e3 * time_steps(
                         torch, lambda: model.nll_and_grad_device(xr_t, xa_t, out=out), steps, 2)})
         if c["S"]:
             S = c["S"]
             tune = model.autotune(samples=S, reps=3)
-            rec.update({"tile_ms_batch": tune["ms"], "col_div_batch": tune["col_div"]})
+            rec.update({"tile_ms_batch": tune["ms"], "col_div_batch": tune["col_div"],
+                        "row_div_batch": tune["row_div"]})
             model.nll_and_grad_device(xr_t, xa_t, out=out)
             v = torch.randn((S // 2, H, H), device="cuda", dtype=torch.float32)
             vr = torch.cat([v, -v])
@@ -336,7 +337,7 @@ def sweep(args):
         print(json.dumps(rec), flush=True)
         if name.startswith("grid_"):
             # kept for the uneven-grid graph below, with its single-evaluation tile
-            model.set_tile(rec["col_div"])
+            model.set_tile(rec["col_div"], rec["row_div"])
             uneven.append((name, model, xr_t, xa_t))
         del model, sky, psf
         torch.cuda.empty_cache()
@@ -430,12 +431,13 @@ def main():
     inp = build_inputs(H, E, n, m)
     counts = draw_counts(ja, torch, inp)
     mixed = make_model(ja, torch, inp, torch.float32, counts)
-    # column tile of the column passes picked by the autotuner (set-up)
-    if args.tile == "auto":
+    # tiles of the column and row passes picked by the autotuner (set-up)
+    fixed = None if args.tile == "auto" else tuple(int(v) for v in args.tile.split(","))
+    if fixed is None:
         tune = mixed.autotune(samples=1, reps=10)
     else:
-        mixed.set_tile(int(args.tile))
-        tune = {"col_div": int(args.tile), "ms": None}
+        mixed.set_tile(*fixed)
+        tune = {"col_div": fixed[0], "row_div": fixed[1] if len(fixed) > 1 else 1, "ms": None}
     xi32 = [torch.as_tensor(v, dtype=torch.float32, device="cuda") for v in inp["xi"]]
     out32 = mixed._grads()
     step = lambda: mixed.nll_and_grad_device(*xi32, out=out32)
@@ -470,17 +472,17 @@ def main():
     if args.samples > 1:
         S = args.samples
         bm = make_model(ja, torch, inp, torch.float32, counts, max_samples=S)
-        if args.tile == "auto":
+        if fixed is None:
             tune_b = bm.autotune(samples=S, reps=3)
         else:
-            bm.set_tile(int(args.tile))
-            tune_b = {"col_div": int(args.tile)}
+            bm.set_tile(*fixed)
+            tune_b = {"col_div": fixed[0], "row_div": fixed[1] if len(fixed) > 1 else 1}
         brng = np.random.default_rng(17)
         xb = [torch.as_tensor(v[None] + 0.05 * brng.normal(size=(S,) + v.shape),
                               dtype=torch.float32, device="cuda") for v in inp["xi"]]
         t_b = time_steps(torch, lambda: bm.nll_and_grad_batch(*xb), max(3, args.steps // S), 2)
         batched = {"samples": S, "ms_per_call": t_b * 1e3, "evals_per_s": S / t_b,
-                   "column_tile_divisor": tune_b["col_div"]}
+                   "tile_divisors": [tune_b["col_div"], tune_b["row_div"]]}
         del bm, xb
         torch.cuda.empty_cache()
 
@@ -575,7 +577,7 @@ def main():
                                f"margin={m} (L={2 * (H // n) + 2 * m}), Gaussian+Moffat PSFs, "
                                "exposure 5e2-2e3 s, 5% mask, Poisson NLL (FP64 reduction)",
                    "precision": "FP32 FFTs/PSF products/exp, FP64 NLL reductions"},
-        "autotune": {"column_tile_divisor": tune["col_div"],
+        "autotune": {"column_tile_divisor": tune["col_div"], "row_tile_divisor": tune["row_div"],
                      "ms_per_forward_adjoint": tune["ms"]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK,

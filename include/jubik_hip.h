@@ -84,17 +84,19 @@ int jubik_linpatch_adjoint(jubik_linpatch* plan, const void* gy, void* gx, doubl
                            void* stream);
 /* Tile autotuning (no counterpart in the reference, whose FFTs XLA plans):
  * the forward and adjoint column passes run V / col_div columns per
- * work-group, col_div = 1, 2 or 4 (a divisor is compiled in when V / col_div
- * >= 2).  Results agree to rounding across tiles (the same butterflies run
- * per column; multiply-adds contract differently per tile).
- * jubik_linpatch_autotune times one forward + adjoint per divisor (reps
- * repetitions, zero inputs, after the kernel is set), keeps the fastest in
- * the plan and returns it; ms[0..2] receive the milliseconds of divisors 1,
- * 2, 4 (-1 where not compiled in).  It allocates and synchronises: call it
- * at set-up. */
-int jubik_linpatch_set_tile(jubik_linpatch* plan, int col_div);
-int jubik_linpatch_autotune(jubik_linpatch* plan, int reps, int* col_div, double* ms,
-                            void* stream);
+ * work-group and the signal and cotangent row passes VR / row_div row pairs,
+ * col_div and row_div = 1, 2 or 4 (a divisor is compiled in when V / col_div
+ * >= 2, VR / row_div >= 1).  Results agree to rounding across tiles (the
+ * same butterflies run per column or row; multiply-adds contract differently
+ * per tile).  jubik_linpatch_autotune times one forward + adjoint per column
+ * tile (row tile 1, reps repetitions, zero inputs, after the kernel is set),
+ * then per row tile with the fastest column tile, keeps the fastest pair in
+ * the plan and returns it; ms[0..2] receive the milliseconds of column
+ * divisors 1, 2, 4 and ms[3..5] those of row divisors 1, 2, 4 (-1 where not
+ * compiled in).  It allocates and synchronises: call it at set-up. */
+int jubik_linpatch_set_tile(jubik_linpatch* plan, int col_div, int row_div);
+int jubik_linpatch_autotune(jubik_linpatch* plan, int reps, int* col_div, int* row_div,
+                            double* ms, void* stream);
 
 /* ---------------------------------------------------------------- primitives
  * slice_patches of jubik/convolve.py:28-63 on x (B, H, W): out (n^2, B,
@@ -254,12 +256,12 @@ int jubik_model_grad_batch(jubik_model* model, int64_t S, const void* amp_ref,
 int jubik_model_metric_batch(jubik_model* model, int64_t S, const void* amp_ref,
                              const void* amp_alpha, const void* dnu, const void* v_ref,
                              const void* v_alpha, void* out_ref, void* out_alpha, void* stream);
-/* The column tile of the model's own evaluations (S samples per pass), as
- * for jubik_linpatch_autotune; col_div = 0 in jubik_model_set_tile follows
+/* The tiles of the model's own evaluations (S samples per pass), as for
+ * jubik_linpatch_autotune; a divisor of 0 in jubik_model_set_tile follows
  * the plan's tile again. */
-int jubik_model_set_tile(jubik_model* model, int col_div);
-int jubik_model_autotune(jubik_model* model, int64_t S, int reps, int* col_div, double* ms,
-                         void* stream);
+int jubik_model_set_tile(jubik_model* model, int col_div, int row_div);
+int jubik_model_autotune(jubik_model* model, int64_t S, int reps, int* col_div, int* row_div,
+                         double* ms, void* stream);
 /* Graph capture and launch (no counterpart in the reference, which relies
  * on XLA's compilation of the jitted likelihood): the gradient evaluation of
  * jubik_model_grad (S = 0) or jubik_model_grad_batch (S >= 1) with these

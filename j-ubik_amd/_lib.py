@@ -50,10 +50,10 @@ SIGNATURES = {
     "jubik_model_grad_hyper": [_vp, _vp, _vp, _vp, _vp, _dbl, _dbl, _vp, _vp, _vp, _vp, _vp, _vp,
                                _vp, _vp],
     "jubik_model_metric": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
-    "jubik_linpatch_set_tile": [_vp, _int],
-    "jubik_linpatch_autotune": [_vp, _int, C.POINTER(_int), _dblp, _vp],
-    "jubik_model_set_tile": [_vp, _int],
-    "jubik_model_autotune": [_vp, _i64, _int, C.POINTER(_int), _dblp, _vp],
+    "jubik_linpatch_set_tile": [_vp, _int, _int],
+    "jubik_linpatch_autotune": [_vp, _int, C.POINTER(_int), C.POINTER(_int), _dblp, _vp],
+    "jubik_model_set_tile": [_vp, _int, _int],
+    "jubik_model_autotune": [_vp, _i64, _int, C.POINTER(_int), C.POINTER(_int), _dblp, _vp],
     "jubik_model_create_batched": [C.POINTER(_vp), _vp, _vp, _dbl, _i64],
     "jubik_model_grad_batch": [_vp, _i64, _vp, _vp, _vp, _vp, _dbl, _dbl, _vp, _vp, _vp, _vp,
                                _vp],

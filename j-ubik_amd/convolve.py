@@ -60,7 +60,7 @@ class LinpatchOperator:
         self.dx, self.window, self.L, self.P, self.pad, self.J = (int(v) for v in info[:6])
         self.workspace_bytes, self.spectra_bytes = int(info[6]), int(info[7])
         self._kernel_ref = None
-        self.col_div = 1
+        self.col_div, self.row_div = 1, 1
 
     def __del__(self):
         plan = getattr(self, "_plan", None)
@@ -129,33 +129,41 @@ class LinpatchOperator:
         return (f"{arch}/{dt}/L{self.L}/{self.H}x{self.W}/E{self.E}/M{self.M}/P{self.P}"
                 f"/S{int(samples)}/xs{int(self.x_shared)}/ke{int(self.kernel_per_energy)}")
 
-    def set_tile(self, col_div):
-        """Column-pass tile divisor: 1, 2 or 4 (results agree to rounding)."""
-        _lib.call("jubik_linpatch_set_tile", self._plan, int(col_div))
-        self.col_div = int(col_div)
+    def set_tile(self, col_div, row_div=1):
+        """Column- and row-pass tile divisors: 1, 2 or 4 each (results agree
+        to rounding)."""
+        _lib.call("jubik_linpatch_set_tile", self._plan, int(col_div), int(row_div))
+        self.col_div, self.row_div = int(col_div), int(row_div)
         return self
 
     def autotune(self, reps=20, cache=None):
-        """Pick the fastest column-pass tile for this geometry by timing one
-        forward + adjoint per candidate (the kernel must be set).  ``cache``:
-        a JSON file of earlier choices, read first and updated."""
+        """Pick the fastest column- and row-pass tiles for this geometry by
+        timing one forward + adjoint per candidate (the kernel must be set).
+        ``cache``: a JSON file of earlier choices, read first and updated."""
         key = self.tune_key()
         hit = tile_cache_get(cache, key)
         if hit is not None:
-            self.set_tile(hit)
-            return {"col_div": hit, "ms": None, "cached": True}
-        best = ctypes.c_int()
-        ms = (ctypes.c_double * 3)()
-        _lib.call("jubik_linpatch_autotune", self._plan, int(reps), ctypes.byref(best), ms,
-                  dev.stream())
-        self.col_div = int(best.value)
-        tile_cache_put(cache, key, self.col_div)
-        return {"col_div": self.col_div, "cached": False,
-                "ms": {d: t for d, t in zip((1, 2, 4), ms) if t >= 0}}
+            self.set_tile(*hit)
+            return {"col_div": hit[0], "row_div": hit[1], "ms": None, "cached": True}
+        col, row = ctypes.c_int(), ctypes.c_int()
+        ms = (ctypes.c_double * 6)()
+        _lib.call("jubik_linpatch_autotune", self._plan, int(reps), ctypes.byref(col),
+                  ctypes.byref(row), ms, dev.stream())
+        self.col_div, self.row_div = int(col.value), int(row.value)
+        tile_cache_put(cache, key, (self.col_div, self.row_div))
+        return tune_result(self.col_div, self.row_div, ms)
+
+
+def tune_result(col_div, row_div, ms):
+    """The tuner's pick and its timings (ms per forward + adjoint) by divisor."""
+    return {"col_div": col_div, "row_div": row_div, "cached": False,
+            "ms": {"col": {d: t for d, t in zip((1, 2, 4), ms[:3]) if t >= 0},
+                   "row": {d: t for d, t in zip((1, 2, 4), ms[3:6]) if t >= 0}}}
 
 
 def tile_cache_get(cache, key):
-    """Tile divisor stored under ``key`` in the JSON file ``cache``, or None."""
+    """(col_div, row_div) stored under ``key`` in the JSON file ``cache``,
+    or None."""
     if cache is None:
         return None
     try:
@@ -163,11 +171,11 @@ def tile_cache_get(cache, key):
     except FileNotFoundError:
         return None
     hit = table.get(key)
-    return None if hit is None else int(hit)
+    return None if hit is None else (int(hit[0]), int(hit[1]))
 
 
-def tile_cache_put(cache, key, col_div):
-    """Store a tile divisor under ``key`` in the JSON file ``cache``."""
+def tile_cache_put(cache, key, tiles):
+    """Store (col_div, row_div) under ``key`` in the JSON file ``cache``."""
     if cache is None:
         return
     path = Path(cache)
@@ -175,7 +183,7 @@ def tile_cache_put(cache, key, col_div):
         table = json.loads(path.read_text())
     except FileNotFoundError:
         table = {}
-    table[key] = int(col_div)
+    table[key] = [int(tiles[0]), int(tiles[1])]
     tmp = path.with_suffix(path.suffix + ".tmp")
     tmp.write_text(json.dumps(table, indent=1, sort_keys=True) + "\n")
     tmp.replace(path)

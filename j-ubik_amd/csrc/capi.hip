@@ -146,8 +146,9 @@ struct jubik_linpatch {
   void* KB = nullptr;
   size_t ws_bytes = 0, spec_bytes = 0;
   bool ready = false;
-  int col_div = 1;  // column-pass tile divisor (set by the autotuner)
-  LinBuffers bufs() const { return LinBuffers{tw, tent, KT, R1, R2, KB, col_div}; }
+  int col_div = 1;  // column- and row-pass tile divisors (set by the autotuner)
+  int row_div = 1;
+  LinBuffers bufs() const { return LinBuffers{tw, tent, KT, R1, R2, KB, col_div, row_div}; }
   void release() {
     dev_free(tw);
     dev_free(tent);
@@ -194,7 +195,8 @@ struct jubik_model {
   void* R2 = nullptr;
   void* gwork = nullptr;    // own GP workspace (max_samples > 1)
   bool have_point = false;
-  int col_div = 0;  // column-pass tile divisor; 0 = the plan's
+  int col_div = 0;  // column- and row-pass tile divisors; 0 = the plan's
+  int row_div = 0;
   void release() {
     dev_free(sky);
     dev_free(dsky);
@@ -211,6 +213,7 @@ struct jubik_model {
     if (R1) b.R1 = R1;
     if (R2) b.R2 = R2;
     if (col_div) b.col_div = col_div;
+    if (row_div) b.row_div = row_div;
     return b;
   }
   GpBuffers gbufs() const {
@@ -421,12 +424,14 @@ int jubik_linpatch_adjoint(jubik_linpatch* plan, const void* gy, void* gx, doubl
   return linpatch_adjoint(plan->dtype, plan->g, plan->bufs(), gy, gx, scale, as_stream(stream));
 }
 
-// Times one linpatch forward + adjoint (S samples, the given workspaces) for
-// every column-tile divisor compiled in for L, on zero inputs, and returns
-// the fastest divisor; ms[0..2] receive the times of divisors 1, 2, 4
-// (-1 where not compiled in).  Set-up work: allocates and synchronises.
-static int autotune_cols(int dtype, const LinGeom& g0, LinBuffers b, int S, int reps,
-                         int* best, double* ms, hipStream_t s) {
+// Times one linpatch forward + adjoint (S samples, the given workspaces) on
+// zero inputs for every column-tile divisor compiled in for L (row tile 1),
+// then for every row-tile divisor with the fastest column tile, and returns
+// the fastest pair; ms[0..2] receive the times of column divisors 1, 2, 4
+// and ms[3..5] those of row divisors 1, 2, 4 (-1 where not compiled in).
+// Set-up work: allocates and synchronises.
+static int autotune_tiles(int dtype, const LinGeom& g0, LinBuffers b, int S, int reps,
+                          int* best_col, int* best_row, double* ms, hipStream_t s) {
   LinGeom g = g0;
   g.S = S;
   const size_t bytes = (size_t)S * g.J * g.H * g.W * elem_size(dtype);
@@ -439,32 +444,49 @@ static int autotune_cols(int dtype, const LinGeom& g0, LinBuffers b, int S, int 
   if (!st && (hipMemsetAsync(x, 0, bytes, s) != hipSuccess ||
               hipMemsetAsync(y, 0, bytes, s) != hipSuccess))
     st = fail(kRuntimeError, "clearing autotune buffers");
-  double best_ms = -1.0;
-  *best = 1;
-  const int divs[3] = {1, 2, 4};
-  for (int k = 0; k < 3 && !st; ++k) {
-    ms[k] = -1.0;
-    if (!linpatch_col_tile(dtype, g.L, divs[k])) continue;
-    b.col_div = divs[k];
+  auto time_tiles = [&](double& out) {
     auto once = [&]() {
       int r = linpatch_forward(dtype, g, b, x, y, 1.0, s);
       if (!r) r = linpatch_adjoint(dtype, g, b, y, x, 1.0, s);
       return r;
     };
-    for (int w = 0; w < 2 && !st; ++w) st = once();
-    if (st || hipEventRecord(e0, s) != hipSuccess) break;
-    for (int r = 0; r < reps && !st; ++r) st = once();
-    if (st) break;
+    int r = 0;
+    for (int w = 0; w < 2 && !r; ++w) r = once();
+    if (r) return r;
+    if (hipEventRecord(e0, s) != hipSuccess) return fail(kRuntimeError, "timing a tile");
+    for (int k = 0; k < reps && !r; ++k) r = once();
+    if (r) return r;
     float t = 0.f;
     if (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
-        hipEventElapsedTime(&t, e0, e1) != hipSuccess) {
-      st = fail(kRuntimeError, "timing a column tile");
-      break;
-    }
-    ms[k] = (double)t / reps;
-    if (best_ms < 0.0 || ms[k] < best_ms) {
+        hipEventElapsedTime(&t, e0, e1) != hipSuccess)
+      return fail(kRuntimeError, "timing a tile");
+    out = (double)t / reps;
+    return 0;
+  };
+  const int divs[3] = {1, 2, 4};
+  *best_col = 1;
+  *best_row = 1;
+  for (int k = 0; k < 6; ++k) ms[k] = -1.0;
+  double best_ms = -1.0;
+  b.row_div = 1;
+  for (int k = 0; k < 3 && !st; ++k) {
+    if (!linpatch_col_tile(dtype, g.L, divs[k])) continue;
+    b.col_div = divs[k];
+    st = time_tiles(ms[k]);
+    if (!st && (best_ms < 0.0 || ms[k] < best_ms)) {
       best_ms = ms[k];
-      *best = divs[k];
+      *best_col = divs[k];
+    }
+  }
+  b.col_div = *best_col;
+  best_ms = -1.0;
+  for (int k = 0; k < 3 && !st; ++k) {
+    if (!linpatch_row_tile(dtype, g.L, divs[k])) continue;
+    b.row_div = divs[k];
+    st = time_tiles(ms[3 + k]);
+    if (!st && (best_ms < 0.0 || ms[3 + k] < best_ms)) {
+      best_ms = ms[3 + k];
+      *best_row = divs[k];
     }
   }
   if (e0) (void)hipEventDestroy(e0);
@@ -475,27 +497,42 @@ static int autotune_cols(int dtype, const LinGeom& g0, LinBuffers b, int S, int 
   return st;
 }
 
-int jubik_linpatch_set_tile(jubik_linpatch* plan, int col_div) {
-  if (!plan) return fail(kInvalidArgument, "NULL argument");
-  if (!linpatch_col_tile(plan->dtype, plan->g.L, col_div))
+// 0 if (col_div, row_div) is compiled in for the plan's length; col_div = 0
+// or row_div = 0 is accepted where `follow` allows it (the plan's value).
+static int check_tiles(const jubik_linpatch* plan, int col_div, int row_div, bool follow) {
+  const int L = plan->g.L;
+  if (!(follow && col_div == 0) && !linpatch_col_tile(plan->dtype, L, col_div))
     return fail(kInvalidArgument, "column tile divisor " + std::to_string(col_div) +
-                                      " is not compiled in for L = " + std::to_string(plan->g.L));
-  plan->col_div = col_div;
+                                      " is not compiled in for L = " + std::to_string(L));
+  if (!(follow && row_div == 0) && !linpatch_row_tile(plan->dtype, L, row_div))
+    return fail(kInvalidArgument, "row tile divisor " + std::to_string(row_div) +
+                                      " is not compiled in for L = " + std::to_string(L));
   return kOk;
 }
 
-int jubik_linpatch_autotune(jubik_linpatch* plan, int reps, int* col_div, double* ms,
-                            void* stream) {
-  if (!plan || !col_div || !ms) return fail(kInvalidArgument, "NULL argument");
+int jubik_linpatch_set_tile(jubik_linpatch* plan, int col_div, int row_div) {
+  if (!plan) return fail(kInvalidArgument, "NULL argument");
+  const int st = check_tiles(plan, col_div, row_div, false);
+  if (st) return st;
+  plan->col_div = col_div;
+  plan->row_div = row_div;
+  return kOk;
+}
+
+int jubik_linpatch_autotune(jubik_linpatch* plan, int reps, int* col_div, int* row_div,
+                            double* ms, void* stream) {
+  if (!plan || !col_div || !row_div || !ms) return fail(kInvalidArgument, "NULL argument");
   if (reps < 1) return fail(kInvalidArgument, "reps must be positive");
   if (!plan->ready) return fail(kNotReady, "no kernel set (call jubik_linpatch_set_kernel)");
   DeviceGuard dg(plan->device);
-  int best = 1;
-  const int st = autotune_cols(plan->dtype, plan->g, plan->bufs(), 1, reps, &best, ms,
-                               as_stream(stream));
+  int bc = 1, br = 1;
+  const int st = autotune_tiles(plan->dtype, plan->g, plan->bufs(), 1, reps, &bc, &br, ms,
+                                as_stream(stream));
   if (st) return st;
-  plan->col_div = best;
-  *col_div = best;
+  plan->col_div = bc;
+  plan->row_div = br;
+  *col_div = bc;
+  *row_div = br;
   return kOk;
 }
 
@@ -956,19 +993,18 @@ int jubik_model_grad_batch(jubik_model* model, int64_t S, const void* amp_ref,
                     g_ref, g_alpha, as_stream(stream));
 }
 
-int jubik_model_set_tile(jubik_model* model, int col_div) {
+int jubik_model_set_tile(jubik_model* model, int col_div, int row_div) {
   if (!model || !model->lp) return fail(kInvalidArgument, "NULL argument");
-  if (col_div != 0 && !linpatch_col_tile(model->lp->dtype, model->lp->g.L, col_div))
-    return fail(kInvalidArgument, "column tile divisor " + std::to_string(col_div) +
-                                      " is not compiled in for L = " +
-                                      std::to_string(model->lp->g.L));
+  const int st = check_tiles(model->lp, col_div, row_div, true);
+  if (st) return st;
   model->col_div = col_div;
+  model->row_div = row_div;
   return kOk;
 }
 
-int jubik_model_autotune(jubik_model* model, int64_t S, int reps, int* col_div, double* ms,
-                         void* stream) {
-  if (!model || !col_div || !ms) return fail(kInvalidArgument, "NULL argument");
+int jubik_model_autotune(jubik_model* model, int64_t S, int reps, int* col_div, int* row_div,
+                         double* ms, void* stream) {
+  if (!model || !col_div || !row_div || !ms) return fail(kInvalidArgument, "NULL argument");
   if (reps < 1) return fail(kInvalidArgument, "reps must be positive");
   if (S < 1 || S > model->max_samples)
     return fail(kInvalidArgument, "batch of " + std::to_string(S) +
@@ -977,13 +1013,14 @@ int jubik_model_autotune(jubik_model* model, int64_t S, int reps, int* col_div, 
   int st = check_model(model);
   if (st) return st;
   DeviceGuard dg(model->lp->device);
-  LinBuffers b = model->lbufs();
-  int best = 1;
-  st = autotune_cols(model->lp->dtype, model->lp->g, b, (int)S, reps, &best, ms,
-                     as_stream(stream));
+  int bc = 1, br = 1;
+  st = autotune_tiles(model->lp->dtype, model->lp->g, model->lbufs(), (int)S, reps, &bc, &br, ms,
+                      as_stream(stream));
   if (st) return st;
-  model->col_div = best;
-  *col_div = best;
+  model->col_div = bc;
+  model->row_div = br;
+  *col_div = bc;
+  *row_div = br;
   return kOk;
 }
 

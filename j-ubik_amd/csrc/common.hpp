@@ -166,6 +166,7 @@ __device__ __forceinline__ T gather_load(const Gather& g, int off) {
 #ifndef JUBIK_COL_TILE_BYTES
 #define JUBIK_COL_TILE_BYTES 53248
 #endif
+#define JUBIK_ROW_TILE_BYTES 13312
 __host__ __device__ constexpr int tile_vectors(int bytes, int L, int elem) {
   const int raw = bytes / ((L | 1) * elem);
   return raw >= 32 ? 32 : raw >= 16 ? 16 : raw >= 8 ? 8 : raw >= 4 ? 4 : raw >= 2 ? 2 : 1;
@@ -180,7 +181,7 @@ struct Tile {
 // (at L = 384, 4 vectors per tile measured faster than 8 or 16).
 template <typename T, int L>
 struct RowTile {
-  static constexpr int V = tile_vectors(13312, L, (int)sizeof(cplx<T>));
+  static constexpr int V = tile_vectors(JUBIK_ROW_TILE_BYTES, L, (int)sizeof(cplx<T>));
 };
 
 // Tile of work-group b out of nb: the dispatcher places work-group b on XCD

@@ -39,6 +39,12 @@ int linpatch_col_tile(int dtype, int L, int col_div) {
   return (col_div == 1 || v >= 2) ? v : 0;
 }
 
+int linpatch_row_tile(int dtype, int L, int row_div) {
+  if ((dtype != 0 && dtype != 1) || !linpatch_length_supported(L)) return 0;
+  if (row_div != 1 && row_div != 2 && row_div != 4) return 0;
+  return tile_vectors(JUBIK_ROW_TILE_BYTES, L, dtype == 0 ? 8 : 16) / row_div;
+}
+
 int linpatch_prepare(int dtype, const LinGeom& g, const LinBuffers& b, const void* kernel,
                      int layout, hipStream_t s) {
   if (dtype == 0) return lp_prepare_t<float>(g, b, kernel, layout, s);

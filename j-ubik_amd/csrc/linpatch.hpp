@@ -13,6 +13,7 @@ struct LinBuffers {
   void* R2;          // column-pass workspace, S x J x P x L x L2p
   void* KB;          // beta of the packed DC/Nyquist column, (M, P, Ek) x L
   int col_div;       // column-pass tile: 1, 2 or 4 (a full, half or quarter tile)
+  int row_div;       // row-pass tile: 1, 2 or 4
 };
 
 // FFT lengths L = 2 (H / n) + 2 margin compiled into the library.
@@ -24,6 +25,9 @@ bool linpatch_length_supported(int L);
 // column-tile divisor (1, 2 or 4), or 0 when that divisor is not compiled in
 // for L (fewer than 2 columns).
 int linpatch_col_tile(int dtype, int L, int col_div);
+// Row pairs per work-group of the signal and cotangent row passes for a
+// row-tile divisor (1, 2 or 4), or 0 when not compiled in (under one pair).
+int linpatch_row_tile(int dtype, int L, int row_div);
 
 // dtype: 0 = float32, 1 = float64.  layout: 0 = full frame (H, W), 1 = cut
 // window (w, w).  Kernels are laid out (M, P, Ek, ...).

@@ -39,12 +39,14 @@ constexpr int NT = 256;
 
 // ------------------------------------------------------------------ rows
 // grid (ceil(nrows/2 / V), P, Z); Z = jobs (signal modes) or M*Ek (kernel).
-template <typename T, int L, int MODE>
+// VRC row pairs per work-group: RowTile<T, L>::V, or a half or quarter of it
+// (chosen by the autotuner for grids with too few row tiles).
+template <typename T, int L, int MODE, int VRC = RowTile<T, L>::V>
 __global__ __launch_bounds__(NT) void lp_rows(LinGeom g, const T* __restrict__ src,
                                               const T* __restrict__ tent,
                                               const cplx<T>* __restrict__ twg,
                                               cplx<T>* __restrict__ R1, T scale, int nrows) {
-  constexpr int V = RowTile<T, L>::V;
+  constexpr int V = VRC;
   constexpr int VS = lds_stride(L, V);
   constexpr int LH = L / 2;
   constexpr int L2P = half_stride(L);
@@ -954,11 +956,14 @@ struct LinOps {
 
   static void fwd_front(const LinGeom& g, const LinBuffers& b, const void* x, double scale,
                         hipStream_t s) {
-    dim3 gr((g.w / 2 + VR - 1) / VR, g.P, g.S * g.J);
     {
       TimedLaunch _t("lp_rows_fwd", s);
-      lp_rows<T, L, ROW_SIGNAL><<<gr, NT, 0, s>>>(g, (const T*)x, (const T*)b.tent, (const C*)b.tw,
-                                                  (C*)b.R1, T(scale), g.w);
+      with_rows(b, [&](auto vr) {
+        constexpr int VRC = decltype(vr)::value;
+        dim3 gr((g.w / 2 + VRC - 1) / VRC, g.P, g.S * g.J);
+        lp_rows<T, L, ROW_SIGNAL, VRC><<<gr, NT, 0, s>>>(g, (const T*)x, (const T*)b.tent,
+                                                         (const C*)b.tw, (C*)b.R1, T(scale), g.w);
+      });
     }
     TimedLaunch _t("lp_cols_fwd", s);
     with_cols(b, [&](auto vc) {
@@ -972,11 +977,14 @@ struct LinOps {
 
   static void adj_front(const LinGeom& g, const LinBuffers& b, const void* gy, hipStream_t s,
                         NllFinal fin = NllFinal{}) {
-    dim3 gr((L / 2 + VR - 1) / VR, g.P, g.S * g.J);
     {
       TimedLaunch _t("lp_rows_adj", s);
-      lp_rows<T, L, ROW_COTANGENT><<<gr, NT, 0, s>>>(g, (const T*)gy, (const T*)b.tent,
-                                                     (const C*)b.tw, (C*)b.R1, T(1), L);
+      with_rows(b, [&](auto vr) {
+        constexpr int VRC = decltype(vr)::value;
+        dim3 gr((L / 2 + VRC - 1) / VRC, g.P, g.S * g.J);
+        lp_rows<T, L, ROW_COTANGENT, VRC><<<gr, NT, 0, s>>>(g, (const T*)gy, (const T*)b.tent,
+                                                            (const C*)b.tw, (C*)b.R1, T(1), L);
+      });
     }
     TimedLaunch _t("lp_cols_adj", s);
     with_cols(b, [&](auto vc) {
@@ -985,6 +993,17 @@ struct LinOps {
       lp_cols<T, L, COL_ADJ, VC><<<gc, NT, 0, s>>>(g, (const C*)b.R1, (const C*)b.KT,
                                                    (const C*)b.tw, (C*)b.R2, (C*)b.KB, fin);
     });
+  }
+
+  // Row tile of the signal and cotangent row passes: VR / b.row_div row
+  // pairs (row_div 1, 2 or 4; at least one pair).
+  template <class F>
+  static void with_rows(const LinBuffers& b, F&& f) {
+    if constexpr (VR / 4 >= 1)
+      if (b.row_div == 4) return f(std::integral_constant<int, VR / 4>{});
+    if constexpr (VR / 2 >= 1)
+      if (b.row_div == 2) return f(std::integral_constant<int, VR / 2>{});
+    f(std::integral_constant<int, VR>{});
   }
 
   // Column tile of the forward and adjoint column passes: V / b.col_div

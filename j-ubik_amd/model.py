@@ -65,7 +65,7 @@ class XrayModel:
                   dev.ptr(self.counts))
         self._nll = torch.empty(1, dtype=torch.float64, device="cuda")
         self.J = J
-        self.col_div = 0
+        self.col_div, self.row_div = 0, 0
 
     def __del__(self):
         m = getattr(self, "_model", None)
@@ -77,31 +77,30 @@ class XrayModel:
             self._model = None
 
     def autotune(self, samples=1, reps=10, cache=None):
-        """Pick the fastest column-pass tile for passes of ``samples``
-        positions (1: single evaluations) by timing one PSF forward +
-        adjoint per candidate; the model then uses it for all its passes.
-        ``cache``: a JSON file of earlier choices, read first and updated."""
-        from .convolve import tile_cache_get, tile_cache_put
+        """Pick the fastest column- and row-pass tiles for passes of
+        ``samples`` positions (1: single evaluations) by timing one PSF
+        forward + adjoint per candidate; the model then uses them for all its
+        passes.  ``cache``: a JSON file of earlier choices, read first and
+        updated."""
+        from .convolve import tile_cache_get, tile_cache_put, tune_result
         key = "model/" + self.psf.tune_key(samples)
         hit = tile_cache_get(cache, key)
         if hit is not None:
-            _lib.call("jubik_model_set_tile", self._model, int(hit))
-            self.col_div = hit
-            return {"col_div": hit, "ms": None, "cached": True}
-        best = ctypes.c_int()
-        ms = (ctypes.c_double * 3)()
+            self.set_tile(*hit)
+            return {"col_div": hit[0], "row_div": hit[1], "ms": None, "cached": True}
+        col, row = ctypes.c_int(), ctypes.c_int()
+        ms = (ctypes.c_double * 6)()
         _lib.call("jubik_model_autotune", self._model, int(samples), int(reps),
-                  ctypes.byref(best), ms, dev.stream())
-        self.col_div = int(best.value)
-        tile_cache_put(cache, key, self.col_div)
-        return {"col_div": self.col_div, "cached": False,
-                "ms": {d: t for d, t in zip((1, 2, 4), ms) if t >= 0}}
+                  ctypes.byref(col), ctypes.byref(row), ms, dev.stream())
+        self.col_div, self.row_div = int(col.value), int(row.value)
+        tile_cache_put(cache, key, (self.col_div, self.row_div))
+        return tune_result(self.col_div, self.row_div, ms)
 
-    def set_tile(self, col_div):
-        """Column-pass tile divisor of the model's passes (0: the PSF
-        operator's); results agree to rounding."""
-        _lib.call("jubik_model_set_tile", self._model, int(col_div))
-        self.col_div = int(col_div)
+    def set_tile(self, col_div, row_div=0):
+        """Column- and row-pass tile divisors of the model's passes (0: the
+        PSF operator's); results agree to rounding."""
+        _lib.call("jubik_model_set_tile", self._model, int(col_div), int(row_div))
+        self.col_div, self.row_div = int(col_div), int(row_div)
         return self
 
     def _grads(self):

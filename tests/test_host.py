@@ -65,12 +65,14 @@ def test_invalid_arguments_fail_before_device_work(ja):
 def test_tuning_and_batch_graph_entry_points_validate_arguments(ja):
     from jubik_amd import _lib
     lib = _lib.load()
-    best = ctypes.c_int()
-    ms = (ctypes.c_double * 3)()
-    assert lib.jubik_linpatch_set_tile(None, 1) == _lib.EINVAL
-    assert lib.jubik_linpatch_autotune(None, 5, ctypes.byref(best), ms, None) == _lib.EINVAL
-    assert lib.jubik_model_set_tile(None, 0) == _lib.EINVAL
-    assert lib.jubik_model_autotune(None, 1, 5, ctypes.byref(best), ms, None) == _lib.EINVAL
+    col, row = ctypes.c_int(), ctypes.c_int()
+    ms = (ctypes.c_double * 6)()
+    assert lib.jubik_linpatch_set_tile(None, 1, 1) == _lib.EINVAL
+    assert lib.jubik_linpatch_autotune(None, 5, ctypes.byref(col), ctypes.byref(row), ms,
+                                       None) == _lib.EINVAL
+    assert lib.jubik_model_set_tile(None, 0, 0) == _lib.EINVAL
+    assert lib.jubik_model_autotune(None, 1, 5, ctypes.byref(col), ctypes.byref(row), ms,
+                                    None) == _lib.EINVAL
     graph = ctypes.c_void_p()
     assert lib.jubik_graph_capture_batch(ctypes.byref(graph), None, 1) == _lib.EINVAL
     descs = (_lib.GradDesc * 1)()
@@ -140,15 +142,15 @@ def test_tile_cache_round_trip(ja, tmp_path):
     cache = tmp_path / "tiles.json"
     assert tile_cache_get(None, "k") is None
     assert tile_cache_get(cache, "k") is None
-    tile_cache_put(None, "k", 2)  # no cache: nothing written
+    tile_cache_put(None, "k", (2, 1))  # no cache: nothing written
     assert not cache.exists()
-    tile_cache_put(cache, "gfx950/f32/L384", 2)
-    tile_cache_put(cache, "gfx950/f64/L96", 4)
-    tile_cache_put(cache, "gfx950/f32/L384", 1)
-    assert tile_cache_get(cache, "gfx950/f32/L384") == 1
-    assert tile_cache_get(cache, "gfx950/f64/L96") == 4
+    tile_cache_put(cache, "gfx950/f32/L384", (2, 1))
+    tile_cache_put(cache, "gfx950/f64/L96", (4, 2))
+    tile_cache_put(cache, "gfx950/f32/L384", (1, 4))
+    assert tile_cache_get(cache, "gfx950/f32/L384") == (1, 4)
+    assert tile_cache_get(cache, "gfx950/f64/L96") == (4, 2)
     assert tile_cache_get(cache, "other") is None
-    assert json.loads(cache.read_text()) == {"gfx950/f32/L384": 1, "gfx950/f64/L96": 4}
+    assert json.loads(cache.read_text()) == {"gfx950/f32/L384": [1, 4], "gfx950/f64/L96": [4, 2]}
 
 
 def test_python_api_errors_match_reference(ja):
