@@ -302,16 +302,10 @@ class GradGraph:
 
     def __init__(self, model: XrayModel, samples):
         s = model.sky
-        if not 0 <= samples <= model.max_samples:
-            raise ValueError(f"samples must be in [0, max_samples={model.max_samples}]")
-        lead = (samples,) if samples else ()
-        shape = lead + (s.N, s.N)
+        slot = _GradSlot(model, samples)
         self.model = model
-        self.xi_ref = torch.zeros(shape, dtype=model.dtype, device="cuda")
-        self.xi_alpha = torch.zeros(shape, dtype=model.dtype, device="cuda") if s.has_alpha else None
-        self.g_ref = torch.empty(shape, dtype=model.dtype, device="cuda")
-        self.g_alpha = torch.empty(shape, dtype=model.dtype, device="cuda") if s.has_alpha else None
-        self.nll = torch.empty(max(samples, 1), dtype=torch.float64, device="cuda")
+        self.xi_ref, self.xi_alpha = slot.xi_ref, slot.xi_alpha
+        self.g_ref, self.g_alpha, self.nll = slot.g_ref, slot.g_alpha, slot.nll
         self._graph = ctypes.c_void_p()
         _lib.call("jubik_model_grad_graph", ctypes.byref(self._graph), model._model, samples,
                   dev.ptr(s.amp_ref), dev.ptr(s.amp_alpha), dev.ptr(self.xi_ref),
