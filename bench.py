@@ -59,10 +59,10 @@ def parse():
     return ap.parse_args()
 
 
-def build_inputs(H, E, n, m, seed=2):
-    from tests.problems import powerlaw_amp, psf_windows
+def build_inputs(ja, H, E, n, m, seed=2):
+    psf_windows = ja.synthetic.psf_windows
     rng = np.random.default_rng(seed)
-    amp_ref = powerlaw_amp(H)
+    amp_ref = ja.powerlaw_amplitude(H)
     amp_alpha = 0.5 * amp_ref
     energies = np.array([1.0, 1.6, 2.5, 4.0][:E]) if E <= 4 else np.geomspace(1, 4, E)
     dnu = np.log(energies / energies[0])
@@ -166,14 +166,21 @@ TRAFFIC_FILE = ROOT / "profiles" / "grad_eval_1024x4E_L384_hbm_traffic.txt"
 def measured_traffic(kernel, H, E, n, m):
     """HBM bytes per dispatch of ``kernel`` from the committed rocprofv3 PMC
     summary (FETCH_SIZE and WRITE_SIZE, scaled by a known-size copy), for the
-    default workload only; None otherwise."""
+    default workload only; None otherwise.  ``__total__`` sums the kernels
+    of one evaluation."""
     if (H, E, n, m) != (1024, 4, 8, 64) or not TRAFFIC_FILE.exists():
         return None
+    rows = {}
     for line in TRAFFIC_FILE.read_text().splitlines():
         parts = line.split()
-        if parts and not line.startswith("#") and parts[0] == kernel:
-            return float(parts[-1])
-    return None
+        if parts and not line.startswith("#") and len(parts) > 1:
+            try:
+                rows[parts[0]] = float(parts[-1])
+            except ValueError:
+                continue
+    if kernel == "__total__":
+        return sum(v for k, v in rows.items() if k in kernel_bytes(H, E, n, m)) or None
+    return rows.get(kernel)
 
 
 def eval_bytes(H, E, n, m, M=1):
@@ -191,6 +198,17 @@ def fft_flops(H, E, n, m, M=1):
     t = 2.5 * L * L * np.log2(L * L)
     gp = 2 * 2 * 2.5 * H * H * np.log2(H * H)
     return pe * (4 * t + 2 * 6 * L * (L // 2 + 1)) + gp
+
+
+def cpu_model():
+    """The host CPU's model name (first /proc/cpuinfo entry)."""
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
 
 
 def cpu_baseline(inp, counts, budget):
@@ -216,6 +234,7 @@ def cpu_baseline(inp, counts, budget):
     timed = times[1:] if len(times) > 1 else times
     med = statistics.median(timed)
     return dict(value=1.0 / med, unit="grad evals/s", cores=ocv._WORKERS, kind="port",
+                cpu_model=cpu_model(),
                 sample=f"{len(timed)} full gradient evaluations of the same workload "
                        f"(float64 numpy/scipy oracle, scipy.fft workers={ocv._WORKERS}), "
                        f"median {med:.2f} s"), result
@@ -252,8 +271,7 @@ def sweep(args):
     import torch
     import __graft_entry__ as ge
     ja = ge.load_package()
-    from tests.problems import powerlaw_amp, psf_windows
-    from oracle import model as om
+    powerlaw_amp, psf_windows = ja.powerlaw_amplitude, ja.synthetic.psf_windows
     uneven = []
     for name, c in CONFIGS.items():
         H, E, M, n, m = c["H"], c["E"], c["M"], c["n"], c["m"]
@@ -280,16 +298,24 @@ def sweep(args):
             cnt = counts if counts is not None else np.zeros(keep.shape, np.int32)
             return sky, psf, ja.XrayModel(sky, psf, 1.0, expo, cnt, max_samples=max(S, 1))
 
-        # counts from the true position (FP64 where the GP grid allows it)
-        sky64, psf64, _ = model_for(torch.float64 if H <= 1024 else torch.float32)
+        # counts from the true position (FP64 forward)
+        sky64, psf64, _ = model_for(torch.float64)
         s_true = sky64(torch.as_tensor(xt, device="cuda").to(sky64.dtype),
                        None if xat is None else torch.as_tensor(xat, device="cuda").to(sky64.dtype))
         lam = psf64.forward(s_true).reshape((M, E, H, H)) * torch.as_tensor(expo, device="cuda")
         counts = np.where(keep, ja.poisson_sample(lam, seed=c["seed"]).cpu().numpy(), -1)
         del sky64, psf64, s_true, lam
-        sky, psf, model = model_for(torch.float32, c["S"], counts.astype(np.int32))
         xi = xt + 0.1 * rng.normal(size=(H, H))
         xa = xat + 0.1 * rng.normal(size=(H, H)) if spectral else None
+        # the gradient of the FP64 build of the same kernels, for the error column
+        _, _, m64 = model_for(torch.float64, 1, counts.astype(np.int32))
+        n64, g64r, g64a = m64.nll_and_grad(torch.as_tensor(xi, device="cuda"),
+                                           None if xa is None else torch.as_tensor(xa, device="cuda"))
+        g64 = np.concatenate([g64r.cpu().numpy().ravel()] +
+                             ([g64a.cpu().numpy().ravel()] if spectral else []))
+        del m64, g64r, g64a
+        torch.cuda.empty_cache()
+        sky, psf, model = model_for(torch.float32, c["S"], counts.astype(np.int32))
         xr_t = torch.as_tensor(xi, dtype=torch.float32, device="cuda")
         xa_t = torch.as_tensor(xa, dtype=torch.float32, device="cuda") if spectral else None
         ex_t = torch.as_tensor(expo, dtype=torch.float32, device="cuda")
@@ -299,8 +325,13 @@ def sweep(args):
             return psf.forward(sky(xr_t, xa_t)).reshape(ex_t.shape) * ex_t
 
         out = model._grads()
+        n32, g32r, g32a = model.nll_and_grad(xr_t, xa_t)
+        g32 = np.concatenate([g32r.double().cpu().numpy().ravel()] +
+                             ([g32a.double().cpu().numpy().ravel()] if spectral else []))
         rec = {"config": name, "H": H, "E": E, "M": M, "n": n, "margin": m,
                "L": 2 * (H // n) + 2 * m,
+               "grad_rel_err_fp32_vs_fp64": rel(g32, g64), "grad_nmax_err_fp32_vs_fp64":
+               nmax(g32, g64), "nll_rel_err_fp32_vs_fp64": abs(n32 - n64) / abs(n64),
                "forward_ms": 1e3 * time_steps(torch, fwd, steps, 2),
                "grad_eval_ms": 1e3 * time_steps(
                    torch, lambda: model.nll_and_grad_device(xr_t, xa_t, out=out), steps, 2)}
@@ -324,7 +355,10 @@ def sweep(args):
             t_g = time_steps(torch, lambda: model.nll_and_grad_batch(xb, xab), max(3, steps // 8), 1)
             rec.update({"samples": S, "metric_batch_ms_per_product": 1e3 * t_m / S,
                         "grad_batch_ms_per_eval": 1e3 * t_g / S})
-        if not args.no_cpu and H * H * E * M <= 1024 * 1024 * 4:
+        if not args.no_cpu:
+            # one evaluation of the float64 CPU oracle: timing and the error
+            # of both GPU builds against it
+            from oracle import model as om
             prob = om.Problem(shape=(E, H, H), distances=(1.0, 1.0, 1.0), n=n, margin=m,
                               pixel_area=1.0, amp_ref=amp_ref, amp_alpha=amp_alpha, zm=-2.0,
                               mu_alpha=-1.0 if spectral else 0.0, dnu=dnu)
@@ -332,8 +366,13 @@ def sweep(args):
                                       data=counts[k][keep[k]].astype(np.float64))
                             for k in range(M)]
             t0 = time.perf_counter()
-            om.nll_and_grad(prob, xi, xa)
+            no, (gor, goa) = om.nll_and_grad(prob, xi, xa)
             rec["cpu_oracle_grad_eval_ms"] = 1e3 * (time.perf_counter() - t0)
+            go = np.concatenate([gor.ravel()] + ([goa.ravel()] if spectral else []))
+            rec.update({"grad_rel_err_fp32_vs_oracle": rel(g32, go),
+                        "grad_nmax_err_fp32_vs_oracle": nmax(g32, go),
+                        "grad_rel_err_fp64_vs_oracle": rel(g64, go),
+                        "nll_rel_err_fp32_vs_oracle": abs(n32 - no) / abs(no)})
         print(json.dumps(rec), flush=True)
         if name.startswith("grid_"):
             # kept for the uneven-grid graph below, with its single-evaluation tile
@@ -428,7 +467,7 @@ def main():
     from jubik_amd import _lib
 
     H, E, n, m = args.H, args.E, args.n, args.margin
-    inp = build_inputs(H, E, n, m)
+    inp = build_inputs(ja, H, E, n, m)
     counts = draw_counts(ja, torch, inp)
     mixed = make_model(ja, torch, inp, torch.float32, counts)
     # tiles of the column and row passes picked by the autotuner (set-up)
@@ -491,19 +530,35 @@ def main():
     g32 = np.concatenate([g32r.double().cpu().numpy().ravel(), g32a.double().cpu().numpy().ravel()])
     nll32 = float(nll32.item())
 
-    # per-kernel HIP-event timing on the launch stream (outside the timed region)
-    _lib.timing_enable(True)
+    # Per-kernel HIP-event timing on the launch stream (outside the timed
+    # region).  With every launch bracketed the per-kernel times carry the
+    # events' own cost, so they only rank the kernels; the dominant kernel is
+    # then timed alone (events around its launches only, the rest of each
+    # evaluation runs unbracketed).
     nprof = max(10, min(args.steps, 30))
+    _lib.timing_enable(True)
     for _ in range(nprof):
         step()
     torch.cuda.synchronize()
     rep = _lib.timing_report()
-    _lib.timing_enable(False)
-    kb = kernel_bytes(H, E, n, m)
-    kernels = {k: {"launches": c, "avg_us": 1e3 * tot / c} for k, (c, tot, lo, hi) in rep.items()}
+    kernels = {k: {"launches": c, "avg_us_bracketed": 1e3 * tot / c}
+               for k, (c, tot, lo, hi) in rep.items()}
     dom = max(rep.items(), key=lambda kv: kv[1][1])[0]
-    dom_avg = rep[dom][1] / rep[dom][0] * 1e-3
-    achieved = kb.get(dom, 0) / dom_avg
+    _lib.timing_select(dom)
+    for _ in range(nprof):
+        step()
+    torch.cuda.synchronize()
+    rep_dom = _lib.timing_report()
+    _lib.timing_select(None)
+    _lib.timing_enable(False)
+    dom_avg = rep_dom[dom][1] / rep_dom[dom][0] * 1e-3
+    kb = kernel_bytes(H, E, n, m)
+    # B_alg of one gradient evaluation; one launch of the dominant kernel
+    # processes all J * P patch-energies of one evaluation.
+    b_alg = eval_bytes(H, E, n, m)
+    achieved = b_alg / dom_avg
+    traffic_dom = measured_traffic(dom, H, E, n, m)
+    traffic_eval = measured_traffic("__total__", H, E, n, m)
 
     # fused kernels in FP64
     f64 = make_model(ja, torch, inp, torch.float64, counts)
@@ -519,7 +574,7 @@ def main():
 
     plain = None
     if not args.no_plain:
-        from jubik_amd.plain_fp64 import PlainFP64Model
+        from plain_fp64 import PlainFP64Model
         w = 2 * (H // n)
         kfull = torch.zeros((n * n, E, H, H), dtype=torch.float64, device="cuda")
         c = (H - w) // 2
@@ -580,15 +635,23 @@ def main():
         "autotune": {"column_tile_divisor": tune["col_div"], "row_tile_divisor": tune["row_div"],
                      "ms_per_forward_adjoint": tune["ms"]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK,
-                     "traffic": measured_traffic(dom, H, E, n, m), "kernel": dom,
-                     "kernel_avg_us": dom_avg * 1e6, "bytes_per_launch": kb.get(dom, 0),
-                     "device_copy_GBps": copy_bw / 1e9, "frac_of_copy": achieved / copy_bw},
-        "eval_roofline": {"B_alg": eval_bytes(H, E, n, m),
-                          "achieved": eval_bytes(H, E, n, m) / t_mixed,
-                          "frac_hbm": eval_bytes(H, E, n, m) / t_mixed / HBM_PEAK,
+                     "frac": achieved / HBM_PEAK, "traffic": traffic_dom, "kernel": dom,
+                     "kernel_avg_us": dom_avg * 1e6, "alg_bytes_per_launch": b_alg,
+                     "units_per_launch": f"one gradient evaluation ({E * n * n} patch-energies)",
+                     "traffic_over_alg": (traffic_dom / b_alg) if traffic_dom else None},
+        "eval_roofline": {"B_alg": b_alg, "achieved": b_alg / t_mixed,
+                          "frac": b_alg / t_mixed / HBM_PEAK,
+                          "traffic": traffic_eval,
+                          "traffic_over_alg": (traffic_eval / b_alg) if traffic_eval else None,
                           "fft_flops": fft_flops(H, E, n, m),
-                          "frac_fp32": fft_flops(H, E, n, m) / t_mixed / FP32_PEAK},
+                          "frac_fp32": fft_flops(H, E, n, m) / t_mixed / FP32_PEAK,
+                          "kernel_sum_us_bracketed": sum(k["avg_us_bracketed"]
+                                                         for k in kernels.values())},
+        "kernel_streaming": {"kernel": dom, "design_bytes_per_launch": kb.get(dom, 0),
+                             "GBps": kb.get(dom, 0) / dom_avg / 1e9,
+                             "frac_of_peak": kb.get(dom, 0) / dom_avg / HBM_PEAK,
+                             "device_copy_GBps": copy_bw / 1e9,
+                             "frac_of_copy": kb.get(dom, 0) / dom_avg / copy_bw},
         "cpu_baseline": cpu,
         "fused_fp64": {"ms_per_step": t_f64 * 1e3, "evals_per_s": 1.0 / t_f64,
                        "grad_rel_err_mixed_vs_fp64": rel(g32, g64),

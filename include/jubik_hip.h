@@ -6,8 +6,8 @@
  * Conventions
  *   - Every entry point returns 0 on success or a negative status
  *     (JUBIK_EINVAL, JUBIK_EUNSUPPORTED, JUBIK_EHIP, JUBIK_ENOMEM,
- *     JUBIK_ENOTREADY); jubik_last_error() gives the message of the calling
- *     thread's last failure.
+ *     JUBIK_ENOTREADY, JUBIK_EDOMAIN); jubik_last_error() gives the message
+ *     of the calling thread's last failure.
  *   - Array arguments are device pointers owned by the caller, C-contiguous,
  *     typed by the plan's dtype (0 = float32, 1 = float64) unless stated.
  *   - ``stream`` is a hipStream_t (NULL = the default stream).  Execution
@@ -32,6 +32,7 @@ extern "C" {
 #define JUBIK_EHIP (-3)
 #define JUBIK_ENOMEM (-4)
 #define JUBIK_ENOTREADY (-5)
+#define JUBIK_EDOMAIN (-6) /* lambda <= 0 at a read-out pixel */
 
 #define JUBIK_FLOAT32 0
 #define JUBIK_FLOAT64 1
@@ -51,6 +52,10 @@ int jubik_abi_version(void);
  * kernel, "name count total_ms min_ms max_ms", then forgets them. */
 int jubik_timing_enable(int on);
 int jubik_timing_report(char* buf, int64_t len);
+/* Restrict the timing to the kernels named ``name`` (NULL or "": every
+ * kernel), so one kernel can be timed without events between the other
+ * launches of an evaluation. */
+int jubik_timing_select(const char* name);
 
 /* ---------------------------------------------------------------- linpatch
  * Replaces jubik/convolve.py:66-145 linpatch_convolve(x, domain, kernel,
@@ -143,6 +148,13 @@ int jubik_poisson_sample(int dtype, const void* lam, int64_t n, uint64_t seed, i
  * NULL).  ``work``: JUBIK_POISSON_WORK_BYTES of device memory. */
 int jubik_poisson_nll_grad(int dtype, const void* lam, const void* data, int64_t n, double* nll,
                            void* grad, void* work, void* stream);
+/* Fisher metric of that likelihood in data space, applied to t:
+ * out = t / lam (sqrt_form = 0, nifty.re Poissonian.metric) or its left
+ * square root out = t / sqrt(lam) (sqrt_form = 1, left_sqrt_metric), the
+ * two products nifty.re's sampling calls on the likelihood composed at
+ * jubik/instruments/chandra/chandra_likelihood.py:37. */
+int jubik_poisson_metric(int dtype, const void* lam, const void* t, int64_t n, int sqrt_form,
+                         void* out, void* stream);
 
 /* ---------------------------------------------------------------- DFT comparison
  * Batched 128-point complex DFTs (FP32 complex in and out, B vectors, B a
@@ -240,10 +252,21 @@ int jubik_model_grad_hyper(jubik_model* model, const void* amp_ref, const void* 
                            const void* dnu, double* nll, void* g_ref, void* g_alpha,
                            void* g_amp_ref, void* g_amp_alpha, void* g_offsets, void* stream);
 /* MGVI metric product J^T diag(1/lam) J v at the position of the last
- * jubik_model_grad call (the metric nifty.re's sampling uses). */
+ * jubik_model_grad call (the metric nifty.re's sampling uses).  The base
+ * point holds the sky and lam of that call: pass the same amplitude fields
+ * and dnu, and after the amplitude fields or offsets change evaluate the
+ * gradient again before the next metric product. */
 int jubik_model_metric(jubik_model* model, const void* amp_ref, const void* amp_alpha,
                        const void* dnu, const void* v_ref, const void* v_alpha, void* out_ref,
                        void* out_alpha, void* stream);
+/* Domain check of the last gradient evaluation (single or batch): the
+ * evaluation counts read-out pixels (counts >= 0) where lam <= 0 or is NaN,
+ * where log lam is undefined and the NLL becomes inf or NaN, as the
+ * reference's does (demos/x-ray-imaging.py:312-315: zero-exposure pixels
+ * must be masked).  Synchronises ``stream``; writes the count per sample to
+ * counts[0 .. S) when counts is not NULL and returns JUBIK_EDOMAIN with a
+ * message when any is positive. */
+int jubik_model_check_domain(jubik_model* model, int64_t* counts, void* stream);
 /* S gradient evaluations in one pass: xi_ref, xi_alpha, g_ref, g_alpha are
  * (S, N, N) stacks, nll holds S device doubles.  The metric base point is
  * not changed. */
@@ -268,6 +291,10 @@ int jubik_model_autotune(jubik_model* model, int64_t S, int reps, int* col_div, 
  * fixed buffers, captured once into an executable HIP graph.  Each launch
  * reads the buffers' current contents; launches of an S = 0 graph also set
  * the metric base point.  Capture fails while kernel timing is enabled. */
+/* Lifetime: a graph refers to the models, plans and buffers it captured;
+ * all of them must outlive the graph (destroy the graph first).  zm and mu
+ * are fixed at capture; the amplitude fields and positions are read from
+ * their buffers at every launch. */
 typedef struct jubik_graph jubik_graph;
 int jubik_model_grad_graph(jubik_graph** graph, jubik_model* model, int64_t S,
                            const void* amp_ref, const void* amp_alpha, const void* xi_ref,

@@ -17,6 +17,7 @@ from .sampling import conjugate_gradient, draw_linear_samples, metric_solve
 from .mock import create_mock_counts, poisson_sample
 from .psf import psf_interpolator
 from .dft import DFT128
+from . import synthetic
 from .autograd import linpatch_apply, xray_nll
 
 __all__ = [

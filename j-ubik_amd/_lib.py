@@ -12,7 +12,7 @@ from pathlib import Path
 LIB_PATH = Path(os.environ.get("JUBIK_HIP_LIB") or
                 Path(__file__).resolve().parent / "libjubik_hip.so")
 
-OK, EINVAL, EUNSUPPORTED, EHIP, ENOMEM, ENOTREADY = 0, -1, -2, -3, -4, -5
+OK, EINVAL, EUNSUPPORTED, EHIP, ENOMEM, ENOTREADY, EDOMAIN = 0, -1, -2, -3, -4, -5, -6
 FLOAT32, FLOAT64 = 0, 1
 KERNEL_FULL, KERNEL_WINDOW = 0, 1
 POISSON_WORK_BYTES = 8192
@@ -25,6 +25,7 @@ SIGNATURES = {
     "jubik_abi_version": [],
     "jubik_timing_enable": [_int],
     "jubik_timing_report": [C.c_char_p, _i64],
+    "jubik_timing_select": [C.c_char_p],
     "jubik_linpatch_create": [C.POINTER(_vp), _int, _i64, _i64, _i64, _i64, _i64, _i64, _int,
                               _int],
     "jubik_linpatch_destroy": [_vp],
@@ -39,6 +40,7 @@ SIGNATURES = {
     "jubik_readout_gather": [_int, _vp, _vp, _i64, _vp, _vp],
     "jubik_readout_scatter": [_int, _vp, _vp, _i64, _i64, _vp, _vp],
     "jubik_poisson_nll_grad": [_int, _vp, _vp, _i64, _vp, _vp, _vp, _vp],
+    "jubik_poisson_metric": [_int, _vp, _vp, _i64, _int, _vp, _vp],
     "jubik_gp_create": [C.POINTER(_vp), _int, _i64, _i64, _i64, _i64, _int],
     "jubik_gp_destroy": [_vp],
     "jubik_gp_forward": [_vp, _vp, _vp, _vp, _vp, _dbl, _dbl, _vp, _vp, _vp],
@@ -50,6 +52,7 @@ SIGNATURES = {
     "jubik_model_grad_hyper": [_vp, _vp, _vp, _vp, _vp, _dbl, _dbl, _vp, _vp, _vp, _vp, _vp, _vp,
                                _vp, _vp],
     "jubik_model_metric": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
+    "jubik_model_check_domain": [_vp, _vp, _vp],
     "jubik_linpatch_set_tile": [_vp, _int, _int],
     "jubik_linpatch_autotune": [_vp, _int, C.POINTER(_int), C.POINTER(_int), _dblp, _vp],
     "jubik_model_set_tile": [_vp, _int, _int],
@@ -109,7 +112,7 @@ def load():
 
 
 _EXC = {EINVAL: ValueError, EUNSUPPORTED: NotImplementedError, EHIP: RuntimeError,
-        ENOMEM: MemoryError, ENOTREADY: RuntimeError}
+        ENOMEM: MemoryError, ENOTREADY: RuntimeError, EDOMAIN: FloatingPointError}
 
 
 def check(status):
@@ -126,6 +129,11 @@ def call(name, *args):
 
 def timing_enable(on=True):
     call("jubik_timing_enable", int(bool(on)))
+
+
+def timing_select(name=None):
+    """Time only the kernels called ``name`` (None: every kernel)."""
+    call("jubik_timing_select", None if name is None else name.encode())
 
 
 def timing_report():

@@ -119,6 +119,17 @@ class LinpatchOperator:
                   float(scale), dev.stream())
         return gx
 
+    def __call__(self, x):
+        return self.forward(x)
+
+    # Linearisation protocol of the likelihood composition (Poissonian.amend):
+    # the operator is linear, so its JVP is the forward and its VJP the adjoint.
+    def jvp(self, x, t):
+        return self.forward(t)
+
+    def vjp(self, x, ct):
+        return self.adjoint(ct)
+
     # -------------------------------------------------------------- tuning
     def tune_key(self, samples=1):
         """Key of this geometry in a tile cache: device architecture, dtype,

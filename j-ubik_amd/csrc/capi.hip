@@ -39,9 +39,15 @@ struct TimingRec {
 std::atomic<bool> g_timing{false};
 std::mutex g_timing_mu;
 std::vector<TimingRec> g_timing_recs;
+std::string g_timing_only;  // empty: every kernel
 }  // namespace
 
 bool timing_enabled() { return g_timing.load(std::memory_order_relaxed); }
+bool timing_wanted(const char* name) {
+  if (!timing_enabled()) return false;
+  std::lock_guard<std::mutex> lk(g_timing_mu);
+  return g_timing_only.empty() || g_timing_only == name;
+}
 void timing_record(const char* name, hipEvent_t a, hipEvent_t b) {
   std::lock_guard<std::mutex> lk(g_timing_mu);
   g_timing_recs.push_back({name, a, b});
@@ -189,7 +195,9 @@ struct jubik_model {
   void* lam = nullptr;      // (1 + max_samples) x J x H x W
   void* dsky = nullptr;     // max_samples x E x H x W
   void* resid = nullptr;    // max_samples x J x H x W
-  void* partial = nullptr;  // max_samples x J x H doubles
+  void* partial = nullptr;  // 2 x max_samples x J x H doubles (NLL terms, lam <= 0 counts)
+  void* nbad = nullptr;     // max_samples doubles: read-out pixels with lam <= 0
+  int last_S = 0;           // samples of the last gradient evaluation
   void* u = nullptr;        // max_samples x E x H x W complex
   void* R1 = nullptr;       // own linpatch workspaces (max_samples > 1)
   void* R2 = nullptr;
@@ -203,6 +211,7 @@ struct jubik_model {
     dev_free(resid);
     dev_free(lam);
     dev_free(partial);
+    dev_free(nbad);
     dev_free(u);
     dev_free(R1);
     dev_free(R2);
@@ -246,6 +255,12 @@ int jubik_abi_version(void) { return 1; }
 
 int jubik_timing_enable(int on) {
   g_timing.store(on != 0);
+  return kOk;
+}
+
+int jubik_timing_select(const char* name) {
+  std::lock_guard<std::mutex> lk(g_timing_mu);
+  g_timing_only = name ? name : "";
   return kOk;
 }
 
@@ -624,6 +639,15 @@ int jubik_poisson_nll_grad(int dtype, const void* lam, const void* data, int64_t
   return poisson_nll_grad(dtype, lam, data, n, nll, grad, (double*)work, as_stream(stream));
 }
 
+int jubik_poisson_metric(int dtype, const void* lam, const void* t, int64_t n, int sqrt_form,
+                         void* out, void* stream) {
+  if (dtype != 0 && dtype != 1) return fail(kInvalidArgument, "dtype must be 0 or 1");
+  if (sqrt_form != 0 && sqrt_form != 1) return fail(kInvalidArgument, "sqrt_form must be 0 or 1");
+  if (n < 0) return fail(kInvalidArgument, "negative size");
+  if (n && (!lam || !t || !out)) return fail(kInvalidArgument, "NULL argument");
+  return poisson_metric(dtype, lam, t, n, sqrt_form, out, as_stream(stream));
+}
+
 // ------------------------------------------------------------------ DFT comparison
 struct jubik_dft128 {
   void* frag = nullptr;
@@ -761,7 +785,6 @@ int jubik_gp_create(jubik_gp** plan, int dtype, int64_t N, int64_t H, int64_t W,
     return fail(kInvalidArgument, "sky grid must fit in the GP grid");
   if (!gp_length_supported((int)N))
     return fail(kUnsupported, "GP grid size " + std::to_string(N) + " is not compiled in");
-  if (dtype == 1 && N > 1024) return fail(kUnsupported, "float64 GP grids are compiled up to 1024");
   if (!has_alpha && E != 1)
     return fail(kInvalidArgument, "several energies need the spectral-index field");
   auto* p = new jubik_gp();
@@ -845,7 +868,8 @@ int jubik_model_create_batched(jubik_model** model, jubik_gp* gp, jubik_linpatch
   if (!st) st = dev_alloc(&m->lam, img * g.J * (S + 1) * es, "lambda");
   if (!st) st = dev_alloc(&m->dsky, img * g.E * S * es, "sky tangent");
   if (!st) st = dev_alloc(&m->resid, img * g.J * S * es, "residual");
-  if (!st) st = dev_alloc(&m->partial, (size_t)g.J * g.H * S * sizeof(double), "NLL partials");
+  if (!st) st = dev_alloc(&m->partial, 2 * (size_t)g.J * g.H * S * sizeof(double), "NLL partials");
+  if (!st) st = dev_alloc(&m->nbad, S * sizeof(double), "lambda domain counts");
   if (!st) st = dev_alloc(&m->u, img * g.E * 2 * S * es, "GP cotangent planes");
   if (!st && S > 1) {
     st = dev_alloc(&m->R1, lp->ws_bytes * S, "batched workspace R1");
@@ -911,6 +935,8 @@ int model_grad(jubik_model* model, int S, int slot0, const void* amp_ref, const 
   fin.seglen = (int64_t)g.J * g.H;
   fin.nseg = S;
   fin.out = nll;
+  fin.bad_out = (double*)model->nbad;
+  model->last_S = S;
   st = linpatch_adjoint_gp(lp->dtype, g, model->lbufs(), model->resid, model->pixel_area, sky, dnu,
                            model->u, s, fin);
   if (st) return st;
@@ -1249,6 +1275,29 @@ int jubik_model_metric(jubik_model* model, const void* amp_ref, const void* amp_
                        void* out_alpha, void* stream) {
   return jubik_model_metric_batch(model, 1, amp_ref, amp_alpha, dnu, v_ref, v_alpha, out_ref,
                                   out_alpha, stream);
+}
+
+int jubik_model_check_domain(jubik_model* model, int64_t* counts, void* stream) {
+  if (!model) return fail(kInvalidArgument, "NULL argument");
+  if (model->last_S < 1) return fail(kNotReady, "no gradient evaluation yet");
+  DeviceGuard dg(model->lp->device);
+  std::vector<double> nb(model->last_S);
+  hipStream_t s = as_stream(stream);
+  JUBIK_HIP(hipMemcpyAsync(nb.data(), model->nbad, nb.size() * sizeof(double),
+                           hipMemcpyDeviceToHost, s));
+  JUBIK_HIP(hipStreamSynchronize(s));
+  std::string bad;
+  for (size_t k = 0; k < nb.size(); ++k) {
+    if (counts) counts[k] = (int64_t)nb[k];
+    if (nb[k] > 0)
+      bad += (bad.empty() ? "" : ", ") + std::to_string((int64_t)nb[k]) + " at sample " +
+             std::to_string(k);
+  }
+  if (!bad.empty())
+    return fail(kDomain, "lambda <= 0 at read-out pixels (" + bad +
+                             "): the Poisson NLL is inf or NaN there; mask pixels with zero "
+                             "exposure");
+  return kOk;
 }
 
 }  // extern "C"

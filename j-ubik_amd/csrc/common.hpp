@@ -17,6 +17,7 @@ enum Status : int {
   kRuntimeError = -3,
   kOutOfMemory = -4,
   kNotReady = -5,
+  kDomain = -6,
 };
 
 void set_error(const std::string& msg);
@@ -24,15 +25,17 @@ int fail(int code, const std::string& msg);
 int check_hip(hipError_t e, const char* what);
 
 // Optional per-launch timing (jubik_timing_enable): HIP events recorded on
-// the launch stream around each kernel, aggregated by name on report.
+// the launch stream around each kernel (or only around the kernels of one
+// name, jubik_timing_select), aggregated by name on report.
 bool timing_enabled();
+bool timing_wanted(const char* name);
 void timing_record(const char* name, hipEvent_t start, hipEvent_t stop);
 struct TimedLaunch {
   const char* name;
   hipStream_t stream;
   hipEvent_t a = nullptr, b = nullptr;
   TimedLaunch(const char* n, hipStream_t s) : name(n), stream(s) {
-    if (timing_enabled() && hipEventCreate(&a) == hipSuccess && hipEventCreate(&b) == hipSuccess)
+    if (timing_wanted(n) && hipEventCreate(&a) == hipSuccess && hipEventCreate(&b) == hipSuccess)
       (void)hipEventRecord(a, s);
     else
       a = b = nullptr;

@@ -377,8 +377,6 @@ struct GpOps {
 
 template <typename T, typename F>
 static int gp_dispatch_len(int N, F&& f) {
-  if (sizeof(T) == 8 && N > 1024)
-    return fail(kUnsupported, "float64 GP grids are compiled up to 1024");
   switch (N) {
 #define JUBIK_CASE(l) \
   case l:             \

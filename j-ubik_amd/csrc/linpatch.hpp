@@ -37,7 +37,9 @@ int linpatch_prepare(int dtype, const LinGeom& g, const LinBuffers& b, const voi
 int linpatch_forward(int dtype, const LinGeom& g, const LinBuffers& b, const void* x, void* y,
                      double scale, hipStream_t s);
 // Forward fused with exposure and the Poisson residual: lam = expo * PSF,
-// resid = expo * (1 - d / lam) where d >= 0 (else 0), NLL partials per row.
+// resid = expo * (1 - d / lam) where d >= 0 (else 0), NLL partials per row;
+// ``partial`` holds 2 S J H doubles: the row sums of the NLL terms, then the
+// row counts of read-out pixels with lam <= 0.
 int linpatch_forward_poisson(int dtype, const LinGeom& g, const LinBuffers& b, const void* x,
                              double scale, const void* expo, const int* counts, void* resid,
                              void* lam, double* partial, hipStream_t s);
@@ -53,11 +55,14 @@ int linpatch_adjoint(int dtype, const LinGeom& g, const LinBuffers& b, const voi
 // NLL finalisation folded into the adjoint column pass, which starts after
 // the forward has written every partial: out[b] = sum of partial[b * seglen
 // .. (b + 1) * seglen) for b < nseg, in a fixed order (no extra launch).
+// With bad_out set, bad_out[b] also receives the count of read-out pixels
+// with lam <= 0 of segment b (stored nseg * seglen after the NLL terms).
 struct NllFinal {
   const double* partial = nullptr;
   int64_t seglen = 0;
   int nseg = 0;
   double* out = nullptr;
+  double* bad_out = nullptr;
 };
 
 // Adjoint chained into the GP: u = sum_jobs scale * sky[e] * PSF^T(gy)_job *

@@ -485,10 +485,18 @@ __global__ __launch_bounds__(NT) void lp_cols(LinGeom g, const cplx<T>* __restri
     // that wrote the partials (same order as a one-block reduction).
     if (fin.out && (int)blockIdx.x < fin.nseg) {
       const double* pp = fin.partial + (int64_t)blockIdx.x * fin.seglen;
-      double acc = 0.0;
+      double acc = 0.0, nb = 0.0;
       for (int64_t k = tid; k < fin.seglen; k += NT) acc += pp[k];
       acc = block_sum<NT>(acc, red);
-      if (tid == 0) fin.out[blockIdx.x] = acc;
+      if (fin.bad_out) {
+        const double* pb = pp + (int64_t)fin.nseg * fin.seglen;
+        for (int64_t k = tid; k < fin.seglen; k += NT) nb += pb[k];
+        nb = block_sum<NT>(nb, red);
+      }
+      if (tid == 0) {
+        fin.out[blockIdx.x] = acc;
+        if (fin.bad_out) fin.bad_out[blockIdx.x] = nb;
+      }
     }
   }
 }
@@ -632,6 +640,9 @@ __device__ __forceinline__ void sum_pair_spectra(cplx<T>* buf, const cplx<T>* __
 
 // grid (H, S * J).  Exposure, counts and (metric) the base-point lam are
 // per (mod, e) and shared by the samples; y, resid and partial are per job.
+// EPI 1 writes two doubles per (job, row): the NLL terms at partial[job H +
+// i] and the number of read-out pixels with lam <= 0 (or NaN) S J H values
+// further on.
 template <typename T, int L, int EPI, int JPT>
 __global__ __launch_bounds__(NTO) void lp_fwd_out(LinGeom g, const cplx<T>* __restrict__ R2,
                                                  const cplx<T>* __restrict__ twg,
@@ -740,7 +751,7 @@ __global__ __launch_bounds__(NTO) void lp_fwd_out(LinGeom g, const cplx<T>* __re
       }
     }
   } else {
-    double nll = 0.0;
+    double nll = 0.0, bad = 0.0;
 #pragma unroll
     for (int t = 0; t < JPT; ++t) {
       const int j = tid + t * NTO;
@@ -750,6 +761,10 @@ __global__ __launch_bounds__(NTO) void lp_fwd_out(LinGeom g, const cplx<T>* __re
         const int d = cnt_v[t];
         T r = T(0);
         if (d >= 0) {
+          // A read-out pixel needs lam > 0 (zero exposure must be masked,
+          // demos/x-ray-imaging.py:314-315); the terms stay inf or NaN as
+          // in the reference and the pixel is counted.
+          if (!(lam > T(0))) bad += 1.0;
           // The log of an FP32 lam is taken in FP32 (<= 2 ulp; the gradient
           // does not use it), the terms are summed in FP64.
           const double lg = std::is_same<T, float>::value ? (double)logf((float)lam)
@@ -762,7 +777,12 @@ __global__ __launch_bounds__(NTO) void lp_fwd_out(LinGeom g, const cplx<T>* __re
       }
     }
     nll = block_sum<NTO>(nll, red);
-    if (tid == 0) partial[(int64_t)job * g.H + i] = nll;
+    bad = block_sum<NTO>(bad, red);
+    if (tid == 0) {
+      const int64_t o = (int64_t)job * g.H + i;
+      partial[o] = nll;
+      partial[o + (int64_t)g.S * g.J * g.H] = bad;
+    }
   }
 }
 

@@ -2,6 +2,8 @@
 // (jubik/response.py:12-112; the NLL formula of nifty.re's Poissonian as
 // documented at demos/x-ray-imaging.py:312-313).  Reductions run in FP64 in
 // a fixed order: per-block partial sums, then one block sums the partials.
+#include <type_traits>
+
 #include "response.hpp"
 
 namespace jubik {
@@ -59,6 +61,17 @@ __global__ __launch_bounds__(RNT) void k_poisson(const T* __restrict__ lam,
   }
   acc = block_sum<RNT>(acc, red);
   if (threadIdx.x == 0) partial[blockIdx.x] = acc;
+}
+
+// Metric of the Poisson likelihood in data space, diag(1/lam) (SQRT = 0),
+// and its left square root diag(lam^-1/2) (SQRT = 1), applied to t.
+template <typename T, bool SQRT>
+__global__ __launch_bounds__(RNT) void k_poisson_metric(const T* __restrict__ lam,
+                                                        const T* __restrict__ t, int64_t n,
+                                                        T* __restrict__ out) {
+  for (int64_t k = blockIdx.x * (int64_t)RNT + threadIdx.x; k < n;
+       k += (int64_t)gridDim.x * RNT)
+    out[k] = SQRT ? t[k] / sqrt(lam[k]) : t[k] / lam[k];
 }
 
 // out[0] = sum of partial[0 .. n) in a fixed order.
@@ -218,9 +231,32 @@ int poisson_sample(int dtype, const void* lam, int64_t n, uint64_t seed, int* co
   return check_hip(hipGetLastError(), "poisson_sample");
 }
 
+int poisson_metric(int dtype, const void* lam, const void* t, int64_t n, int sqrt_form,
+                   void* out, hipStream_t s) {
+  if (n <= 0) return kOk;
+  auto go = [&](auto tag, auto sq) {
+    using T = decltype(tag);
+    k_poisson_metric<T, decltype(sq)::value><<<grid_for(n), RNT, 0, s>>>(
+        (const T*)lam, (const T*)t, n, (T*)out);
+  };
+  if (dtype == 0) {
+    if (sqrt_form) go(float{}, std::true_type{});
+    else go(float{}, std::false_type{});
+  } else {
+    if (sqrt_form) go(double{}, std::true_type{});
+    else go(double{}, std::false_type{});
+  }
+  return check_hip(hipGetLastError(), "poisson_metric");
+}
+
+// Blocks of the Poisson reduction for n elements: one partial per block.
+static int poisson_blocks(int64_t n) {
+  return (int)std::min<int64_t>(std::max<int64_t>((n + RNT - 1) / RNT, 1), RBLOCKS);
+}
+
 int poisson_nll_grad(int dtype, const void* lam, const void* d, int64_t n, double* nll,
                      void* grad, double* work, hipStream_t s) {
-  const int blocks = (int)std::min<int64_t>(std::max<int64_t>((n + RNT - 1) / RNT, 1), RBLOCKS);
+  const int blocks = poisson_blocks(n);
   if (dtype == 0)
     k_poisson<float><<<blocks, RNT, 0, s>>>((const float*)lam, (const float*)d, n, (float*)grad,
                                             work);

@@ -19,6 +19,9 @@ int poisson_sample(int dtype, const void* lam, int64_t n, uint64_t seed, int* co
 // (n pixels per sample, S samples).
 int sqrt_metric_cot(int dtype, const void* expo, const int* counts, const void* lam,
                     const void* noise, void* cot, int64_t n, int S, hipStream_t s);
+// out = t / lam (sqrt_form 0) or t / sqrt(lam) (sqrt_form 1).
+int poisson_metric(int dtype, const void* lam, const void* t, int64_t n, int sqrt_form,
+                   void* out, hipStream_t s);
 int poisson_nll_grad(int dtype, const void* lam, const void* d, int64_t n, double* nll,
                      void* grad, double* work, hipStream_t s);
 

@@ -2,7 +2,8 @@
 J-UBIK composes with its response (chandra_likelihood.py:37,
 erosita_likelihood.py:89).  Energy sum(lam - d log lam)
 (demos/x-ray-imaging.py:312-313), reduced in FP64; gradient 1 - d/lam;
-metric diag(1/lam).
+metric diag(1/lam) and its left square root diag(lam^-1/2), composed with the
+linearisation of an amended forward.
 """
 import torch
 
@@ -66,3 +67,56 @@ class Poissonian:
 
     def __call__(self, x):
         return self.energy_and_gradient(self._lam(x))[0]
+
+    # ---------------------------------------------------------------- metric
+    def _data_metric(self, lam, t, sqrt_form):
+        dev.require_device()
+        leaves_l, leaves_t = _leaves(lam), _leaves(t)
+        if len(leaves_l) != len(leaves_t):
+            raise ValueError("lambda and tangent have different structure")
+        out = []
+        for l, v in zip(leaves_l, leaves_t):
+            dtype = dev.float_dtype(l, v)
+            lt = dev.to_device(l, dtype).reshape(-1)
+            vt = dev.to_device(v, dtype).reshape(-1)
+            if lt.numel() != vt.numel():
+                raise ValueError("lambda and tangent sizes differ")
+            o = torch.empty_like(vt)
+            _lib.call("jubik_poisson_metric", dev.code(dtype), dev.ptr(lt), dev.ptr(vt),
+                      lt.numel(), int(sqrt_form), dev.ptr(o), dev.stream())
+            out.append(dev.like_input(o.reshape(tuple(v.shape)), v))
+        if isinstance(t, Vector):
+            return Vector(dict(zip(t.tree.keys(), out)))
+        if isinstance(t, dict):
+            return dict(zip(t.keys(), out))
+        return out[0]
+
+    def _linearised(self):
+        f = self._forward
+        if f is None:
+            return None
+        if not (hasattr(f, "jvp") and hasattr(f, "vjp")):
+            raise NotImplementedError(
+                "the metric of an amended likelihood needs a forward with jvp(x, t) and "
+                "vjp(x, ct) (LinpatchOperator has them; XrayModel.metric is the fused product)")
+        return f
+
+    def metric(self, primals, tangents):
+        """Fisher metric applied to ``tangents`` at ``primals``: diag(1/lam)
+        in data space, J^T diag(1/lam) J through an amended forward, as
+        nifty.re's Likelihood.metric."""
+        f = self._linearised()
+        if f is None:
+            return self._data_metric(primals, tangents, False)
+        lam = f(primals)
+        return f.vjp(primals, self._data_metric(lam, f.jvp(primals, tangents), False))
+
+    def left_sqrt_metric(self, primals, tangents):
+        """Left square root of the metric applied to data-space ``tangents``:
+        diag(lam^-1/2) t, or J^T diag(lam^-1/2) t through an amended forward
+        (nifty.re's Likelihood.left_sqrt_metric, which the sample draws of
+        optimize_kl call)."""
+        f = self._linearised()
+        if f is None:
+            return self._data_metric(primals, tangents, True)
+        return f.vjp(primals, self._data_metric(f(primals), tangents, True))

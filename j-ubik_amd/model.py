@@ -66,6 +66,7 @@ class XrayModel:
         self._nll = torch.empty(1, dtype=torch.float64, device="cuda")
         self.J = J
         self.col_div, self.row_div = 0, 0
+        self._point_version = None  # sky version of the metric base point
 
     def __del__(self):
         m = getattr(self, "_model", None)
@@ -119,14 +120,37 @@ class XrayModel:
         _lib.call("jubik_model_grad", self._model, dev.ptr(s.amp_ref), dev.ptr(s.amp_alpha),
                   dev.ptr(xr), dev.ptr(xa), s.zm, s.mu, dev.ptr(s.dnu_dev), dev.ptr(self._nll),
                   dev.ptr(g_ref), dev.ptr(g_alpha), dev.stream())
+        self._point_version = s.version
         return self._nll, g_ref, g_alpha
 
+    def check_domain(self, raise_error=True):
+        """Counts of read-out pixels with lam <= 0 per sample of the last
+        gradient evaluation (synchronises).  Raises FloatingPointError when
+        any is positive (the NLL is then inf or NaN, as in the reference:
+        zero-exposure pixels must be masked) unless ``raise_error`` is
+        False."""
+        n = max(self.max_samples, 1)
+        counts = (ctypes.c_int64 * n)()
+        st = _lib.load().jubik_model_check_domain(self._model, counts, dev.stream())
+        if st != _lib.EDOMAIN or raise_error:
+            _lib.check(st)
+        return counts
+
     def nll_and_grad(self, xi_ref, xi_alpha=None):
-        """(NLL as a Python float, grad_xi_ref, grad_xi_alpha)."""
+        """(NLL as a Python float, grad_xi_ref, grad_xi_alpha).  Raises
+        FloatingPointError when lam <= 0 at a read-out pixel."""
         nll, g_ref, g_alpha = self.nll_and_grad_device(xi_ref, xi_alpha)
+        self.check_domain()
         val = float(nll.item())
         return val, dev.like_input(g_ref, xi_ref), (
             dev.like_input(g_alpha, xi_ref) if g_alpha is not None else None)
+
+    def _check_point(self):
+        if self._point_version is None:
+            raise RuntimeError("no base point (evaluate the gradient first)")
+        if self._point_version != self.sky.version:
+            raise RuntimeError("the sky parameters changed since the last gradient evaluation: "
+                               "evaluate the gradient at the new parameters first")
 
     def nll_and_grad_hyper(self, xi_ref, xi_alpha=None):
         """One gradient evaluation that also returns the gradients with
@@ -145,6 +169,8 @@ class XrayModel:
                   dev.ptr(s.amp_alpha), dev.ptr(xr), dev.ptr(xa), s.zm, s.mu,
                   dev.ptr(s.dnu_dev), dev.ptr(self._nll), dev.ptr(g_ref), dev.ptr(g_alpha),
                   dev.ptr(ga_ref), dev.ptr(ga_alpha), dev.ptr(off), dev.stream())
+        self._point_version = s.version
+        self.check_domain()
         val = float(self._nll.item())
         offs = off.double().cpu().numpy()
         like = lambda t: None if t is None else dev.like_input(t, xi_ref)
@@ -153,6 +179,7 @@ class XrayModel:
 
     def metric(self, v_ref, v_alpha=None):
         """J^T diag(1/lam) J v at the position of the last gradient call."""
+        self._check_point()
         s = self.sky
         vr = dev.to_device(v_ref, self.dtype)
         va = dev.to_device(v_alpha, self.dtype) if s.has_alpha else None
@@ -190,12 +217,14 @@ class XrayModel:
         _lib.call("jubik_model_grad_batch", self._model, S, dev.ptr(s.amp_ref),
                   dev.ptr(s.amp_alpha), dev.ptr(xr), dev.ptr(xa), s.zm, s.mu, dev.ptr(s.dnu_dev),
                   dev.ptr(nll), dev.ptr(g_ref), dev.ptr(g_alpha), dev.stream())
+        self.check_domain()
         return nll.cpu().numpy(), dev.like_input(g_ref, xi_ref), (
             dev.like_input(g_alpha, xi_ref) if g_alpha is not None else None)
 
     def metric_batch(self, v_ref, v_alpha=None):
         """S metric products J^T diag(1/lam) J v at the last gradient position;
         v of shape (S, N, N)."""
+        self._check_point()
         s = self.sky
         S = self._batch_size(v_ref)
         vr = self._stack(v_ref, S, "v_ref")
@@ -212,6 +241,7 @@ class XrayModel:
         """J^T diag(lam^-1/2) noise at the last gradient position for S
         data-space noise vectors of shape (S, M, E, H, W) (entries where the
         counts are masked are ignored); returns (S, N, N) stacks."""
+        self._check_point()
         s = self.sky
         S = self._batch_size(noise)
         shape = (S,) + tuple(self.exposure.shape)
@@ -253,6 +283,14 @@ class _GradSlot:
         self.g_ref = torch.empty(shape, dtype=model.dtype, device="cuda")
         self.g_alpha = torch.empty(shape, dtype=model.dtype, device="cuda") if s.has_alpha else None
         self.nll = torch.empty(max(samples, 1), dtype=torch.float64, device="cuda")
+        self.offsets_version = s.offsets_version
+
+    def check_current(self):
+        """The captured offsets (zm, mu) are launch arguments fixed at
+        capture: refuse to launch after they changed."""
+        if self.model.sky.offsets_version != self.offsets_version:
+            raise RuntimeError("the sky offsets (zm, mu) changed since this graph was captured: "
+                               "capture it again")
 
     def desc(self):
         s = self.model.sky
@@ -285,7 +323,12 @@ class BatchGradGraph:
 
     def launch(self):
         """Enqueue one replay on the current stream (no synchronisation)."""
+        for sl in self.slots:
+            sl.check_current()
         _lib.call("jubik_graph_launch", self._graph, dev.stream())
+        for sl in self.slots:
+            if sl.samples == 0:
+                sl.model._point_version = sl.model.sky.version
 
     def __del__(self):
         g = getattr(self, "_graph", None)
@@ -303,6 +346,7 @@ class GradGraph:
     def __init__(self, model: XrayModel, samples):
         s = model.sky
         slot = _GradSlot(model, samples)
+        self._slot = slot
         self.model = model
         self.xi_ref, self.xi_alpha = slot.xi_ref, slot.xi_alpha
         self.g_ref, self.g_alpha, self.nll = slot.g_ref, slot.g_alpha, slot.nll
@@ -314,7 +358,10 @@ class GradGraph:
 
     def launch(self):
         """Enqueue one replay on the current stream (no synchronisation)."""
+        self._slot.check_current()
         _lib.call("jubik_graph_launch", self._graph, dev.stream())
+        if self._slot.samples == 0:
+            self.model._point_version = self.model.sky.version
 
     def __del__(self):
         g = getattr(self, "_graph", None)

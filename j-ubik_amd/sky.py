@@ -80,6 +80,8 @@ class CorrelatedFieldSky:
         for a in (self.amp_ref, self.amp_alpha):
             if a is not None and tuple(a.shape) != (self.N, self.N):
                 raise ValueError(f"amplitude fields must be ({self.N}, {self.N})")
+        self.version = 0          # bumped by every set_parameters call
+        self.offsets_version = 0  # bumped when zm or mu change
         self._plan = ctypes.c_void_p()
         _lib.call("jubik_gp_create", ctypes.byref(self._plan), dev.code(dtype), self.N, self.H,
                   self.W, self.E, int(self.has_alpha))
@@ -96,7 +98,15 @@ class CorrelatedFieldSky:
     def set_parameters(self, amp_ref=None, amp_alpha=None, zm=None, mu=None):
         """Replace the amplitude fields and/or offsets (the outputs of the
         reference's amplitude models and zero-mode priors, which change as
-        their hyperparameters are inferred); shapes and dtype are fixed."""
+        their hyperparameters are inferred); shapes and dtype are fixed.
+
+        The amplitude fields are overwritten in place, so their device
+        addresses stay valid for captured graphs, which read them at every
+        launch.  The offsets are launch arguments: a graph captured before
+        they change refuses to launch (``offsets_version``).  Every call
+        bumps ``version``; a model's metric base point taken at an older
+        version is refused."""
+        new = []
         for name, a in (("amp_ref", amp_ref), ("amp_alpha", amp_alpha)):
             if a is None:
                 continue
@@ -105,11 +115,16 @@ class CorrelatedFieldSky:
             t = dev.to_device(a, self.dtype)
             if tuple(t.shape) != (self.N, self.N):
                 raise ValueError(f"amplitude fields must be ({self.N}, {self.N})")
-            setattr(self, name, t.contiguous())
+            new.append((getattr(self, name), t))
+        for dst, t in new:
+            dst.copy_(t)
+        if (zm is not None and float(zm) != self.zm) or (mu is not None and float(mu) != self.mu):
+            self.offsets_version += 1
         if zm is not None:
             self.zm = float(zm)
         if mu is not None:
             self.mu = float(mu)
+        self.version += 1
 
     @property
     def shape(self):

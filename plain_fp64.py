@@ -1,13 +1,18 @@
 """Plain FP64 GPU path: the reference's op order with torch ops and autograd.
 
-This is the comparison baseline for the fused kernels, not part of the
-library's compute path: it restates jubik/convolve.py:28-176 (pad, window
-slicing, tent weights, margin pad, per-call kernel fftshift/cut/pad/
-ifftshift/normalise, complex128 FFT convolution, overlap-add, crop), the
-exposure/read-out of jubik/response.py and the Poisson energy as separate
-torch operations (hipFFT Z2Z for the transforms), and takes the gradient by
-autograd, as JAX does for the reference.  Every intermediate is a complex128
-(P, E, L, L) array in HBM.
+This is the comparison baseline that bench.py times beside the fused
+kernels; it is not part of the library.  It restates jubik/convolve.py:28-176
+(pad, window slicing, tent weights, margin pad, per-call kernel fftshift/
+cut/pad/ifftshift/normalise, complex128 FFT convolution, overlap-add, crop),
+the exposure/read-out of jubik/response.py and the Poisson energy as
+separate torch operations (hipFFT Z2Z for the transforms), and takes the
+gradient by autograd, as JAX does for the reference.  Every intermediate is a
+complex128 (P, E, L, L) array in HBM.
+
+The window slicing is one gather over all patches (what XLA makes of the
+vmapped ``lax.dynamic_slice`` at jubik/convolve.py:55-63) and the
+overlap-add one scatter-add (the ``linear_transpose`` of that slicing at
+jubik/convolve.py:138-143); autograd transposes each into the other.
 """
 import numpy as np
 import torch
@@ -28,8 +33,8 @@ class PlainFP64Model:
     """NLL(xi) = sum(lam - d log lam), lam = expo * linpatch(a * s(xi), K)."""
 
     def __init__(self, amp_ref, amp_alpha, zm, mu, dnu, kernel_full, n, margin, pixel_area,
-                 exposure, keep, data, prepare_each_call=True):
-        dev = "cuda"
+                 exposure, keep, data, prepare_each_call=True, device="cuda"):
+        dev = device
         f64 = torch.float64
         self.amp_ref = torch.as_tensor(amp_ref, dtype=f64, device=dev)
         self.amp_alpha = None if amp_alpha is None else torch.as_tensor(amp_alpha, dtype=f64,
@@ -50,6 +55,21 @@ class PlainFP64Model:
         self.h = self.dx // 2
         self.L = self.w + 2 * self.m
         self.weights = _tent(self.w, dev)
+        n, w, h, m, L = self.n, self.w, self.h, self.m, self.L
+        # Patch p = iy * n + ix starts at (sx[ix], sy[iy]) in the padded image
+        # (convolve.py:59-63); one index tensor per axis serves the gather,
+        # one flat canvas index the scatter.
+        sx = torch.as_tensor(_starts(n, self.dx, H + 2 * h, w), device=dev)
+        sy = torch.as_tensor(_starts(n, self.dx, W + 2 * h, w), device=dev)
+        p = torch.arange(n * n, device=dev)
+        px, py = sx[p % n], sy[p // n]
+        self._rows = (px[:, None] + torch.arange(w, device=dev)[None, :])[:, :, None]
+        self._cols = (py[:, None] + torch.arange(w, device=dev)[None, :])[:, None, :]
+        Wc = W + 2 * m + 2 * h
+        self._canvas_shape = (E, H + 2 * m + 2 * h, Wc)
+        ar = torch.arange(L, device=dev)
+        self._flat = ((px[:, None, None] + ar[None, :, None]) * Wc +
+                      (py[:, None, None] + ar[None, None, :])).reshape(-1)
         self._khat = None if prepare_each_call else self._kernel_spectrum()
 
     def _kernel_spectrum(self):
@@ -73,19 +93,16 @@ class PlainFP64Model:
 
     def linpatch(self, x):
         E, H, W = self.shape
-        n, w, h, m, L = self.n, self.w, self.h, self.m, self.L
+        h, m = self.h, self.m
         xp = torch.nn.functional.pad(x, (h, h, h, h))
-        sx = _starts(n, self.dx, H + 2 * h, w)
-        sy = _starts(n, self.dx, W + 2 * h, w)
-        order = [(p % n, p // n) for p in range(n * n)]
-        slices = torch.stack([xp[:, sx[i]:sx[i] + w, sy[j]:sy[j] + w] for i, j in order])
+        slices = xp[:, self._rows, self._cols].transpose(0, 1)  # (P, E, w, w), one gather
         z = torch.nn.functional.pad(self.weights * slices, (m, m, m, m))
         khat = self._kernel_spectrum() if self.prepare_each_call else self._khat
         y = torch.fft.ifftn(khat * torch.fft.fftn(z, dim=(-2, -1)), dim=(-2, -1)).real
-        canvas = torch.zeros((E, H + 2 * m + 2 * h, W + 2 * m + 2 * h), dtype=y.dtype,
-                             device=y.device)
-        for p, (i, j) in enumerate(order):
-            canvas[:, sx[i]:sx[i] + L, sy[j]:sy[j] + L] += y[p]
+        vals = y.transpose(0, 1).reshape(E, -1)  # (E, P * L * L)
+        canvas = torch.zeros((E, self._canvas_shape[1] * self._canvas_shape[2]), dtype=y.dtype,
+                             device=y.device).index_add(1, self._flat, vals)  # one scatter-add
+        canvas = canvas.reshape(self._canvas_shape)
         return canvas[:, h + m:h + m + H, h + m:h + m + W]
 
     def nll(self, xi_ref, xi_alpha=None):

@@ -1,4 +1,4 @@
-"""Seeded synthetic X-ray problems shared by the tests, smoke() and bench.py.
+"""Seeded synthetic X-ray problems shared by the tests and smoke().
 
 Inputs follow the workload descriptions: a GP sky with a power-law spectrum
 (slope -4, fluctuation std 1, offset -2), per-patch unit-sum PSF windows
@@ -8,51 +8,22 @@ the model at a true position.  Everything is numpy ``default_rng(seed)``.
 """
 import numpy as np
 
+import __graft_entry__ as _ge
 from oracle import model as om
+
+_ja = _ge.load_package()
+# The synthetic inputs come from the package (the benchmark uses the same
+# generators without importing the oracle).
+psf_windows = _ja.synthetic.psf_windows
 
 
 def powerlaw_amp(N, slope=-4.0, fluct=1.0, distance=1.0):
     """amp = hdvol * f * A[pd], A = |k|^(slope/2), A[0] = 0, sum amp^2 = f^2."""
-    k = np.fft.fftfreq(N, d=distance)
-    kk = np.sqrt(k[:, None] ** 2 + k[None, :] ** 2)
-    amp = np.zeros((N, N))
-    nz = kk > 0
-    amp[nz] = kk[nz] ** (slope / 2.0)
-    return amp * fluct / np.sqrt(np.sum(amp ** 2))
-
-
-def psf_windows(rng, n, E, w, sigma=(1.0, 3.0), ellip=(0.0, 0.3), moffat=None,
-                rotation=0.0):
-    """(P, E, w, w) windows; the PSF centre sits at index (w/2, w/2).
-
-    sigma and ellipticity vary linearly with the patch index across the
-    field; with ``moffat`` = beta the profile is 0.5 Gaussian + 0.5 Moffat
-    (alpha = 2 sigma).  Each window sums to one.
-    """
-    P = n * n
-    c = np.arange(w) - w // 2
-    yy, xx = np.meshgrid(c, c, indexing="ij")
-    out = np.empty((P, E, w, w))
-    for p in range(P):
-        t = p / max(P - 1, 1)
-        for e in range(E):
-            sg = (sigma[0] + (sigma[1] - sigma[0]) * t) * (1.0 + 0.15 * e)
-            el = ellip[0] + (ellip[1] - ellip[0]) * t
-            ang = rng.uniform(0, np.pi) + rotation
-            ca, sa = np.cos(ang), np.sin(ang)
-            u = ca * xx + sa * yy
-            v = (-sa * xx + ca * yy) / (1.0 - el)
-            r2 = (u ** 2 + v ** 2) / sg ** 2
-            g = np.exp(-0.5 * r2)
-            if moffat is not None:
-                g = 0.5 * g / g.sum() + 0.5 * (1.0 + r2 / 4.0) ** (-moffat) / \
-                    ((1.0 + r2 / 4.0) ** (-moffat)).sum()
-            out[p, e] = g / g.sum()
-    return out
+    return _ja.powerlaw_amplitude(N, distance=distance, slope=slope, fluctuations=fluct)
 
 
 def make_problem(H=64, n=4, margin=8, E=1, M=1, seed=0, N=None, mask_frac=0.1,
-                 expo_range=(1e3, 1e3), moffat=None, pixel_area=1.0):
+                 expo_range=(1e3, 1e3), moffat=None, pixel_area=1.0, sigma=(1.0, 3.0)):
     """Return (oracle Problem, (xi_ref, xi_alpha)) at the evaluation point."""
     rng = np.random.default_rng(seed)
     N = H if N is None else N
@@ -69,7 +40,8 @@ def make_problem(H=64, n=4, margin=8, E=1, M=1, seed=0, N=None, mask_frac=0.1,
                       pixel_area=pixel_area, amp_ref=amp_ref, amp_alpha=amp_alpha, zm=-2.0,
                       mu_alpha=-1.0 if spectral else 0.0, dnu=dnu)
     for mod in range(M):
-        win = psf_windows(rng, n, E, w, moffat=moffat, rotation=mod * np.pi / max(M, 1))
+        win = psf_windows(rng, n, E, w, sigma=sigma, moffat=moffat,
+                          rotation=mod * np.pi / max(M, 1))
         ramp = np.linspace(expo_range[0], expo_range[1], H)[None, None, :]
         expo = np.broadcast_to(ramp, (E, H, H)).copy()
         expo[rng.random(expo.shape) < mask_frac] = 0.0

@@ -39,8 +39,19 @@ def _compiled(op, col, row):
         return False
 
 
-@pytest.mark.parametrize("H,n,m,E", [(128, 4, 16, 1), (256, 2, 64, 2), (303, 3, 61, 1),
-                                     (8, 2, 4, 1), (16, 2, 8, 2), (128, 2, 48, 1)])
+# One geometry per compiled FFT length L = 2 (H / n) + 2 m.
+@pytest.mark.parametrize("H,n,m,E", [(8, 2, 4, 1),      # L = 16
+                                     (16, 2, 4, 1),     # 24
+                                     (16, 2, 8, 2),     # 32
+                                     (32, 2, 8, 1),     # 48
+                                     (32, 2, 16, 2),    # 64
+                                     (128, 4, 16, 1),   # 96
+                                     (64, 2, 32, 1),    # 128
+                                     (128, 2, 32, 1),   # 192
+                                     (128, 2, 48, 1),   # 224
+                                     (128, 2, 64, 2),   # 256
+                                     (303, 3, 61, 1),   # 324
+                                     (256, 2, 64, 2)])  # 384
 @pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
 def test_every_tile_agrees_to_rounding(ja, H, n, m, E, dtype):
     op, x, gy = _operator(ja, H, n, m, E, dtype, seed=H + n)

@@ -111,8 +111,9 @@ def test_wide_2048x4E_fp32_vs_fp64(ja):
     assert _rel(out[torch.float32][1], out[torch.float64][1]) <= 2e-6
 
 
-def test_gp_2048_vs_oracle(ja):
-    """The 2048^2 spectral GP (FP32) against the oracle."""
+@pytest.mark.parametrize("dtype,tol", [(torch.float64, 1e-12), (torch.float32, 2e-6)])
+def test_gp_2048_vs_oracle(ja, dtype, tol):
+    """The 2048^2 spectral GP and its VJP against the oracle."""
     from oracle import sky as osky
     from tests.problems import powerlaw_amp
     N, E = 2048, 4
@@ -122,6 +123,61 @@ def test_gp_2048_vs_oracle(ja):
     xr, xa = rng.normal(size=(N, N)), rng.normal(size=(N, N))
     want = osky.gp_sky(xr, xa, amp, 0.5 * amp, -2.0, -1.0, dnu)
     sky = ja.CorrelatedFieldSky(N, (N, N), amp, 0.5 * amp, zm=-2.0, mu=-1.0, dnu=dnu,
-                                dtype=torch.float32)
+                                dtype=dtype)
     got = sky(torch.as_tensor(xr, device="cuda"), torch.as_tensor(xa, device="cuda"))
-    assert _rel(got.double().cpu().numpy(), want) <= 2e-6
+    assert _rel(got.double().cpu().numpy(), want) <= tol
+    cot = rng.normal(size=want.shape)
+    g_ref, g_alpha, _, _ = osky.gp_adjoint(want * cot, amp, 0.5 * amp, dnu, grid_shape=(N, N))
+    gr, ga = sky.vjp(torch.as_tensor(want, device="cuda"), torch.as_tensor(cot, device="cuda"))
+    assert _rel(gr.double().cpu().numpy(), g_ref) <= tol
+    assert _rel(ga.double().cpu().numpy(), g_alpha) <= tol
+
+
+def test_wide_2048x4E_gradient_fp32_vs_fp64(ja):
+    """The full 2048^2 x 4 energy gradient (GP, PSF with n = 16, margin 64,
+    exposure, Poisson) in FP32 against the FP64 build, for one evaluation
+    and for a batch of VI samples in one pass."""
+    H, n, m, E = 2048, 16, 64, 4
+    rng = np.random.default_rng(5)
+    from tests.problems import powerlaw_amp
+    amp = powerlaw_amp(H)
+    dnu = np.log(np.array([1.0, 1.6, 2.5, 4.0]))
+    win = psf_windows(rng, n, E, 2 * (H // n), sigma=(1.0, 4.0), moffat=2.5)
+    ramp = np.linspace(5e2, 2e3, H)[None, None, :]
+    expo = np.broadcast_to(ramp, (E, H, H)).copy()
+    expo[rng.random(expo.shape) < 0.05] = 0.0
+    keep = expo > 0
+    xt, xat = rng.normal(size=(H, H)), rng.normal(size=(H, H))
+    S = 3
+    xs = [xt[None] + 0.1 * rng.normal(size=(S, H, H)), xat[None] + 0.1 * rng.normal(size=(S, H, H))]
+
+    def build(dtype, counts, S=1):
+        sky = ja.CorrelatedFieldSky(H, (H, H), amp, 0.5 * amp, zm=-2.0, mu=-1.0, dnu=dnu,
+                                    dtype=dtype)
+        psf = ja.LinpatchOperator((E, H, H), n, m, dtype=dtype)
+        psf.set_kernel(win, layout="window")
+        return sky, psf, ja.XrayModel(sky, psf, 1.0, expo[None], counts[None], max_samples=S)
+
+    sky64, psf64, _ = build(torch.float64, np.zeros(keep.shape, np.int32))
+    lam = psf64.forward(sky64(torch.as_tensor(xt, device="cuda"),
+                              torch.as_tensor(xat, device="cuda"))) * torch.as_tensor(
+        expo, device="cuda")
+    counts = np.where(keep, ja.poisson_sample(lam, seed=5).cpu().numpy(), -1).astype(np.int32)
+    del sky64, psf64, lam
+    res = {}
+    for dtype in (torch.float64, torch.float32):
+        _, _, model = build(dtype, counts, S)
+        nll, gr, ga = model.nll_and_grad_batch(*[torch.as_tensor(v, dtype=dtype, device="cuda")
+                                                 for v in xs])
+        res[dtype] = (nll, gr.double().cpu().numpy(), ga.double().cpu().numpy())
+        del model, gr, ga
+        torch.cuda.empty_cache()
+    for s in range(S):
+        g32 = np.concatenate([res[torch.float32][1][s].ravel(), res[torch.float32][2][s].ravel()])
+        g64 = np.concatenate([res[torch.float64][1][s].ravel(), res[torch.float64][2][s].ravel()])
+        err = _rel(g32, g64)
+        nmax = float(np.max(np.abs(g32 - g64)) / np.max(np.abs(g64)))
+        print(f"2048^2 x 4E sample {s}: FP32 vs FP64 grad rel {err:.2e}, normalised max {nmax:.2e}")
+        assert err <= 1e-5
+        n64 = res[torch.float64][0][s]
+        assert abs(res[torch.float32][0][s] - n64) <= 1e-6 * abs(n64)

@@ -2,7 +2,8 @@
 differences of its own NLL, in float64 on the GPU (step 1e-6): the
 directional derivative g . v must match (NLL(xi + e v) - NLL(xi - e v)) / 2e.
 The 1024^2 case is the plain-vs-adjoint geometry (one energy, 8 x 8 patches,
-margin 64, L = 384)."""
+margin 64, L = 384) with its Gaussian+Moffat PSFs (beta 2.5, core sigma
+1-4 px), exposure ramp 5e2-2e3 s and 5 % mask."""
 import numpy as np
 import pytest
 import torch
@@ -14,7 +15,8 @@ pytestmark = pytest.mark.gpu
 
 @pytest.mark.parametrize("cfg", [
     dict(H=64, n=4, margin=8, E=2, M=1, seed=71),
-    dict(H=1024, n=8, margin=64, E=1, M=1, seed=72, expo_range=(5e2, 2e3), mask_frac=0.05),
+    dict(H=1024, n=8, margin=64, E=1, M=1, seed=72, expo_range=(5e2, 2e3), mask_frac=0.05,
+         moffat=2.5, sigma=(1.0, 4.0)),
 ])
 def test_adjoint_matches_central_differences(ja, cfg):
     prob, xi = make_problem(**cfg)

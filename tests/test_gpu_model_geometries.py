@@ -58,8 +58,8 @@ def test_grad_and_metric_vs_oracle(ja, cfg, dtype, tol):
 
 def test_wide_image_fp32_vs_oracle(ja):
     """W = 1280 (8 output columns per thread in the overlap kernels) on a
-    2048 GP grid, FP32 (the FP64 GP is compiled up to 1024): NLL, gradient
-    and a metric product against the oracle."""
+    2048 GP grid, FP32: NLL, gradient and a metric product against the
+    oracle."""
     prob, xi = make_problem(H=1280, n=10, margin=64, E=1, M=1, seed=66, N=2048)
     want_nll, (want_r, _) = om.nll_and_grad(prob, *xi)
     model = to_product(ja, prob, dtype=torch.float32)

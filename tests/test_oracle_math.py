@@ -158,6 +158,38 @@ def test_powerlaw_amplitude_pixel_variance():
     assert np.var(samples) == pytest.approx(f ** 2, rel=0.08)
 
 
+def test_spectral_composition_closed_forms():
+    """The spectral sky's energy dependence in closed form
+    (spectral_product_mf_sky.py:206-219): at xi = 0 every pixel holds
+    log s_e = zm + mu dnu_e; with only xi_ref set the energies differ by
+    mu (dnu_e - dnu_0) everywhere (the reference field carries no dnu);
+    with only xi_alpha set log s_e - zm = (HT(amp_a xi_a) + mu) dnu_e, which
+    a direct cas-sum gives for a single excited mode."""
+    N = 8
+    zm, mu = -2.0, -1.3
+    dnu = np.log([1.0, 1.6, 2.5, 4.0])
+    amp = powerlaw_amp(N)
+    zero = np.zeros((N, N))
+    ls = gp.gp_log_sky(zero, zero, amp, 0.5 * amp, zm, mu, dnu)
+    np.testing.assert_allclose(ls, np.broadcast_to((zm + mu * dnu)[:, None, None], ls.shape),
+                               rtol=0, atol=1e-15)
+    rng = np.random.default_rng(4)
+    xr = rng.normal(size=(N, N))
+    ls = gp.gp_log_sky(xr, zero, amp, 0.5 * amp, zm, mu, dnu)
+    diff = ls - ls[0]
+    np.testing.assert_allclose(diff, np.broadcast_to((mu * (dnu - dnu[0]))[:, None, None],
+                                                     diff.shape), atol=1e-13)
+    # one excited mode k = (1, 2) of the spectral-index field
+    xa = np.zeros((N, N))
+    xa[1, 2] = 1.5
+    x = np.arange(N)
+    cas = (np.cos(2 * np.pi * (x[:, None] * 1 + x[None, :] * 2) / N) -
+           np.sin(2 * np.pi * (x[:, None] * 1 + x[None, :] * 2) / N))
+    t_alpha = 0.5 * amp[1, 2] * 1.5 * cas
+    ls = gp.gp_log_sky(zero, xa, amp, 0.5 * amp, zm, mu, dnu)
+    np.testing.assert_allclose(ls - zm, (t_alpha[None] + mu) * dnu[:, None, None], atol=1e-13)
+
+
 def test_gp_adjoint_dot_product():
     N, E = 16, 3
     rng = np.random.default_rng(7)
@@ -195,7 +227,8 @@ def test_poisson_gradient_finite_differences():
 
 # ---------------------------------------------------------------- model
 @pytest.mark.parametrize("cfg", [dict(H=32, n=4, margin=4, E=1, M=1, seed=1),
-                                 dict(H=32, n=4, margin=4, E=3, M=2, seed=2)])
+                                 dict(H=32, n=4, margin=4, E=3, M=2, seed=2),
+                                 dict(H=32, n=4, margin=4, E=2, M=2, seed=3, pixel_area=0.37)])
 def test_model_gradient_finite_differences(cfg):
     prob, xi = make_problem(**cfg)
     _, (gr, ga) = om.nll_and_grad(prob, *xi)
@@ -210,7 +243,8 @@ def test_model_gradient_finite_differences(cfg):
 
 
 @pytest.mark.parametrize("cfg", [dict(H=32, n=4, margin=4, E=1, M=1, seed=5, N=40),
-                                 dict(H=32, n=4, margin=4, E=3, M=2, seed=6)])
+                                 dict(H=32, n=4, margin=4, E=3, M=2, seed=6),
+                                 dict(H=32, n=4, margin=4, E=2, M=1, seed=7, pixel_area=0.37)])
 def test_hyper_gradients_finite_differences(cfg):
     """Amplitude-field and offset gradients against central differences of
     the NLL in the amplitude fields, zm and mu_alpha."""
