@@ -335,7 +335,9 @@ int jubik_linpatch_create(jubik_linpatch** plan, int dtype, int64_t H, int64_t W
   g.h = (int)(dx / 2);
   g.w = (int)(2 * dx);
   g.m = (int)margin;
-  g.L = g.w + 2 * g.m;
+  g.Lw = g.w + 2 * g.m;
+  g.L = linpatch_fft_length(g.w, g.m);
+  g.mf = (g.L - g.w) / 2;
   g.L2 = g.L / 2;
   g.limx = g.H + 2 * g.h - g.w;
   g.limy = g.W + 2 * g.h - g.w;
@@ -350,10 +352,10 @@ int jubik_linpatch_create(jubik_linpatch** plan, int dtype, int64_t H, int64_t W
   if (g.limx < 0 || g.limy < 0)
     return fail(kInvalidArgument, "patch window of " + std::to_string(g.w) +
                                       " does not fit the padded grid");
-  if (!linpatch_length_supported(g.L))
-    return fail(kUnsupported, "FFT length " + std::to_string(g.L) + " (window " +
-                                  std::to_string(g.w) + " + 2 * margin " +
-                                  std::to_string(g.m) + ") is not compiled in");
+  if (g.L == 0)
+    return fail(kUnsupported, "no compiled FFT length for window " + std::to_string(g.w) +
+                                  " + 2 * margin " + std::to_string(g.m) +
+                                  " (at most 512, or twice the window in fold mode)");
   auto* p = new jubik_linpatch();
   p->dtype = dtype;
   p->g = g;
@@ -484,7 +486,8 @@ static int autotune_tiles(int dtype, const LinGeom& g0, LinBuffers b, int S, int
   for (int k = 0; k < 6; ++k) ms[k] = -1.0;
   double best_ms = -1.0;
   b.row_div = 1;
-  for (int k = 0; k < 3 && !st; ++k) {
+  const int ndiv = g.Lw == g.L ? 3 : 1;  // fold mode runs the full tiles only
+  for (int k = 0; k < ndiv && !st; ++k) {
     if (!linpatch_col_tile(dtype, g.L, divs[k])) continue;
     b.col_div = divs[k];
     st = time_tiles(ms[k]);
@@ -495,7 +498,7 @@ static int autotune_tiles(int dtype, const LinGeom& g0, LinBuffers b, int S, int
   }
   b.col_div = *best_col;
   best_ms = -1.0;
-  for (int k = 0; k < 3 && !st; ++k) {
+  for (int k = 0; k < ndiv && !st; ++k) {
     if (!linpatch_row_tile(dtype, g.L, divs[k])) continue;
     b.row_div = divs[k];
     st = time_tiles(ms[3 + k]);
@@ -516,6 +519,10 @@ static int autotune_tiles(int dtype, const LinGeom& g0, LinBuffers b, int S, int
 // or row_div = 0 is accepted where `follow` allows it (the plan's value).
 static int check_tiles(const jubik_linpatch* plan, int col_div, int row_div, bool follow) {
   const int L = plan->g.L;
+  if (plan->g.Lw != L && (col_div > 1 || row_div > 1))
+    return fail(kInvalidArgument, "window length " + std::to_string(plan->g.Lw) +
+                                      " runs in fold mode at L = " + std::to_string(L) +
+                                      ", which has the full tiles only");
   if (!(follow && col_div == 0) && !linpatch_col_tile(plan->dtype, L, col_div))
     return fail(kInvalidArgument, "column tile divisor " + std::to_string(col_div) +
                                       " is not compiled in for L = " + std::to_string(L));

@@ -79,8 +79,18 @@ struct LinGeom {
   int h;          // zero padding of the slicing, d / 2
   int w;          // window 2d
   int m;          // margin
-  int L;          // FFT length w + 2m
+  int L;          // FFT length: w + 2m, or a longer compiled length (below)
   int L2;         // L / 2 stored half-spectrum columns (DC and Nyquist packed)
+  // Window length Lw = w + 2m of the reference's circular convolution and
+  // the margin mf = (L - w) / 2 of the FFT frame.  Normally L = Lw and
+  // mf = m.  When w + 2m has no compiled plan the transforms run at a
+  // compiled L >= max(Lw, 2w): with mf >= w / 2 a window's size-L circular
+  // convolution is its linear convolution g (support [-w/2, 3w/2 - 1) in
+  // window coordinates), and the size-Lw result is the fold
+  // sum_j g[u + j Lw] for u in [-m, w + m), j in {-1, 0, 1}; the adjoint
+  // reads the periodic extension of the Lw window ("fold mode").
+  int Lw;
+  int mf;
   int limx, limy; // clamp of the window starts: H + 2h - w, W + 2h - w
   int P;          // n * n
   int E;          // energies per module

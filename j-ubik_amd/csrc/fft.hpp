@@ -313,10 +313,14 @@ __host__ __device__ constexpr int plan_radix(int L, int s, int pl = 0) {
       case 192: w0 = 8; w1 = 4; w2 = 6; break;
       case 224: w0 = 8; w1 = 4; w2 = 7; break;
       case 256: w0 = 4; w1 = 8; w2 = 8; break;
+      case 320: w0 = 8; w1 = 8; w2 = 5; break;
       case 324: w0 = 6; w1 = 6; w2 = 9; break;
       case 384: w0 = 8; w1 = 8; w2 = 6; break;
+      case 432: w0 = 8; w1 = 6; w2 = 9; break;
+      case 567: w0 = 7; w1 = 9; w2 = 9; break;
       case 512: w0 = 8; w1 = 8; w2 = 8; break;
       case 1024: w0 = 8; w1 = 8; w2 = 16; break;
+      case 1134: w0 = 14; w1 = 9; w2 = 9; break;
       default: break;
     }
     if (w0 > 0) return s == 0 ? w0 : s == 1 ? w1 : s == 2 ? w2 : 0;
@@ -336,11 +340,15 @@ __host__ __device__ constexpr int plan_radix(int L, int s, int pl = 0) {
     case 192: r0 = 16; r1 = 12; break;
     case 224: r0 = 16; r1 = 14; break;
     case 256: r0 = 16; r1 = 16; break;
+    case 320: r0 = 16; r1 = 20; break;
     case 324: r0 = 18; r1 = 18; break;
     case 384: r0 = 16; r1 = 24; break;
+    case 432: r0 = 16; r1 = 27; break;
     case 512: r0 = 16; r1 = 32; break;
+    case 567: r0 = 21; r1 = 27; break;
     case 768: r0 = 16; r1 = 16; r2 = 3; break;
     case 1024: r0 = 32; r1 = 32; break;
+    case 1134: r0 = 27; r1 = 42; break;
     case 2048: r0 = 16; r1 = 16; r2 = 8; break;
     default: return -1;
   }

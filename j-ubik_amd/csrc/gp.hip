@@ -33,9 +33,11 @@ struct GpTile {
   static constexpr int V = (MODE == 1 && V0 > 2) ? V0 / 2 : V0;
 };
 
-// grid (N / V, S).  MODE 0: c = amp * xi (forward / tangent);  MODE 1: c =
-// the sum of the uplanes complex (H, W) cotangent planes, zero outside the
-// sky grid.  Sample blockIdx.y reads xi[s] / u[s] and writes work[s].
+// grid (ceil(N / V), S).  MODE 0: c = amp * xi (forward / tangent);  MODE 1:
+// c = the sum of the uplanes complex (H, W) cotangent planes, zero outside
+// the sky grid.  Sample blockIdx.y reads xi[s] / u[s] and writes work[s].
+// When V does not divide N (odd grids such as 567) the last tile's columns
+// beyond N read zero and are not stored.
 template <typename T, int N, int MODE>
 __global__ __launch_bounds__(GNT) void gp_cols(GpGeom g, const T* __restrict__ amp_ref,
                                                const T* __restrict__ amp_alpha,
@@ -88,7 +90,7 @@ __global__ __launch_bounds__(GNT) void gp_cols(GpGeom g, const T* __restrict__ a
         const int e = tid + (u0 + c) * GNT;
         const int x1 = e / V, v = e - (e / V) * V;
         const int x2 = c0 + v;
-        const bool ok = V * N % GNT == 0 || e < V * N;
+        const bool ok = (V * N % GNT == 0 || e < V * N) && (N % V == 0 || x2 < N);
         const int o = x1 * N + x2;
         const int orf = gather_off<T>(gar, ok, o), oal = gather_off<T>(gaa, ok && g.has_alpha, o);
         val[c].re = gather_load<T>(gar, orf) * gather_load<T>(gxr, orf);
@@ -124,7 +126,7 @@ __global__ __launch_bounds__(GNT) void gp_cols(GpGeom g, const T* __restrict__ a
   fft_lds<T, N, V, VS, GNT, false, kFewVectorPlan<N, V, GNT>>(buf, tw, tid);
   for (int e = tid; e < V * N; e += GNT) {
     const int x1 = e / V, v = e - (e / V) * V;
-    Tw[(int64_t)x1 * N + c0 + v] = buf[v * VS + x1];
+    if (N % V == 0 || c0 + v < N) Tw[(int64_t)x1 * N + c0 + v] = buf[v * VS + x1];
   }
 }
 
@@ -291,7 +293,11 @@ __global__ __launch_bounds__(GNT) void gp_fold(GpGeom g, const T* __restrict__ s
 }
 
 // ====================================================================== host
-#define JUBIK_GP_LENGTHS(X) X(16) X(32) X(64) X(128) X(256) X(512) X(768) X(1024) X(2048)
+// GP grid sizes: powers of two, 768, and the 7-smooth grids 567 and 1134 that
+// ducc0's good_size gives for the demos' padded grids (int(512 * 1.1075),
+// int(1024 * 1.1075); jubik/sky_models.py:306).
+#define JUBIK_GP_LENGTHS(X) \
+  X(16) X(32) X(64) X(128) X(256) X(512) X(567) X(768) X(1024) X(1134) X(2048)
 
 bool gp_length_supported(int N) {
   switch (N) {
@@ -313,7 +319,7 @@ struct GpOps {
                      const void* dnu, void* sky, hipStream_t s) {
     {
       TimedLaunch _t("gp_cols_fwd", s);
-      gp_cols<T, N, 0><<<dim3(N / V, g.S), GNT, 0, s>>>(g, (const T*)amp_ref, (const T*)amp_alpha,
+      gp_cols<T, N, 0><<<dim3((N + V - 1) / V, g.S), GNT, 0, s>>>(g, (const T*)amp_ref, (const T*)amp_alpha,
                                              (const T*)xi_ref, (const T*)xi_alpha, nullptr,
                                              (const C*)b.tw, (C*)b.work);
     }
@@ -330,7 +336,7 @@ struct GpOps {
                      const void* dnu, const void* sky, void* dsky, hipStream_t s) {
     {
       TimedLaunch _t("gp_cols_fwd", s);
-      gp_cols<T, N, 0><<<dim3(N / V, g.S), GNT, 0, s>>>(g, (const T*)amp_ref, (const T*)amp_alpha,
+      gp_cols<T, N, 0><<<dim3((N + V - 1) / V, g.S), GNT, 0, s>>>(g, (const T*)amp_ref, (const T*)amp_alpha,
                                              (const T*)v_ref, (const T*)v_alpha, nullptr,
                                              (const C*)b.tw, (C*)b.work);
     }
@@ -347,7 +353,7 @@ struct GpOps {
                        hipStream_t s, GpHyper hy) {
     {
       TimedLaunch _t("gp_cols_adj", s);
-      gp_cols<T, N, 1><<<dim3(N / GpTile<T, N, 1>::V, g.S), GNT, 0, s>>>(g, nullptr, nullptr, nullptr, nullptr, (const C*)u,
+      gp_cols<T, N, 1><<<dim3((N + GpTile<T, N, 1>::V - 1) / GpTile<T, N, 1>::V, g.S), GNT, 0, s>>>(g, nullptr, nullptr, nullptr, nullptr, (const C*)u,
                                              (const C*)b.tw, (C*)b.work);
     }
     if (hy.g_amp_ref) {

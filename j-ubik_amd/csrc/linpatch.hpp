@@ -16,11 +16,22 @@ struct LinBuffers {
   int row_div;       // row-pass tile: 1, 2 or 4
 };
 
-// FFT lengths L = 2 (H / n) + 2 margin compiled into the library.
+// FFT lengths compiled into the library, in groups of one translation unit
+// per group and element type (lp_<group>_f32.hip, lp_<group>_f64.hip).  A
+// window length w + 2m outside the list runs at the smallest listed length
+// L >= max(w + 2m, 2w) in fold mode (LinGeom).
+#define JUBIK_LP_LENGTHS_A(X) X(16) X(24) X(32) X(48) X(64) X(96) X(128)
+#define JUBIK_LP_LENGTHS_B(X) X(192) X(224) X(256)
+#define JUBIK_LP_LENGTHS_C(X) X(320) X(324)
+#define JUBIK_LP_LENGTHS_D(X) X(384) X(432) X(512)
 #define JUBIK_LP_LENGTHS(X) \
-  X(16) X(24) X(32) X(48) X(64) X(96) X(128) X(192) X(224) X(256) X(324) X(384)
+  JUBIK_LP_LENGTHS_A(X) JUBIK_LP_LENGTHS_B(X) JUBIK_LP_LENGTHS_C(X) JUBIK_LP_LENGTHS_D(X)
 
 bool linpatch_length_supported(int L);
+// FFT length for window w and margin m: w + 2m when compiled in, else the
+// fold-mode length; 0 when none is compiled in.
+int linpatch_fft_length(int w, int m);
+
 // Columns per work-group of the forward and adjoint column passes for a
 // column-tile divisor (1, 2 or 4), or 0 when that divisor is not compiled in
 // for L (fewer than 2 columns).
@@ -70,5 +81,24 @@ struct NllFinal {
 int linpatch_adjoint_gp(int dtype, const LinGeom& g, const LinBuffers& b, const void* gy,
                         double scale, const void* sky, const void* dnu, void* u, hipStream_t s,
                         NllFinal fin = NllFinal{});
+
+// Arguments of one linpatch operation (the entry points above), passed to
+// the per-length instantiations (lp_run<T, L> in linpatch_impl.hpp).
+struct LinCall {
+  enum Op { kPrepare, kForward, kForwardPoisson, kForwardMetric, kAdjoint, kAdjointGp };
+  int op = kForward;
+  const void* in = nullptr;  // x, dx, gy or the kernel
+  void* out = nullptr;       // y, metric cotangent, gx or the GP cotangent planes
+  double scale = 1.0;
+  const void* expo = nullptr;
+  const int* counts = nullptr;
+  void* resid = nullptr;
+  void* lam = nullptr;
+  double* partial = nullptr;
+  const void* sky = nullptr;
+  const void* dnu = nullptr;
+  int layout = 0;
+  NllFinal fin{};
+};
 
 }  // namespace jubik

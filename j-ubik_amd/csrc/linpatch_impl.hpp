@@ -37,11 +37,26 @@ enum ColMode { COL_FWD = 0, COL_ADJ = 1, COL_KERNEL = 2 };
 
 constexpr int NT = 256;
 
+// Cotangent window column (in [0, Lw)) of FFT-frame position f in [0, L):
+// f itself normally; in fold mode the periodic extension of the Lw window
+// (window coordinate f - mf taken modulo Lw into [-m, w + m)).
+template <bool FOLD>
+__device__ __forceinline__ int cot_col(const LinGeom& g, int f) {
+  if constexpr (!FOLD) {
+    return f;
+  } else {
+    int q = (f - g.mf + g.m) % g.Lw;
+    return q < 0 ? q + g.Lw : q;
+  }
+}
+
 // ------------------------------------------------------------------ rows
 // grid (ceil(nrows/2 / V), P, Z); Z = jobs (signal modes) or M*Ek (kernel).
 // VRC row pairs per work-group: RowTile<T, L>::V, or a half or quarter of it
 // (chosen by the autotuner for grids with too few row tiles).
-template <typename T, int L, int MODE, int VRC = RowTile<T, L>::V>
+// FOLD (cotangent mode): the row of L frame positions f holds the periodic
+// extension of the Lw-wide cotangent window (fold mode of LinGeom).
+template <typename T, int L, int MODE, int VRC = RowTile<T, L>::V, bool FOLD = false>
 __global__ __launch_bounds__(NT) void lp_rows(LinGeom g, const T* __restrict__ src,
                                               const T* __restrict__ tent,
                                               const cplx<T>* __restrict__ twg,
@@ -122,12 +137,12 @@ __global__ __launch_bounds__(NT) void lp_rows(LinGeom g, const T* __restrict__ s
       bool cok;
       T wc = T(1);
       if constexpr (MODE == ROW_SIGNAL) {
-        const int c = f - g.m;
+        const int c = f - g.mf;
         j = sy + c - g.h;
         cok = c >= 0 && c < g.w && j >= 0 && j < g.W;
         wc = gather_load<T>(gt, gather_off<T>(gt, cok, c)) * scale;
       } else {
-        j = sy + f - g.h - g.m;
+        j = sy + cot_col<FOLD>(g, f) - g.h - g.m;
         cok = j >= 0 && j < g.W;
       }
       if (L % NT != 0 && f >= L) cok = false;
@@ -167,8 +182,8 @@ __global__ __launch_bounds__(NT) void lp_rows(LinGeom g, const T* __restrict__ s
       const int r0 = 2 * (q0 + v);
       if constexpr (MODE == ROW_SIGNAL || MODE == ROW_COTANGENT) {
         // Zero-reading gathers, as in the column mapping above.
-        const int c = f - g.m;
-        const int j = MODE == ROW_SIGNAL ? sy + c - g.h : sy + f - g.h - g.m;
+        const int c = f - g.mf;
+        const int j = MODE == ROW_SIGNAL ? sy + c - g.h : sy + cot_col<FOLD>(g, f) - g.h - g.m;
         const int i0 = MODE == ROW_SIGNAL ? sx + r0 - g.h : sx + r0 - g.h - g.m;
         const bool cok = r0 < nrows && (MODE != ROW_SIGNAL || (c >= 0 && c < g.w)) && j >= 0 &&
                          j < g.W;
@@ -181,13 +196,13 @@ __global__ __launch_bounds__(NT) void lp_rows(LinGeom g, const T* __restrict__ s
         bv[u] = wb * gather_load<T>(gi, gather_off<T>(gi, rb, (i0 + 1) * g.W + j));
       } else if (r0 < nrows) {
         if constexpr (MODE == ROW_KERNEL_WIN) {
-          const int c = f - g.m;
+          const int c = f - g.mf;
           if (c >= 0 && c < g.w) {
             av[u] = img[r0 * g.w + c];
             bv[u] = img[(r0 + 1) * g.w + c];
           }
         } else {  // ROW_KERNEL_FULL: cut[r][c] = K[(r - d) mod H][(c - d) mod W]
-          const int c = f - g.m;
+          const int c = f - g.mf;
           if (c >= 0 && c < g.w) {
             const int j = (c - g.d + g.W) % g.W;
             const int ia = (r0 - g.d + g.H) % g.H;
@@ -290,7 +305,10 @@ __device__ __forceinline__ void share_packed(cplx<T>* buf, typename FL::Regs& la
 // Every column runs the same butterflies whatever VC is; the instantiations
 // differ only in how the compiler contracts multiply-adds, so results agree
 // to rounding (measured: at most 2.0e-7 relative in FP32, 2.3e-16 in FP64).
-template <typename T, int L, int MODE, int VC = Tile<T, L>::V>
+// FOLD: fold mode of LinGeom.  The forward folds the L output rows into the
+// Lw window rows (through LDS); the adjoint loads the periodic extension of
+// the Lw cotangent rows.
+template <typename T, int L, int MODE, int VC = Tile<T, L>::V, bool FOLD = false>
 __global__ __launch_bounds__(NT) void lp_cols(LinGeom g, const cplx<T>* __restrict__ R1,
                                               const cplx<T>* __restrict__ KT,
                                               const cplx<T>* __restrict__ twg,
@@ -325,9 +343,10 @@ __global__ __launch_bounds__(NT) void lp_cols(LinGeom g, const cplx<T>* __restri
     idx = (int64_t)(smp * g.J + z) * g.P + p;
     spec = spec_index(g, z, p);
   }
-  const int nin = (MODE == COL_ADJ) ? L : g.w;
+  const int lw = FOLD ? g.Lw : L;  // window rows of the cotangent / output
+  const int nin = (MODE == COL_ADJ) ? lw : g.w;
   const cplx<T>* in = R1 + idx * L2P * nin;
-  const int m = g.m, w = g.w;
+  const int m = g.mf, w = g.w;
 
   // Twiddles are fetched into registers now and stored to LDS once the tile
   // loads are issued: stage 0 needs none, and the barrier after it orders
@@ -378,8 +397,9 @@ __global__ __launch_bounds__(NT) void lp_cols(LinGeom g, const cplx<T>* __restri
     const int k = k0 + vec;
     if (k >= LH) return cplx<T>{T(0), T(0)};
     if constexpr (MODE == COL_ADJ) {
-      const int ir = row0 + pos;
-      return (ir >= 0 && ir < g.H) ? ld_stream(&in[pos * L2P + k]) : cplx<T>{T(0), T(0)};
+      const int q = cot_col<FOLD>(g, pos);
+      const int ir = row0 + q;
+      return (ir >= 0 && ir < g.H) ? ld_stream(&in[q * L2P + k]) : cplx<T>{T(0), T(0)};
     } else {
       const int rr = pos - m;
       return (rr >= 0 && rr < w) ? ld_stream(&in[rr * L2P + k]) : cplx<T>{T(0), T(0)};
@@ -453,7 +473,7 @@ __global__ __launch_bounds__(NT) void lp_cols(LinGeom g, const cplx<T>* __restri
     using IL = Stage<T, L, true, NS - 1, V, VS, NT>;
     static_assert(I0::R == FL::R && I0::BPT == FL::BPT, "shared register layout");
     I0::compute(last, tw, tid);
-    cplx<T>* o = out + idx * L2P * (MODE == COL_ADJ ? w : L);
+    cplx<T>* o = out + idx * L2P * (MODE == COL_ADJ ? w : lw);
     auto store = [&](int vec, int pos, cplx<T> val) {
       const int k = k0 + vec;
       if (k >= LH) return;
@@ -467,8 +487,36 @@ __global__ __launch_bounds__(NT) void lp_cols(LinGeom g, const cplx<T>* __restri
         if (ir >= 0 && ir < g.H) st_stream(&o[pos * L2P + k], val);
       }
     };
+    // Fold mode, forward: the transformed columns go to LDS, and output row
+    // q of the Lw window sums frame positions c, c + Lw and c - Lw, c = q +
+    // mf - m (in [0, L) because L >= Lw); rows outside the image are skipped
+    // as above.
+    auto to_lds = [&](int vec, int pos, cplx<T> val) { buf[vec * VS + pos] = val; };
+    auto fold_out = [&]() {
+      __syncthreads();
+      const int off = g.mf - g.m;
+      for (int e = tid; e < V * lw; e += NT) {
+        const int vec = e & (V - 1), q = e / V;
+        const int k = k0 + vec;
+        const int ir = row0 + q;
+        if (k >= LH || ir < 0 || ir >= g.H) continue;
+        const int c = q + off;
+        const cplx<T>* col = buf + vec * VS;
+        cplx<T> acc = col[c];
+        if (c + lw < L) acc = cadd(acc, col[c + lw]);
+        if (c >= lw) acc = cadd(acc, col[c - lw]);
+        st_stream(&o[q * L2P + k], acc);
+      }
+    };
+    constexpr bool kFoldOut = FOLD && MODE == COL_FWD;
     if constexpr (NS == 1) {
-      I0::emit(last, tid, store);
+      if constexpr (kFoldOut) {
+        __syncthreads();
+        I0::emit(last, tid, to_lds);
+        fold_out();
+      } else {
+        I0::emit(last, tid, store);
+      }
     } else {
       __syncthreads();
       I0::scatter_lds(buf, last, tid);
@@ -477,7 +525,13 @@ __global__ __launch_bounds__(NT) void lp_cols(LinGeom g, const cplx<T>* __restri
       typename IL::Regs v;
       IL::gather_lds(buf, v, tid);
       IL::compute(v, tw, tid);
-      IL::emit(v, tid, store);
+      if constexpr (kFoldOut) {
+        __syncthreads();
+        IL::emit(v, tid, to_lds);
+        fold_out();
+      } else {
+        IL::emit(v, tid, store);
+      }
     }
   }
   if constexpr (MODE == COL_ADJ) {
@@ -643,7 +697,9 @@ __device__ __forceinline__ void sum_pair_spectra(cplx<T>* buf, const cplx<T>* __
 // EPI 1 writes two doubles per (job, row): the NLL terms at partial[job H +
 // i] and the number of read-out pixels with lam <= 0 (or NaN) S J H values
 // further on.
-template <typename T, int L, int EPI, int JPT>
+// FOLD: fold mode of LinGeom; a patch column's output pixel at window
+// column c sums frame positions c', c' + Lw and c' - Lw, c' = c - m + mf.
+template <typename T, int L, int EPI, int JPT, bool FOLD = false>
 __global__ __launch_bounds__(NTO) void lp_fwd_out(LinGeom g, const cplx<T>* __restrict__ R2,
                                                  const cplx<T>* __restrict__ twg,
                                                  T* __restrict__ y, const T* __restrict__ expo,
@@ -661,8 +717,9 @@ __global__ __launch_bounds__(NTO) void lp_fwd_out(LinGeom g, const cplx<T>* __re
   const int i = blockIdx.x;
   const int job = blockIdx.y;
   const int R = i + g.h + g.m;
+  const int lw = FOLD ? g.Lw : L;  // window rows and columns (Lw)
   int lo, nc;
-  cover_range(g, R, L, lo, nc);
+  cover_range(g, R, lw, lo, nc);
 
   // Twiddles and epilogue operands are fetched up front; their latency
   // overlaps the spectrum loads.
@@ -681,14 +738,14 @@ __global__ __launch_bounds__(NTO) void lp_fwd_out(LinGeom g, const cplx<T>* __re
   T ex_v[JPT], lam_v[JPT];
   int cnt_v[JPT];
   const int npair = (g.n + 1) / 2;
-  const cplx<T>* base = R2 + (int64_t)job * g.P * L * L2P;
+  const cplx<T>* base = R2 + (int64_t)job * g.P * lw * L2P;
 
   // One batch of V patch-column pairs.  The first batch (all of them for
   // n <= 2V) is peeled into straight-line code: inside the batch loop the
   // loads pending at its head (twiddles, epilogue operands) would be waited
   // for before the spectrum loads are issued.
   auto pass = [&](int q0, auto first) {
-    sum_pair_spectra<T, L, false>(buf, base, g, L, R, lo, nc, nullptr, q0, tid);
+    sum_pair_spectra<T, L, false>(buf, base, g, lw, R, lo, nc, nullptr, q0, tid);
     if constexpr (decltype(first)::value) {
 #pragma unroll
       for (int u = 0; u < TPT; ++u) {
@@ -716,14 +773,22 @@ __global__ __launch_bounds__(NTO) void lp_fwd_out(LinGeom g, const cplx<T>* __re
       if (j < g.W) {
         const int C = j + g.h + g.m;
         T s = T(0);
-        for (int iy = first_cover(C, L, g.d, rd); iy < g.n; ++iy) {
+        for (int iy = first_cover(C, lw, g.d, rd); iy < g.n; ++iy) {
           const int sy = start_y(g, iy);
           if (sy > C) break;
           const int c = C - sy;
           const int v = (iy >> 1) - q0;
-          if (c < L && v >= 0 && v < V) {
-            const cplx<T> z = buf[v * VS + c];
-            s += (iy & 1) ? z.im : z.re;
+          if (c < lw && v >= 0 && v < V) {
+            if constexpr (FOLD) {
+              const int cf = c - g.m + g.mf;
+              cplx<T> z = buf[v * VS + cf];
+              if (cf + lw < L) z = cadd(z, buf[v * VS + cf + lw]);
+              if (cf >= lw) z = cadd(z, buf[v * VS + cf - lw]);
+              s += (iy & 1) ? z.im : z.re;
+            } else {
+              const cplx<T> z = buf[v * VS + c];
+              s += (iy & 1) ? z.im : z.re;
+            }
           }
         }
         acc[t] += s;
@@ -879,7 +944,7 @@ __global__ __launch_bounds__(NTO) void lp_adj_out(LinGeom g, const cplx<T>* __re
           const int c = Cp - sy;
           const int v = (iy >> 1) - q0;
           if (c < g.w && v >= 0 && v < V) {
-            const cplx<T> z = buf[v * VS + g.m + c];
+            const cplx<T> z = buf[v * VS + g.mf + c];
             s += tent_s[c] * ((iy & 1) ? z.im : z.re);
           }
         }
@@ -974,11 +1039,20 @@ struct LinOps {
     return launch_status("linpatch_prepare");
   }
 
+  // Fold mode (g.Lw != L) runs the full tiles only.
+  template <class F>
+  static void with_fold(const LinGeom& g, F&& f) {
+    if (g.Lw != L)
+      f(std::true_type{});
+    else
+      f(std::false_type{});
+  }
+
   static void fwd_front(const LinGeom& g, const LinBuffers& b, const void* x, double scale,
                         hipStream_t s) {
     {
       TimedLaunch _t("lp_rows_fwd", s);
-      with_rows(b, [&](auto vr) {
+      with_rows(g, b, [&](auto vr) {
         constexpr int VRC = decltype(vr)::value;
         dim3 gr((g.w / 2 + VRC - 1) / VRC, g.P, g.S * g.J);
         lp_rows<T, L, ROW_SIGNAL, VRC><<<gr, NT, 0, s>>>(g, (const T*)x, (const T*)b.tent,
@@ -986,54 +1060,66 @@ struct LinOps {
       });
     }
     TimedLaunch _t("lp_cols_fwd", s);
-    with_cols(b, [&](auto vc) {
-      constexpr int VC = decltype(vc)::value;
-      const int gc = g.S * ((LH + VC - 1) / VC) * g.P * g.J;
-      lp_cols<T, L, COL_FWD, VC><<<gc, NT, 0, s>>>(g, (const C*)b.R1, (const C*)b.KT,
-                                                   (const C*)b.tw, (C*)b.R2, (C*)b.KB,
-                                                   NllFinal{});
+    with_fold(g, [&](auto fold) {
+      constexpr bool FOLD = decltype(fold)::value;
+      with_cols(g, b, [&](auto vc) {
+        constexpr int VC = decltype(vc)::value;
+        const int gc = g.S * ((LH + VC - 1) / VC) * g.P * g.J;
+        lp_cols<T, L, COL_FWD, VC, FOLD><<<gc, NT, 0, s>>>(g, (const C*)b.R1, (const C*)b.KT,
+                                                           (const C*)b.tw, (C*)b.R2, (C*)b.KB,
+                                                           NllFinal{});
+      });
     });
   }
 
   static void adj_front(const LinGeom& g, const LinBuffers& b, const void* gy, hipStream_t s,
                         NllFinal fin = NllFinal{}) {
-    {
-      TimedLaunch _t("lp_rows_adj", s);
-      with_rows(b, [&](auto vr) {
-        constexpr int VRC = decltype(vr)::value;
-        dim3 gr((L / 2 + VRC - 1) / VRC, g.P, g.S * g.J);
-        lp_rows<T, L, ROW_COTANGENT, VRC><<<gr, NT, 0, s>>>(g, (const T*)gy, (const T*)b.tent,
-                                                            (const C*)b.tw, (C*)b.R1, T(1), L);
+    with_fold(g, [&](auto fold) {
+      constexpr bool FOLD = decltype(fold)::value;
+      {
+        TimedLaunch _t("lp_rows_adj", s);
+        with_rows(g, b, [&](auto vr) {
+          constexpr int VRC = decltype(vr)::value;
+          dim3 gr((g.Lw / 2 + VRC - 1) / VRC, g.P, g.S * g.J);
+          lp_rows<T, L, ROW_COTANGENT, VRC, FOLD><<<gr, NT, 0, s>>>(
+              g, (const T*)gy, (const T*)b.tent, (const C*)b.tw, (C*)b.R1, T(1), g.Lw);
+        });
+      }
+      TimedLaunch _t("lp_cols_adj", s);
+      with_cols(g, b, [&](auto vc) {
+        constexpr int VC = decltype(vc)::value;
+        const int gc = g.S * ((LH + VC - 1) / VC) * g.P * g.J;
+        lp_cols<T, L, COL_ADJ, VC, FOLD><<<gc, NT, 0, s>>>(g, (const C*)b.R1, (const C*)b.KT,
+                                                           (const C*)b.tw, (C*)b.R2, (C*)b.KB,
+                                                           fin);
       });
-    }
-    TimedLaunch _t("lp_cols_adj", s);
-    with_cols(b, [&](auto vc) {
-      constexpr int VC = decltype(vc)::value;
-      const int gc = g.S * ((LH + VC - 1) / VC) * g.P * g.J;
-      lp_cols<T, L, COL_ADJ, VC><<<gc, NT, 0, s>>>(g, (const C*)b.R1, (const C*)b.KT,
-                                                   (const C*)b.tw, (C*)b.R2, (C*)b.KB, fin);
     });
   }
 
   // Row tile of the signal and cotangent row passes: VR / b.row_div row
-  // pairs (row_div 1, 2 or 4; at least one pair).
+  // pairs (row_div 1, 2 or 4; at least one pair; fold mode: the full tile).
   template <class F>
-  static void with_rows(const LinBuffers& b, F&& f) {
-    if constexpr (VR / 4 >= 1)
-      if (b.row_div == 4) return f(std::integral_constant<int, VR / 4>{});
-    if constexpr (VR / 2 >= 1)
-      if (b.row_div == 2) return f(std::integral_constant<int, VR / 2>{});
+  static void with_rows(const LinGeom& g, const LinBuffers& b, F&& f) {
+    if (g.Lw == L) {
+      if constexpr (VR / 4 >= 1)
+        if (b.row_div == 4) return f(std::integral_constant<int, VR / 4>{});
+      if constexpr (VR / 2 >= 1)
+        if (b.row_div == 2) return f(std::integral_constant<int, VR / 2>{});
+    }
     f(std::integral_constant<int, VR>{});
   }
 
   // Column tile of the forward and adjoint column passes: V / b.col_div
-  // columns (col_div 1, 2 or 4; at least 2 columns).
+  // columns (col_div 1, 2 or 4; at least 2 columns; fold mode: the full
+  // tile).
   template <class F>
-  static void with_cols(const LinBuffers& b, F&& f) {
-    if constexpr (V / 4 >= 2)
-      if (b.col_div == 4) return f(std::integral_constant<int, V / 4>{});
-    if constexpr (V / 2 >= 2)
-      if (b.col_div == 2) return f(std::integral_constant<int, V / 2>{});
+  static void with_cols(const LinGeom& g, const LinBuffers& b, F&& f) {
+    if (g.Lw == L) {
+      if constexpr (V / 4 >= 2)
+        if (b.col_div == 4) return f(std::integral_constant<int, V / 4>{});
+      if constexpr (V / 2 >= 2)
+        if (b.col_div == 2) return f(std::integral_constant<int, V / 2>{});
+    }
     f(std::integral_constant<int, V>{});
   }
 
@@ -1053,9 +1139,12 @@ struct LinOps {
                       const int* counts, T* resid, double* partial, hipStream_t s,
                       const char* name) {
     TimedLaunch _t(name, s);
-    with_jpt(g, [&](auto jpt) {
-      lp_fwd_out<T, L, EPI, decltype(jpt)::value><<<dim3(g.H, g.S * g.J), NTO, 0, s>>>(
-          g, (const C*)b.R2, (const C*)b.tw, y, expo, counts, resid, partial);
+    with_fold(g, [&](auto fold) {
+      constexpr bool FOLD = decltype(fold)::value;
+      with_jpt(g, [&](auto jpt) {
+        lp_fwd_out<T, L, EPI, decltype(jpt)::value, FOLD><<<dim3(g.H, g.S * g.J), NTO, 0, s>>>(
+            g, (const C*)b.R2, (const C*)b.tw, y, expo, counts, resid, partial);
+      });
     });
   }
 
@@ -1069,105 +1158,45 @@ struct LinOps {
     });
   }
 
-  static int forward(const LinGeom& g, const LinBuffers& b, const void* x, void* y, double scale,
-                     hipStream_t s) {
-    fwd_front(g, b, x, scale, s);
-    fwd_out<0>(g, b, (T*)y, nullptr, nullptr, nullptr, nullptr, s, "lp_fwd_out");
-    return launch_status("linpatch_forward");
-  }
-
-  static int forward_poisson(const LinGeom& g, const LinBuffers& b, const void* x, double scale,
-                             const void* expo, const int* counts, void* resid, void* lam,
-                             double* partial, hipStream_t s) {
-    fwd_front(g, b, x, scale, s);
-    fwd_out<1>(g, b, (T*)lam, (const T*)expo, counts, (T*)resid, partial, s,
-               "lp_fwd_out_poisson");
-    return launch_status("linpatch_forward_poisson");
-  }
-
-  static int forward_metric(const LinGeom& g, const LinBuffers& b, const void* dx, double scale,
-                            const void* expo, const int* counts, const void* lam, void* cot,
-                            hipStream_t s) {
-    fwd_front(g, b, dx, scale, s);
-    fwd_out<2>(g, b, (T*)lam, (const T*)expo, counts, (T*)cot, nullptr, s, "lp_fwd_out_metric");
-    return launch_status("linpatch_forward_metric");
-  }
-
-  static int adjoint(const LinGeom& g, const LinBuffers& b, const void* gy, void* gx,
-                     double scale, hipStream_t s) {
-    adj_front(g, b, gy, s);
-    adj_out<0>(g, b, g.S * (g.x_shared ? g.E : g.J), (T*)gx, scale, nullptr, nullptr, nullptr, s,
-               "lp_adj_out");
-    return launch_status("linpatch_adjoint");
-  }
-
-  static int adjoint_gp(const LinGeom& g, const LinBuffers& b, const void* gy, double scale,
-                        const void* sky, const void* dnu, void* u, hipStream_t s, NllFinal fin) {
-    adj_front(g, b, gy, s, fin);
-    adj_out<1>(g, b, g.S * g.E, nullptr, scale, (const T*)sky, (const T*)dnu, (C*)u, s,
-               "lp_adj_out_gp");
-    return launch_status("linpatch_adjoint_gp");
+  static int run(const LinGeom& g, const LinBuffers& b, const LinCall& c, hipStream_t s) {
+    switch (c.op) {
+      case LinCall::kPrepare:
+        return prepare(g, b, c.in, c.layout, s);
+      case LinCall::kForward:
+        fwd_front(g, b, c.in, c.scale, s);
+        fwd_out<0>(g, b, (T*)c.out, nullptr, nullptr, nullptr, nullptr, s, "lp_fwd_out");
+        return launch_status("linpatch_forward");
+      case LinCall::kForwardPoisson:
+        fwd_front(g, b, c.in, c.scale, s);
+        fwd_out<1>(g, b, (T*)c.lam, (const T*)c.expo, c.counts, (T*)c.resid, c.partial, s,
+                   "lp_fwd_out_poisson");
+        return launch_status("linpatch_forward_poisson");
+      case LinCall::kForwardMetric:
+        fwd_front(g, b, c.in, c.scale, s);
+        fwd_out<2>(g, b, (T*)c.lam, (const T*)c.expo, c.counts, (T*)c.out, nullptr, s,
+                   "lp_fwd_out_metric");
+        return launch_status("linpatch_forward_metric");
+      case LinCall::kAdjoint:
+        adj_front(g, b, c.in, s);
+        adj_out<0>(g, b, g.S * (g.x_shared ? g.E : g.J), (T*)c.out, c.scale, nullptr, nullptr,
+                   nullptr, s, "lp_adj_out");
+        return launch_status("linpatch_adjoint");
+      case LinCall::kAdjointGp:
+        adj_front(g, b, c.in, s, c.fin);
+        adj_out<1>(g, b, g.S * g.E, nullptr, c.scale, (const T*)c.sky, (const T*)c.dnu,
+                   (C*)c.out, s, "lp_adj_out_gp");
+        return launch_status("linpatch_adjoint_gp");
+      default:
+        return fail(kInvalidArgument, "unknown linpatch operation");
+    }
   }
 };
 
-template <template <typename, int> class Ops, typename T, typename F>
-static int dispatch_len(int L, F&& f) {
-  switch (L) {
-#define JUBIK_CASE(l) \
-  case l:             \
-    return f(Ops<T, l>{});
-    JUBIK_LP_LENGTHS(JUBIK_CASE)
-#undef JUBIK_CASE
-    default:
-      return fail(kUnsupported, "FFT length " + std::to_string(L) + " is not compiled in");
-  }
-}
-
-// Entry points for one element type; linpatch.hip dispatches on dtype and
-// linpatch_f32.hip / linpatch_f64.hip instantiate them.
-template <typename T>
-int lp_prepare_t(const LinGeom& g, const LinBuffers& b, const void* kernel, int layout,
-                 hipStream_t s) {
-  return dispatch_len<LinOps, T>(g.L, [&](auto ops) {
-    return decltype(ops)::prepare(g, b, kernel, layout, s);
-  });
-}
-template <typename T>
-int lp_forward_t(const LinGeom& g, const LinBuffers& b, const void* x, void* y, double scale,
-                 hipStream_t s) {
-  return dispatch_len<LinOps, T>(g.L, [&](auto ops) {
-    return decltype(ops)::forward(g, b, x, y, scale, s);
-  });
-}
-template <typename T>
-int lp_forward_poisson_t(const LinGeom& g, const LinBuffers& b, const void* x, double scale,
-                         const void* expo, const int* counts, void* resid, void* lam,
-                         double* partial, hipStream_t s) {
-  return dispatch_len<LinOps, T>(g.L, [&](auto ops) {
-    return decltype(ops)::forward_poisson(g, b, x, scale, expo, counts, resid, lam, partial, s);
-  });
-}
-template <typename T>
-int lp_forward_metric_t(const LinGeom& g, const LinBuffers& b, const void* dx, double scale,
-                        const void* expo, const int* counts, const void* lam, void* cot,
-                        hipStream_t s) {
-  return dispatch_len<LinOps, T>(g.L, [&](auto ops) {
-    return decltype(ops)::forward_metric(g, b, dx, scale, expo, counts, lam, cot, s);
-  });
-}
-template <typename T>
-int lp_adjoint_t(const LinGeom& g, const LinBuffers& b, const void* gy, void* gx, double scale,
-                 hipStream_t s) {
-  return dispatch_len<LinOps, T>(g.L, [&](auto ops) {
-    return decltype(ops)::adjoint(g, b, gy, gx, scale, s);
-  });
-}
-template <typename T>
-int lp_adjoint_gp_t(const LinGeom& g, const LinBuffers& b, const void* gy, double scale,
-                    const void* sky, const void* dnu, void* u, hipStream_t s, NllFinal fin) {
-  return dispatch_len<LinOps, T>(g.L, [&](auto ops) {
-    return decltype(ops)::adjoint_gp(g, b, gy, scale, sky, dnu, u, s, fin);
-  });
+// One entry per (element type, FFT length); the lp_*.hip translation units
+// instantiate them in groups of lengths, linpatch.hip dispatches.
+template <typename T, int L>
+int lp_run(const LinGeom& g, const LinBuffers& b, const LinCall& c, hipStream_t s) {
+  return LinOps<T, L>::run(g, b, c, s);
 }
 
 }  // namespace jubik

@@ -1,0 +1,10 @@
+// float instantiations of the linear-patching kernels for the FFT lengths of
+// group D (JUBIK_LP_LENGTHS_D; see linpatch_impl.hpp).
+#include "linpatch_impl.hpp"
+
+namespace jubik {
+#define JUBIK_INST(l) \
+  template int lp_run<float, l>(const LinGeom&, const LinBuffers&, const LinCall&, hipStream_t);
+JUBIK_LP_LENGTHS_D(JUBIK_INST)
+#undef JUBIK_INST
+}  // namespace jubik

@@ -109,7 +109,8 @@ class Poissonian:
         if f is None:
             return self._data_metric(primals, tangents, False)
         lam = f(primals)
-        return f.vjp(primals, self._data_metric(lam, f.jvp(primals, tangents), False))
+        out = f.vjp(primals, self._data_metric(lam, f.jvp(primals, tangents), False))
+        return dev.like_input(out, tangents)
 
     def left_sqrt_metric(self, primals, tangents):
         """Left square root of the metric applied to data-space ``tangents``:
@@ -119,4 +120,5 @@ class Poissonian:
         f = self._linearised()
         if f is None:
             return self._data_metric(primals, tangents, True)
-        return f.vjp(primals, self._data_metric(f(primals), tangents, True))
+        out = f.vjp(primals, self._data_metric(f(primals), tangents, True))
+        return dev.like_input(out, tangents)

@@ -22,7 +22,7 @@ def _rel(a, b):
 
 @pytest.mark.parametrize("dtype,tol", [(torch.float64, 1e-10), (torch.float32, 1e-5)])
 def test_seven_modules_spectral_vs_oracle(ja, dtype, tol):
-    prob, xi = make_problem(H=306, n=3, margin=60, E=3, M=7, seed=3, sigma=(2.0, 6.0),
+    prob, xi = make_problem(H=306, n=3, margin=60, E=3, M=7, seed=3, N=512, sigma=(2.0, 6.0),
                             expo_range=(1e2, 3e3), mask_frac=0.08)
     want_nll, (wr, wa) = om.nll_and_grad(prob, *xi)
     model = to_product(ja, prob, dtype)

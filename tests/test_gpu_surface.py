@@ -43,7 +43,7 @@ def test_pixel_area_gradient_and_metric(ja, dtype, tol):
 
 @pytest.mark.parametrize("dtype,tol", [(torch.float64, 1e-10), (torch.float32, 1e-5)])
 def test_pixel_area_hyper_and_batch(ja, dtype, tol):
-    prob, xi = make_problem(H=64, n=4, margin=8, E=2, M=1, seed=32, N=96, pixel_area=AREA)
+    prob, xi = make_problem(H=64, n=4, margin=8, E=2, M=1, seed=32, N=128, pixel_area=AREA)
     want = om.nll_and_grad_hyper(prob, *xi)
     model = to_product(ja, prob, dtype, max_samples=2)
     got = model.nll_and_grad_hyper(_dev(xi[0], dtype), _dev(xi[1], dtype))

@@ -46,9 +46,9 @@ def test_invalid_arguments_fail_before_device_work(ja):
     # bad dtype
     st = lib.jubik_linpatch_create(ctypes.byref(plan), 7, 64, 64, 1, 1, 4, 8, 1, 1)
     assert st == _lib.EINVAL
-    # FFT length not compiled in: H=64, n=4, margin 9 -> L = 50
-    st = lib.jubik_linpatch_create(ctypes.byref(plan), 0, 64, 64, 1, 1, 4, 9, 1, 1)
-    assert st == _lib.EUNSUPPORTED and b"FFT length 50" in lib.jubik_last_error()
+    # no compiled FFT length: window 2 * 1024 needs at least 4096 (fold mode)
+    st = lib.jubik_linpatch_create(ctypes.byref(plan), 0, 2048, 2048, 1, 1, 2, 9, 1, 1)
+    assert st == _lib.EUNSUPPORTED and b"window 2048" in lib.jubik_last_error()
     # unequal strides
     st = lib.jubik_linpatch_create(ctypes.byref(plan), 0, 64, 96, 1, 1, 4, 8, 1, 1)
     assert st == _lib.EINVAL
@@ -120,7 +120,8 @@ def test_last_error_is_per_thread(ja):
         plan = ctypes.c_void_p()
         if name == "b":
             barrier.wait()  # fail after thread a
-        st = lib.jubik_linpatch_create(ctypes.byref(plan), 0, 64, 64, 1, 1, 4, margin, 1, 1)
+        H, n = (64, 4) if name == "a" else (2048, 2)
+        st = lib.jubik_linpatch_create(ctypes.byref(plan), 0, H, H, 1, 1, n, margin, 1, 1)
         if name == "a":
             barrier.wait()
         barrier.wait()  # both have failed
@@ -133,7 +134,7 @@ def test_last_error_is_per_thread(ja):
     for t in ts:
         t.join(timeout=30)
     assert seen["a"][0] == _lib.EINVAL and b"margin" in seen["a"][1]
-    assert seen["b"][0] == _lib.EUNSUPPORTED and b"FFT length 50" in seen["b"][1]
+    assert seen["b"][0] == _lib.EUNSUPPORTED and b"window 2048" in seen["b"][1]
 
 
 def test_tile_cache_round_trip(ja, tmp_path):
