@@ -325,6 +325,26 @@ typedef struct jubik_grad_desc {
   void* g_alpha;
 } jubik_grad_desc;
 int jubik_graph_capture_batch(jubik_graph** graph, const jubik_grad_desc* descs, int64_t count);
+/* Metric products of several models in one graph: each descriptor is one
+ * jubik_model_metric_batch call (S tangents at that model's base point),
+ * captured as parallel branches, e.g. the S = 32 antithetic metric products
+ * of three observations on uneven grids per CG iteration of a sample draw
+ * (demos/x-ray-imaging.py:416-419).  Descriptors may not share a model or
+ * a plan; at most 64; each model needs a base point at capture and keeps
+ * the one it has at launch. */
+typedef struct jubik_metric_desc {
+  jubik_model* model;
+  int64_t samples;
+  const void* amp_ref;
+  const void* amp_alpha;
+  const void* dnu;
+  const void* v_ref;
+  const void* v_alpha;
+  void* out_ref;
+  void* out_alpha;
+} jubik_metric_desc;
+int jubik_graph_capture_metric_batch(jubik_graph** graph, const jubik_metric_desc* descs,
+                                     int64_t count);
 /* Left square root of the likelihood metric at the base point, for S noise
  * vectors: out = J^T diag(lam^-1/2) noise, noise (S, M, E, H, W) in data
  * space (ignored where counts < 0).  With noise ~ N(0, 1) this draws from
@@ -334,6 +354,46 @@ int jubik_graph_capture_batch(jubik_graph** graph, const jubik_grad_desc* descs,
 int jubik_model_sqrt_metric_batch(jubik_model* model, int64_t S, const void* amp_ref,
                                   const void* amp_alpha, const void* dnu, const void* noise,
                                   void* out_ref, void* out_alpha, void* stream);
+/* The KL metric's likelihood part at the S sample positions of the last
+ * jubik_model_grad_batch: out = sum_s J_s^T diag(1/lam_s) J_s v for one
+ * tangent v (N x N fields), each J_s and lam_s at its own sample position
+ * (the metric nifty.re's optimize_kl minimises the KL with, Newton-CG
+ * minimizer "M", demos/x-ray-imaging.py:421-425).  Divide by S and add v
+ * for the KL metric of the standardised latents. */
+int jubik_model_metric_samples(jubik_model* model, int64_t S, const void* amp_ref,
+                               const void* amp_alpha, const void* dnu, const void* v_ref,
+                               const void* v_alpha, void* out_ref, void* out_alpha,
+                               void* stream);
+/* Device-resident conjugate gradients (the CG of nifty.re's sample draws
+ * and Newton steps, demos/x-ray-imaging.py:416-425): every scalar stays in
+ * device memory, dot products are FP64 reductions in a fixed order, and an
+ * iteration is a fixed sequence of kernel launches with no host
+ * synchronisation, so a solve can be captured in a graph when
+ * check_every = 0 (then exactly maxiter iterations are launched; systems
+ * that have met their stopping rule take steps of length 0).  With
+ * check_every = k > 0 the stream is synchronised every k iterations to stop
+ * early once every system has converged.  A system stops when
+ * |r| <= rtol |b|, or (absdelta > 0) when the energy 1/2 x.A.x - b.x
+ * changes by at most absdelta in one iteration (cg_kwargs absdelta of the
+ * demo).  info (device, may be NULL) receives 2 doubles per system:
+ * iterations and |r| / |b|.  x may not alias b.  The solver workspace (four
+ * vectors of max_samples x 2 x N^2 elements and the scalars) is allocated
+ * once by jubik_model_reserve_solver, a set-up call.
+ * jubik_model_metric_solve: S systems (I + J^T diag(1/lam) J) x_s = b_s at
+ * the base point (the linear sample draws).  jubik_model_kl_solve: one
+ * system (I + (1/S) sum_s J_s^T diag(1/lam_s) J_s) x = b at the S positions
+ * of the last jubik_model_grad_batch (the Newton step of the KL). */
+int jubik_model_reserve_solver(jubik_model* model);
+int jubik_model_metric_solve(jubik_model* model, int64_t S, const void* amp_ref,
+                             const void* amp_alpha, const void* dnu, const void* b_ref,
+                             const void* b_alpha, void* x_ref, void* x_alpha, int maxiter,
+                             double rtol, double absdelta, int check_every, double* info,
+                             void* stream);
+int jubik_model_kl_solve(jubik_model* model, int64_t S, const void* amp_ref,
+                         const void* amp_alpha, const void* dnu, const void* b_ref,
+                         const void* b_alpha, void* x_ref, void* x_alpha, int maxiter,
+                         double rtol, double absdelta, int check_every, double* info,
+                         void* stream);
 
 #ifdef __cplusplus
 }

@@ -12,12 +12,13 @@ from .convolve import (LinpatchOperator, convolve, integrate, linpatch_convolve,
 from .response import build_exposure_function, build_readout_function
 from .likelihood import Poissonian
 from .sky import CorrelatedFieldSky, powerlaw_amplitude, powerlaw_amplitude_grad
-from .model import XrayModel, capture_grad_batch, dense_counts
+from .model import XrayModel, capture_grad_batch, capture_metric_batch, dense_counts
 from .sampling import conjugate_gradient, draw_linear_samples, metric_solve
+from .kl import kl_value_and_grad, newton_cg, optimize_kl
 from .mock import create_mock_counts, poisson_sample
 from .psf import psf_interpolator
 from .dft import DFT128
-from . import synthetic
+from . import priors, synthetic
 from .autograd import linpatch_apply, xray_nll
 
 __all__ = [
@@ -25,6 +26,7 @@ __all__ = [
     "slice_patches", "slice_patches_transpose",
     "linpatch_convolve_modules", "build_exposure_function", "build_readout_function",
     "Poissonian", "CorrelatedFieldSky", "powerlaw_amplitude", "XrayModel", "dense_counts",
-    "conjugate_gradient", "draw_linear_samples", "metric_solve", "create_mock_counts",
+    "conjugate_gradient", "draw_linear_samples", "metric_solve", "kl_value_and_grad",
+    "newton_cg", "optimize_kl", "create_mock_counts",
     "poisson_sample", "psf_interpolator", "DFT128",
 ]

@@ -62,11 +62,18 @@ SIGNATURES = {
                                _vp],
     "jubik_model_metric_batch": [_vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
     "jubik_model_sqrt_metric_batch": [_vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
+    "jubik_model_metric_samples": [_vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
+    "jubik_model_reserve_solver": [_vp],
+    "jubik_model_metric_solve": [_vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _int, _dbl, _dbl,
+                                 _int, _vp, _vp],
+    "jubik_model_kl_solve": [_vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _int, _dbl, _dbl,
+                             _int, _vp, _vp],
     "jubik_poisson_sample": [_int, _vp, _i64, C.c_uint64, _vp, _vp],
     "jubik_model_grad_graph": [C.POINTER(_vp), _vp, _i64, _vp, _vp, _vp, _vp, _dbl, _dbl, _vp,
                                _vp, _vp, _vp],
     "jubik_graph_launch": [_vp, _vp],
     "jubik_graph_capture_batch": [C.POINTER(_vp), _vp, _i64],
+    "jubik_graph_capture_metric_batch": [C.POINTER(_vp), _vp, _i64],
     "jubik_graph_destroy": [_vp],
     "jubik_dft128_create": [C.POINTER(_vp)],
     "jubik_dft128_run": [_vp, _int, _vp, _vp, _i64, _vp],
@@ -82,6 +89,13 @@ class GradDesc(C.Structure):
     _fields_ = [("model", _vp), ("samples", _i64), ("amp_ref", _vp), ("amp_alpha", _vp),
                 ("xi_ref", _vp), ("xi_alpha", _vp), ("zm", _dbl), ("mu", _dbl), ("dnu", _vp),
                 ("nll", _vp), ("g_ref", _vp), ("g_alpha", _vp)]
+
+
+class MetricDesc(C.Structure):
+    """jubik_metric_desc: one metric batch of a multi-model graph."""
+    _fields_ = [("model", _vp), ("samples", _i64), ("amp_ref", _vp), ("amp_alpha", _vp),
+                ("dnu", _vp), ("v_ref", _vp), ("v_alpha", _vp), ("out_ref", _vp),
+                ("out_alpha", _vp)]
 
 
 _lock = threading.Lock()

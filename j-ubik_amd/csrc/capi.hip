@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "../../include/jubik_hip.h"
+#include "cg.hpp"
 #include "gp.hpp"
 #include "linpatch.hpp"
 #include "dft128.hpp"
@@ -198,7 +199,10 @@ struct jubik_model {
   void* partial = nullptr;  // 2 x max_samples x J x H doubles (NLL terms, lam <= 0 counts)
   void* nbad = nullptr;     // max_samples doubles: read-out pixels with lam <= 0
   int last_S = 0;           // samples of the last gradient evaluation
-  void* u = nullptr;        // max_samples x E x H x W complex
+  int batch_S = 0;          // samples of the last batched gradient (slots 1 .. S)
+  void* cg_vec = nullptr;   // solver vectors r, p, Ap, Mv: 4 x max_samples x 2 x N^2
+  void* cg_work = nullptr;  // solver scalars and partial sums (doubles)
+  void* u = nullptr;        // max_samples x E x H x W (cotangent planes of log s)
   void* R1 = nullptr;       // own linpatch workspaces (max_samples > 1)
   void* R2 = nullptr;
   void* gwork = nullptr;    // own GP workspace (max_samples > 1)
@@ -212,6 +216,8 @@ struct jubik_model {
     dev_free(lam);
     dev_free(partial);
     dev_free(nbad);
+    dev_free(cg_vec);
+    dev_free(cg_work);
     dev_free(u);
     dev_free(R1);
     dev_free(R2);
@@ -804,7 +810,7 @@ int jubik_gp_create(jubik_gp** plan, int dtype, int64_t N, int64_t H, int64_t W,
   const size_t cs = 2 * elem_size(dtype);
   int st = upload_twiddles(&p->tw, dtype, (int)N);
   if (!st) st = dev_alloc(&p->work, (size_t)N * N * cs, "GP workspace");
-  if (!st) st = dev_alloc(&p->u, (size_t)H * W * cs, "GP cotangent");
+  if (!st) st = dev_alloc(&p->u, (size_t)E * H * W * elem_size(dtype), "GP cotangent planes");
   if (st) {
     p->release();
     delete p;
@@ -843,9 +849,11 @@ int jubik_gp_vjp(jubik_gp* plan, const void* amp_ref, const void* amp_alpha, con
   if (plan->g.has_alpha && (!amp_alpha || !dnu || !g_alpha))
     return fail(kInvalidArgument, "spectral-index inputs are NULL");
   DeviceGuard dg(plan->device);
-  int st = gp_fold(plan->dtype, plan->g, sky, cot_sky, dnu, plan->u, as_stream(stream));
+  int st = gp_fold(plan->dtype, plan->g, sky, cot_sky, plan->u, as_stream(stream));
   if (st) return st;
-  return gp_adjoint_u(plan->dtype, plan->g, plan->bufs(), amp_ref, amp_alpha, plan->u, g_ref,
+  GpGeom gg = plan->g;
+  gg.uplanes = gg.E;
+  return gp_adjoint_u(plan->dtype, gg, plan->bufs(), amp_ref, amp_alpha, dnu, plan->u, g_ref,
                       g_alpha, as_stream(stream));
 }
 
@@ -877,7 +885,7 @@ int jubik_model_create_batched(jubik_model** model, jubik_gp* gp, jubik_linpatch
   if (!st) st = dev_alloc(&m->resid, img * g.J * S * es, "residual");
   if (!st) st = dev_alloc(&m->partial, 2 * (size_t)g.J * g.H * S * sizeof(double), "NLL partials");
   if (!st) st = dev_alloc(&m->nbad, S * sizeof(double), "lambda domain counts");
-  if (!st) st = dev_alloc(&m->u, img * g.E * 2 * S * es, "GP cotangent planes");
+  if (!st) st = dev_alloc(&m->u, img * g.E * S * es, "GP cotangent planes");
   if (!st && S > 1) {
     st = dev_alloc(&m->R1, lp->ws_bytes * S, "batched workspace R1");
     if (!st) st = dev_alloc(&m->R2, lp->ws_bytes * S, "batched workspace R2");
@@ -944,11 +952,12 @@ int model_grad(jubik_model* model, int S, int slot0, const void* amp_ref, const 
   fin.out = nll;
   fin.bad_out = (double*)model->nbad;
   model->last_S = S;
+  if (slot0 == 1) model->batch_S = S;
   st = linpatch_adjoint_gp(lp->dtype, g, model->lbufs(), model->resid, model->pixel_area, sky, dnu,
                            model->u, s, fin);
   if (st) return st;
   gg.uplanes = g.E;
-  return gp_adjoint_u(gp->dtype, gg, model->gbufs(), amp_ref, amp_alpha, model->u, g_ref, g_alpha,
+  return gp_adjoint_u(gp->dtype, gg, model->gbufs(), amp_ref, amp_alpha, dnu, model->u, g_ref, g_alpha,
                       s, hy);
 }
 
@@ -1206,41 +1215,194 @@ int jubik_graph_destroy(jubik_graph* graph) {
   return kOk;
 }
 
-int jubik_model_metric_batch(jubik_model* model, int64_t S, const void* amp_ref,
-                             const void* amp_alpha, const void* dnu, const void* v_ref,
-                             const void* v_alpha, void* out_ref, void* out_alpha, void* stream) {
-  if (!model || !v_ref || !out_ref || !amp_ref) return fail(kInvalidArgument, "NULL argument");
+namespace {
+
+// Metric products of the model.  per_sample = false: S tangents v[s] at the
+// base point of the last jubik_model_grad (out[s] = J^T diag(1/lam) J v[s]).
+// per_sample = true: one tangent v at each of the S positions of the last
+// jubik_model_grad_batch, summed: out = sum_s J_s^T diag(1/lam_s) J_s v.
+int model_metric(jubik_model* model, int S, bool per_sample, const void* amp_ref,
+                 const void* amp_alpha, const void* dnu, const void* v_ref, const void* v_alpha,
+                 void* out_ref, void* out_alpha, hipStream_t s) {
+  jubik_linpatch* lp = model->lp;
+  jubik_gp* gp = model->gp;
+  GpGeom gg = gp->g;
+  gg.S = S;
+  gg.sky_shared = per_sample ? 0 : 1;
+  gg.work_shared = per_sample ? 1 : 0;
+  LinGeom g = lp->g;
+  g.S = S;
+  g.sky_shared = per_sample ? 0 : 1;
+  const int slot = per_sample ? 1 : 0;
+  const void* sky = model->sky_slot(slot);
+  int st = gp_tangent(gp->dtype, gg, model->gbufs(), amp_ref, amp_alpha, v_ref, v_alpha, dnu, sky,
+                      model->dsky, s);
+  if (st) return st;
+  st = linpatch_forward_metric(lp->dtype, g, model->lbufs(), model->dsky, model->pixel_area,
+                               model->expo, model->counts, model->lam_slot(slot), model->resid, s);
+  if (st) return st;
+  st = linpatch_adjoint_gp(lp->dtype, g, model->lbufs(), model->resid, model->pixel_area, sky, dnu,
+                           model->u, s);
+  if (st) return st;
+  if (per_sample) {
+    // one GP adjoint of the cotangent planes of every sample (it is linear)
+    gg.S = 1;
+    gg.uplanes = S * g.E;
+  } else {
+    gg.uplanes = g.E;
+  }
+  return gp_adjoint_u(gp->dtype, gg, model->gbufs(), amp_ref, amp_alpha, dnu, model->u, out_ref,
+                      out_alpha, s);
+}
+
+int check_batch(const jubik_model* model, int64_t S) {
   if (S < 1 || S > model->max_samples)
     return fail(kInvalidArgument, "batch of " + std::to_string(S) +
                                       " samples outside [1, max_samples = " +
                                       std::to_string(model->max_samples) + "]");
-  int st = check_alpha(model, amp_alpha, v_alpha, out_alpha, dnu);
+  return kOk;
+}
+
+int check_samples_point(const jubik_model* model, int64_t S) {
+  if (model->batch_S < 1)
+    return fail(kNotReady, "no sample positions (call jubik_model_grad_batch first)");
+  if (S != model->batch_S)
+    return fail(kInvalidArgument, "the last batched gradient evaluated " +
+                                      std::to_string(model->batch_S) + " samples, not " +
+                                      std::to_string(S));
+  return kOk;
+}
+
+// Conjugate gradients on (I + M) x = b, M the metric of model_metric: S
+// systems at the base point (per_sample = false) or one system with M the
+// mean over the S sample positions (per_sample = true).
+int model_solve(jubik_model* model, int S, bool per_sample, const void* amp_ref,
+                const void* amp_alpha, const void* dnu, const void* b_ref, const void* b_alpha,
+                void* x_ref, void* x_alpha, int maxiter, double rtol, double absdelta,
+                int check_every, double* info, hipStream_t s) {
+  const int dtype = model->lp->dtype;
+  const int nsys = per_sample ? 1 : S;
+  const bool alpha = model->gp->g.has_alpha != 0;
+  const int64_t n = (int64_t)model->gp->g.N * model->gp->g.N;
+  const size_t es = elem_size(dtype);
+  const size_t vec = (size_t)model->max_samples * 2 * n * es;  // one solver vector
+  char* base = static_cast<char*>(model->cg_vec);
+  auto vecs = [&](int k) {
+    CgVec v;
+    v.f[0] = base + k * vec;
+    v.f[1] = alpha ? base + k * vec + (size_t)nsys * n * es : nullptr;
+    return v;
+  };
+  CgVec r = vecs(0), p = vecs(1), Ap = vecs(2), Mv = vecs(3);
+  CgVec b, x;
+  b.f[0] = const_cast<void*>(b_ref);
+  b.f[1] = const_cast<void*>(b_alpha);
+  x.f[0] = x_ref;
+  x.f[1] = x_alpha;
+  CgDims d{nsys, alpha ? 2 : 1, n, rtol, absdelta};
+  double* work = static_cast<double*>(model->cg_work);
+  int st = cg_init(dtype, d, b, x, r, p, work, s);
+  for (int it = 0; it < maxiter && !st; ++it) {
+    st = model_metric(model, S, per_sample, amp_ref, amp_alpha, dnu, p.f[0], p.f[1], Mv.f[0],
+                      Mv.f[1], s);
+    if (!st) st = cg_matvec(dtype, d, p, Mv, per_sample ? 1.0 / S : 1.0, Ap, work, s);
+    if (!st) st = cg_update(dtype, d, b, x, r, p, Ap, work, s);
+    if (!st && check_every > 0 && (it + 1) % check_every == 0 && it + 1 < maxiter) {
+      int active = 0;
+      st = cg_active_count(d, work, s, &active);
+      if (!st && active == 0) break;
+    }
+  }
+  if (!st && info) st = cg_finish(d, work, info, s);
+  return st;
+}
+
+int check_solve_args(const jubik_model* model, int maxiter, double rtol, int check_every) {
+  if (!model->cg_vec) return fail(kNotReady, "no solver workspace (call jubik_model_reserve_solver)");
+  if (maxiter < 0) return fail(kInvalidArgument, "maxiter must be non-negative");
+  if (!(rtol >= 0.0)) return fail(kInvalidArgument, "rtol must be non-negative");
+  if (check_every < 0) return fail(kInvalidArgument, "check_every must be non-negative");
+  return kOk;
+}
+
+}  // namespace
+
+int jubik_model_metric_batch(jubik_model* model, int64_t S, const void* amp_ref,
+                             const void* amp_alpha, const void* dnu, const void* v_ref,
+                             const void* v_alpha, void* out_ref, void* out_alpha, void* stream) {
+  if (!model || !v_ref || !out_ref || !amp_ref) return fail(kInvalidArgument, "NULL argument");
+  int st = check_batch(model, S);
+  if (!st) st = check_alpha(model, amp_alpha, v_alpha, out_alpha, dnu);
   if (st) return st;
   if (!model->have_point)
     return fail(kNotReady, "no base point (call jubik_model_grad first)");
-  jubik_linpatch* lp = model->lp;
-  jubik_gp* gp = model->gp;
-  DeviceGuard dg(lp->device);
-  hipStream_t s = as_stream(stream);
-  GpGeom gg = gp->g;
-  gg.S = (int)S;
-  gg.sky_shared = 1;
-  LinGeom g = lp->g;
-  g.S = (int)S;
-  g.sky_shared = 1;
-  const void* sky0 = model->sky_slot(0);
-  st = gp_tangent(gp->dtype, gg, model->gbufs(), amp_ref, amp_alpha, v_ref, v_alpha, dnu, sky0,
-                  model->dsky, s);
+  DeviceGuard dg(model->lp->device);
+  return model_metric(model, (int)S, false, amp_ref, amp_alpha, dnu, v_ref, v_alpha, out_ref,
+                      out_alpha, as_stream(stream));
+}
+
+int jubik_model_metric_samples(jubik_model* model, int64_t S, const void* amp_ref,
+                               const void* amp_alpha, const void* dnu, const void* v_ref,
+                               const void* v_alpha, void* out_ref, void* out_alpha,
+                               void* stream) {
+  if (!model || !v_ref || !out_ref || !amp_ref) return fail(kInvalidArgument, "NULL argument");
+  int st = check_batch(model, S);
+  if (!st) st = check_alpha(model, amp_alpha, v_alpha, out_alpha, dnu);
+  if (!st) st = check_samples_point(model, S);
   if (st) return st;
-  st = linpatch_forward_metric(lp->dtype, g, model->lbufs(), model->dsky, model->pixel_area,
-                               model->expo, model->counts, model->lam_slot(0), model->resid, s);
+  DeviceGuard dg(model->lp->device);
+  return model_metric(model, (int)S, true, amp_ref, amp_alpha, dnu, v_ref, v_alpha, out_ref,
+                      out_alpha, as_stream(stream));
+}
+
+int jubik_model_reserve_solver(jubik_model* model) {
+  if (!model) return fail(kInvalidArgument, "NULL argument");
+  if (model->cg_vec) return kOk;
+  DeviceGuard dg(model->lp->device);
+  const int64_t n = (int64_t)model->gp->g.N * model->gp->g.N;
+  const size_t vec = (size_t)model->max_samples * 2 * n * elem_size(model->lp->dtype);
+  int st = dev_alloc(&model->cg_vec, 4 * vec, "solver vectors");
+  if (!st)
+    st = dev_alloc(&model->cg_work, cg_work_doubles(model->max_samples) * sizeof(double),
+                   "solver scalars");
+  if (st) {
+    dev_free(model->cg_vec);
+    dev_free(model->cg_work);
+  }
+  return st;
+}
+
+int jubik_model_metric_solve(jubik_model* model, int64_t S, const void* amp_ref,
+                             const void* amp_alpha, const void* dnu, const void* b_ref,
+                             const void* b_alpha, void* x_ref, void* x_alpha, int maxiter,
+                             double rtol, double absdelta, int check_every, double* info,
+                             void* stream) {
+  if (!model || !b_ref || !x_ref || !amp_ref) return fail(kInvalidArgument, "NULL argument");
+  int st = check_batch(model, S);
+  if (!st) st = check_alpha(model, amp_alpha, b_alpha, x_alpha, dnu);
+  if (!st) st = check_solve_args(model, maxiter, rtol, check_every);
   if (st) return st;
-  st = linpatch_adjoint_gp(lp->dtype, g, model->lbufs(), model->resid, model->pixel_area, sky0,
-                           dnu, model->u, s);
+  if (!model->have_point)
+    return fail(kNotReady, "no base point (call jubik_model_grad first)");
+  DeviceGuard dg(model->lp->device);
+  return model_solve(model, (int)S, false, amp_ref, amp_alpha, dnu, b_ref, b_alpha, x_ref,
+                     x_alpha, maxiter, rtol, absdelta, check_every, info, as_stream(stream));
+}
+
+int jubik_model_kl_solve(jubik_model* model, int64_t S, const void* amp_ref,
+                         const void* amp_alpha, const void* dnu, const void* b_ref,
+                         const void* b_alpha, void* x_ref, void* x_alpha, int maxiter,
+                         double rtol, double absdelta, int check_every, double* info,
+                         void* stream) {
+  if (!model || !b_ref || !x_ref || !amp_ref) return fail(kInvalidArgument, "NULL argument");
+  int st = check_batch(model, S);
+  if (!st) st = check_alpha(model, amp_alpha, b_alpha, x_alpha, dnu);
+  if (!st) st = check_solve_args(model, maxiter, rtol, check_every);
+  if (!st) st = check_samples_point(model, S);
   if (st) return st;
-  gg.uplanes = g.E;
-  return gp_adjoint_u(gp->dtype, gg, model->gbufs(), amp_ref, amp_alpha, model->u, out_ref,
-                      out_alpha, s);
+  DeviceGuard dg(model->lp->device);
+  return model_solve(model, (int)S, true, amp_ref, amp_alpha, dnu, b_ref, b_alpha, x_ref,
+                     x_alpha, maxiter, rtol, absdelta, check_every, info, as_stream(stream));
 }
 
 int jubik_model_sqrt_metric_batch(jubik_model* model, int64_t S, const void* amp_ref,
@@ -1273,7 +1435,7 @@ int jubik_model_sqrt_metric_batch(jubik_model* model, int64_t S, const void* amp
   st = linpatch_adjoint_gp(lp->dtype, g, model->lbufs(), model->resid, model->pixel_area,
                            model->sky_slot(0), dnu, model->u, s);
   if (st) return st;
-  return gp_adjoint_u(gp->dtype, gg, model->gbufs(), amp_ref, amp_alpha, model->u, out_ref,
+  return gp_adjoint_u(gp->dtype, gg, model->gbufs(), amp_ref, amp_alpha, dnu, model->u, out_ref,
                       out_alpha, s);
 }
 
@@ -1304,6 +1466,82 @@ int jubik_model_check_domain(jubik_model* model, int64_t* counts, void* stream) 
     return fail(kDomain, "lambda <= 0 at read-out pixels (" + bad +
                              "): the Poisson NLL is inf or NaN there; mask pixels with zero "
                              "exposure");
+  return kOk;
+}
+
+int jubik_graph_capture_metric_batch(jubik_graph** out, const jubik_metric_desc* descs,
+                                     int64_t count) {
+  if (!out || !descs) return fail(kInvalidArgument, "NULL argument");
+  *out = nullptr;
+  if (count < 1 || count > 64) return fail(kInvalidArgument, "count must be in [1, 64]");
+  if (timing_enabled())
+    return fail(kInvalidArgument, "disable kernel timing before capturing a graph");
+  for (int64_t i = 0; i < count; ++i) {
+    const jubik_metric_desc& d = descs[i];
+    const std::string at = "descriptor " + std::to_string(i) + ": ";
+    if (!d.model || !d.v_ref || !d.out_ref || !d.amp_ref)
+      return fail(kInvalidArgument, at + "NULL argument");
+    int st = check_batch(d.model, d.samples);
+    if (!st) st = check_alpha(d.model, d.amp_alpha, d.v_alpha, d.out_alpha, d.dnu);
+    if (st) return st;
+    if (!d.model->have_point)
+      return fail(kNotReady, at + "no base point (call jubik_model_grad first)");
+    if (d.model->lp->device != descs[0].model->lp->device)
+      return fail(kInvalidArgument, at + "models on different devices");
+    for (int64_t k = 0; k < i; ++k) {
+      const jubik_model* o = descs[k].model;
+      if (o == d.model || o->lp == d.model->lp || o->gp == d.model->gp)
+        return fail(kInvalidArgument, at + "shares its model or a plan with descriptor " +
+                                          std::to_string(k));
+    }
+  }
+  DeviceGuard dg(descs[0].model->lp->device);
+  const int n = (int)count;
+  hipStream_t cs = nullptr;
+  std::vector<hipStream_t> br(n, nullptr);
+  std::vector<hipEvent_t> ev(n + 1, nullptr);
+  int st = check_hip(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking), "hipStreamCreate");
+  for (int i = 0; i < n && !st; ++i)
+    st = check_hip(hipStreamCreateWithFlags(&br[i], hipStreamNonBlocking), "hipStreamCreate");
+  for (int i = 0; i <= n && !st; ++i)
+    st = check_hip(hipEventCreateWithFlags(&ev[i], hipEventDisableTiming), "hipEventCreate");
+  auto* gr = new jubik_graph();
+  gr->device = descs[0].model->lp->device;
+  if (!st) st = check_hip(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal),
+                          "hipStreamBeginCapture");
+  if (!st) {
+    int cst = check_hip(hipEventRecord(ev[0], cs), "hipEventRecord");
+    for (int i = 0; i < n && !cst; ++i) {
+      const jubik_metric_desc& d = descs[i];
+      cst = check_hip(hipStreamWaitEvent(br[i], ev[0], 0), "hipStreamWaitEvent");
+      if (cst) break;
+      cst = model_metric(d.model, (int)d.samples, false, d.amp_ref, d.amp_alpha, d.dnu, d.v_ref,
+                         d.v_alpha, d.out_ref, d.out_alpha, br[i]);
+      const int jst = check_hip(hipEventRecord(ev[i + 1], br[i]), "hipEventRecord");
+      const int wst = jst ? jst : check_hip(hipStreamWaitEvent(cs, ev[i + 1], 0),
+                                            "hipStreamWaitEvent");
+      if (!cst) cst = wst;
+    }
+    hipGraph_t g = nullptr;
+    const hipError_t e2 = hipStreamEndCapture(cs, &g);
+    gr->graph = g;
+    st = cst;
+    if (!st && e2 != hipSuccess) st = check_hip(e2, "hipStreamEndCapture");
+    if (!st) st = check_hip(hipGraphInstantiate(&gr->exec, gr->graph, nullptr, nullptr, 0),
+                            "hipGraphInstantiate");
+  }
+  for (hipEvent_t e : ev)
+    if (e) (void)hipEventDestroy(e);
+  for (hipStream_t b : br)
+    if (b) (void)hipStreamDestroy(b);
+  if (cs) (void)hipStreamDestroy(cs);
+  if (st) {
+    if (gr->exec) (void)hipGraphExecDestroy(gr->exec);
+    if (gr->graph) (void)hipGraphDestroy(gr->graph);
+    delete gr;
+    return st;
+  }
+  *out = gr;
   return kOk;
 }
 

@@ -34,8 +34,10 @@ struct GpTile {
 };
 
 // grid (ceil(N / V), S).  MODE 0: c = amp * xi (forward / tangent);  MODE 1:
-// c = the sum of the uplanes complex (H, W) cotangent planes, zero outside
-// the sky grid.  Sample blockIdx.y reads xi[s] / u[s] and writes work[s].
+// c = sum_p u_p * (1 + i dnu[p mod E]) over the uplanes real (H, W)
+// cotangent planes u_p (one per energy, or per sample and energy), zero
+// outside the sky grid.  Sample blockIdx.y reads xi[s] / u[s] and writes
+// work[s].
 // When V does not divide N (odd grids such as 567) the last tile's columns
 // beyond N read zero and are not stored.
 template <typename T, int N, int MODE>
@@ -43,7 +45,8 @@ __global__ __launch_bounds__(GNT) void gp_cols(GpGeom g, const T* __restrict__ a
                                                const T* __restrict__ amp_alpha,
                                                const T* __restrict__ xi_ref,
                                                const T* __restrict__ xi_alpha,
-                                               const cplx<T>* __restrict__ u,
+                                               const T* __restrict__ u,
+                                               const T* __restrict__ dnu,
                                                const cplx<T>* __restrict__ twg,
                                                cplx<T>* __restrict__ Tw) {
   constexpr int V = GpTile<T, N, MODE>::V;
@@ -69,11 +72,11 @@ __global__ __launch_bounds__(GNT) void gp_cols(GpGeom g, const T* __restrict__ a
   constexpr int EPT = (V * N + GNT - 1) / GNT;
   constexpr int CH = (MODE == 0 ? 16 : 8) / (sizeof(T) / 4);
   constexpr int CHK = EPT < CH ? EPT : CH;
-  // MODE 1 reads the planes through a zero-reading gather (elements outside
-  // the sky grid read zero), planes outer and the chunk unrolled inside, so
-  // a chunk's loads of one plane are in flight together.
+  // MODE 1 reads the planes through zero-reading gathers (elements outside
+  // the sky grid read zero), one buffer resource per plane (so offsets stay
+  // 32-bit however many planes), planes outer and the chunk unrolled inside,
+  // so a chunk's loads of one plane are in flight together.
   const int64_t plane = (int64_t)g.H * g.W;
-  const Gather gu = make_gather(u, MODE == 1 ? g.uplanes * plane : 0);
   const int64_t nn = MODE == 0 ? (int64_t)N * N : 0;
   const Gather gar = make_gather(amp_ref, nn), gxr = make_gather(xi_ref, nn);
   const Gather gaa = make_gather(amp_alpha, g.has_alpha ? nn : 0);
@@ -98,14 +101,17 @@ __global__ __launch_bounds__(GNT) void gp_cols(GpGeom g, const T* __restrict__ a
       }
     } else {
       for (int pl = 0; pl < g.uplanes; ++pl) {
+        const Gather gu = make_gather(u + pl * plane, plane);
+        const T dn = g.has_alpha ? dnu[pl % g.E] : T(0);
 #pragma unroll
         for (int c = 0; c < CHK; ++c) {
           const int e = tid + (u0 + c) * GNT;
           const int x1 = e / V, v = e - (e / V) * V;
           const int x2 = c0 + v;
           const bool ok = (V * N % GNT == 0 || e < V * N) && x1 < g.H && x2 < g.W;
-          val[c] = cadd(val[c], gather_load<cplx<T>>(
-                                    gu, gather_off<cplx<T>>(gu, ok, (int)(pl * plane) + x1 * g.W + x2)));
+          const T uv = gather_load<T>(gu, gather_off<T>(gu, ok, x1 * g.W + x2));
+          val[c].re += uv;
+          val[c].im += dn * uv;
         }
       }
     }
@@ -155,7 +161,7 @@ __global__ __launch_bounds__(GNR) void gp_rows(GpGeom g, const cplx<T>* __restri
   const int r0 = blockIdx.x;
   const int r1 = (N - r0) % N;
   const int64_t smp = blockIdx.y;
-  Tw += smp * N * N;
+  if (!(MODE == 2 && g.work_shared)) Tw += smp * N * N;
   const T* xi_a = (const T*)hy.xi_ref;
   const T* xi_b = (const T*)hy.xi_alpha;
   T* ga_a = (T*)hy.g_amp_ref;
@@ -273,23 +279,15 @@ __global__ __launch_bounds__(GNR) void gp_rows(GpGeom g, const cplx<T>* __restri
   }
 }
 
-// u(H, W) = sum_e sky_e * cot_e * (1, dnu_e): cotangent of log s folded over
-// the energies (transpose of the energy broadcast).
+// u_e(H, W) = sky_e * cot_e: the cotangent of log s per energy, the planes
+// the adjoint column pass folds over the energies (the transpose of the
+// energy broadcast).
 template <typename T>
 __global__ __launch_bounds__(GNT) void gp_fold(GpGeom g, const T* __restrict__ sky,
-                                               const T* __restrict__ cot,
-                                               const T* __restrict__ dnu,
-                                               cplx<T>* __restrict__ u) {
-  const int64_t n = (int64_t)g.H * g.W;
-  for (int64_t i = blockIdx.x * (int64_t)GNT + threadIdx.x; i < n; i += (int64_t)gridDim.x * GNT) {
-    T a = T(0), b = T(0);
-    for (int e = 0; e < g.E; ++e) {
-      const T v = sky[e * n + i] * cot[e * n + i];
-      a += v;
-      if (g.has_alpha) b += v * dnu[e];
-    }
-    u[i] = cplx<T>{a, b};
-  }
+                                               const T* __restrict__ cot, T* __restrict__ u) {
+  const int64_t n = (int64_t)g.E * g.H * g.W;
+  for (int64_t i = blockIdx.x * (int64_t)GNT + threadIdx.x; i < n; i += (int64_t)gridDim.x * GNT)
+    u[i] = sky[i] * cot[i];
 }
 
 // ====================================================================== host
@@ -321,7 +319,7 @@ struct GpOps {
       TimedLaunch _t("gp_cols_fwd", s);
       gp_cols<T, N, 0><<<dim3((N + V - 1) / V, g.S), GNT, 0, s>>>(g, (const T*)amp_ref, (const T*)amp_alpha,
                                              (const T*)xi_ref, (const T*)xi_alpha, nullptr,
-                                             (const C*)b.tw, (C*)b.work);
+                                             nullptr, (const C*)b.tw, (C*)b.work);
     }
     {
       TimedLaunch _t("gp_rows_fwd", s);
@@ -336,9 +334,9 @@ struct GpOps {
                      const void* dnu, const void* sky, void* dsky, hipStream_t s) {
     {
       TimedLaunch _t("gp_cols_fwd", s);
-      gp_cols<T, N, 0><<<dim3((N + V - 1) / V, g.S), GNT, 0, s>>>(g, (const T*)amp_ref, (const T*)amp_alpha,
-                                             (const T*)v_ref, (const T*)v_alpha, nullptr,
-                                             (const C*)b.tw, (C*)b.work);
+      gp_cols<T, N, 0><<<dim3((N + V - 1) / V, g.work_shared ? 1 : g.S), GNT, 0, s>>>(
+          g, (const T*)amp_ref, (const T*)amp_alpha, (const T*)v_ref, (const T*)v_alpha, nullptr,
+          nullptr, (const C*)b.tw, (C*)b.work);
     }
     {
       TimedLaunch _t("gp_rows_tangent", s);
@@ -349,12 +347,14 @@ struct GpOps {
     return check_hip(hipGetLastError(), "gp_tangent");
   }
   static int adjoint_u(const GpGeom& g, const GpBuffers& b, const void* amp_ref,
-                       const void* amp_alpha, const void* u, void* g_ref, void* g_alpha,
-                       hipStream_t s, GpHyper hy) {
+                       const void* amp_alpha, const void* dnu, const void* u, void* g_ref,
+                       void* g_alpha, hipStream_t s, GpHyper hy) {
     {
       TimedLaunch _t("gp_cols_adj", s);
-      gp_cols<T, N, 1><<<dim3((N + GpTile<T, N, 1>::V - 1) / GpTile<T, N, 1>::V, g.S), GNT, 0, s>>>(g, nullptr, nullptr, nullptr, nullptr, (const C*)u,
-                                             (const C*)b.tw, (C*)b.work);
+      constexpr int VA = GpTile<T, N, 1>::V;
+      gp_cols<T, N, 1><<<dim3((N + VA - 1) / VA, g.S), GNT, 0, s>>>(
+          g, nullptr, nullptr, nullptr, nullptr, (const T*)u, (const T*)dnu, (const C*)b.tw,
+          (C*)b.work);
     }
     if (hy.g_amp_ref) {
       TimedLaunch _t("gp_rows_adj_hyper", s);
@@ -369,13 +369,12 @@ struct GpOps {
     }
     return check_hip(hipGetLastError(), "gp_adjoint");
   }
-  static int fold(const GpGeom& g, const void* sky, const void* cot, const void* dnu, void* u,
-                  hipStream_t s) {
-    const int64_t n = (int64_t)g.H * g.W;
+  static int fold(const GpGeom& g, const void* sky, const void* cot, void* u, hipStream_t s) {
+    const int64_t n = (int64_t)g.E * g.H * g.W;
     const int blocks = (int)std::min<int64_t>((n + GNT - 1) / GNT, 4096);
     {
       TimedLaunch _t("gp_fold", s);
-      gp_fold<T><<<blocks, GNT, 0, s>>>(g, (const T*)sky, (const T*)cot, (const T*)dnu, (C*)u);
+      gp_fold<T><<<blocks, GNT, 0, s>>>(g, (const T*)sky, (const T*)cot, (T*)u);
     }
     return check_hip(hipGetLastError(), "gp_fold");
   }
@@ -416,20 +415,16 @@ int gp_tangent(int dtype, const GpGeom& g, const GpBuffers& b, const void* amp_r
   });
 }
 int gp_adjoint_u(int dtype, const GpGeom& g, const GpBuffers& b, const void* amp_ref,
-                 const void* amp_alpha, const void* u, void* g_ref, void* g_alpha,
-                 hipStream_t s, GpHyper hy) {
-  // The column pass gathers a sample's planes with 32-bit byte offsets.
-  const int64_t bytes = (int64_t)g.uplanes * g.H * g.W * (dtype == 0 ? 8 : 16);
-  if (bytes > INT32_MAX)
-    return fail(kUnsupported, "the cotangent planes of one sample exceed 2 GiB");
+                 const void* amp_alpha, const void* dnu, const void* u, void* g_ref,
+                 void* g_alpha, hipStream_t s, GpHyper hy) {
+  if (g.has_alpha && !dnu) return fail(kInvalidArgument, "spectral-index inputs are NULL");
   return gp_dispatch(dtype, g.N, [&](auto ops) {
-    return decltype(ops)::adjoint_u(g, b, amp_ref, amp_alpha, u, g_ref, g_alpha, s, hy);
+    return decltype(ops)::adjoint_u(g, b, amp_ref, amp_alpha, dnu, u, g_ref, g_alpha, s, hy);
   });
 }
-int gp_fold(int dtype, const GpGeom& g, const void* sky, const void* cot, const void* dnu,
-            void* u, hipStream_t s) {
-  return gp_dispatch(dtype, g.N,
-                     [&](auto ops) { return decltype(ops)::fold(g, sky, cot, dnu, u, s); });
+int gp_fold(int dtype, const GpGeom& g, const void* sky, const void* cot, void* u,
+            hipStream_t s) {
+  return gp_dispatch(dtype, g.N, [&](auto ops) { return decltype(ops)::fold(g, sky, cot, u, s); });
 }
 
 }  // namespace jubik

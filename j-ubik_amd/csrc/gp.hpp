@@ -10,10 +10,12 @@ struct GpGeom {
   int E;          // energies
   int has_alpha;  // spectral-index field present
   double zm, mu;  // zero mode and spectral-index mean
-  int uplanes;    // adjoint input: number of (H, W) complex planes summed on load
+  int uplanes;    // adjoint input: number of real (H, W) planes summed on load
   int S;          // samples per launch (grid axis y); inputs and outputs are
                   // stacked per sample, the amplitude fields are shared
   int sky_shared; // tangent: 1 = one base-point sky for every sample
+  int work_shared; // tangent: 1 = one tangent shared by the samples (its column
+                   // pass runs once; each sample's row pass reads work[0])
 };
 
 // Optional outputs of the adjoint: the gradients with respect to the
@@ -40,10 +42,14 @@ int gp_forward(int dtype, const GpGeom& g, const GpBuffers& b, const void* amp_r
 int gp_tangent(int dtype, const GpGeom& g, const GpBuffers& b, const void* amp_ref,
                const void* amp_alpha, const void* v_ref, const void* v_alpha, const void* dnu,
                const void* sky, void* dsky, hipStream_t s);
+// Adjoint from the uplanes real (H, W) cotangent planes u of log s (per
+// energy; per sample and energy when several samples are summed), plane p
+// entering the spectral-index field with dnu[p mod E].
 int gp_adjoint_u(int dtype, const GpGeom& g, const GpBuffers& b, const void* amp_ref,
-                 const void* amp_alpha, const void* u, void* g_ref, void* g_alpha, hipStream_t s,
-                 GpHyper hy = GpHyper{});
-int gp_fold(int dtype, const GpGeom& g, const void* sky, const void* cot, const void* dnu,
-            void* u, hipStream_t s);
+                 const void* amp_alpha, const void* dnu, const void* u, void* g_ref,
+                 void* g_alpha, hipStream_t s, GpHyper hy = GpHyper{});
+// u_e = sky_e * cot_e, the E cotangent planes of log s for a cotangent of s.
+int gp_fold(int dtype, const GpGeom& g, const void* sky, const void* cot, void* u,
+            hipStream_t s);
 
 }  // namespace jubik

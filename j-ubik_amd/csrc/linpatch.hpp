@@ -76,8 +76,9 @@ struct NllFinal {
   double* bad_out = nullptr;
 };
 
-// Adjoint chained into the GP: u = sum_jobs scale * sky[e] * PSF^T(gy)_job *
-// (1, dnu[e]) stored as complex (H, W); optionally finalises the NLL.
+// Adjoint chained into the GP: u[s, e] = sum_mod scale * sky[e] *
+// PSF^T(gy)_(s, mod, e), real (S, E, H, W) planes (dnu is unused: the GP
+// adjoint applies it); optionally finalises the NLL.
 int linpatch_adjoint_gp(int dtype, const LinGeom& g, const LinBuffers& b, const void* gy,
                         double scale, const void* sky, const void* dnu, void* u, hipStream_t s,
                         NllFinal fin = NllFinal{});

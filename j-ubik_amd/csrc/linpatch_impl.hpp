@@ -692,8 +692,9 @@ __device__ __forceinline__ void sum_pair_spectra(cplx<T>* buf, const cplx<T>* __
   }
 }
 
-// grid (H, S * J).  Exposure, counts and (metric) the base-point lam are
-// per (mod, e) and shared by the samples; y, resid and partial are per job.
+// grid (H, S * J).  Exposure, counts and (metric, g.sky_shared) the
+// base-point lam are per (mod, e) and shared by the samples; y, resid and
+// partial are per job, and so is lam when g.sky_shared = 0.
 // EPI 1 writes two doubles per (job, row): the NLL terms at partial[job H +
 // i] and the number of read-out pixels with lam <= 0 (or NaN) S J H values
 // further on.
@@ -734,6 +735,9 @@ __global__ __launch_bounds__(NTO) void lp_fwd_out(LinGeom g, const cplx<T>* __re
   for (int t = 0; t < JPT; ++t) acc[t] = T(0);
   const int64_t row = ((int64_t)job * g.H + i) * g.W;
   const int64_t rowd = ((int64_t)(job % g.J) * g.H + i) * g.W;
+  // metric: the base-point lam of (mod, e), or of this job's own sample
+  // when the samples have their own base points (sky_shared = 0)
+  const int64_t rowl = g.sky_shared ? rowd : row;
   const float rd = 1.0f / (float)g.d;
   T ex_v[JPT], lam_v[JPT];
   int cnt_v[JPT];
@@ -761,7 +765,7 @@ __global__ __launch_bounds__(NTO) void lp_fwd_out(LinGeom g, const cplx<T>* __re
           const int j = tid + t * NTO;
           ex_v[t] = j < g.W ? expo[rowd + j] : T(0);
           cnt_v[t] = j < g.W ? counts[rowd + j] : -1;
-          if (EPI == 2) lam_v[t] = j < g.W ? y[rowd + j] : T(1);
+          if (EPI == 2) lam_v[t] = j < g.W ? y[rowl + j] : T(1);
         }
       }
     }
@@ -852,9 +856,9 @@ __global__ __launch_bounds__(NTO) void lp_fwd_out(LinGeom g, const cplx<T>* __re
 }
 
 // EPI 0: grid (H, nimg); gx[img] = scale * sum over the jobs of img.
-// EPI 1: grid (H, S * E);  u[s, e] = sum_mod scale * sky[s, e] * xbar_(s,mod,e)
-//        * (1, dnu[e]), one complex (u_ref, u_alpha) plane per sample and
-//        energy (the GP adjoint sums a sample's planes while loading them);
+// EPI 1: grid (H, S * E);  u[s, e] = sum_mod scale * sky[s, e] * xbar_(s,mod,e),
+//        one real plane per sample and energy, the cotangent of log s (the GP
+//        adjoint folds the energies, with their dnu, while loading them);
 //        sky[s, e] is sky[e] when g.sky_shared.
 template <typename T, int L, int EPI, int JPT>
 __global__ __launch_bounds__(NTO) void lp_adj_out(LinGeom g, const cplx<T>* __restrict__ R2,
@@ -862,8 +866,7 @@ __global__ __launch_bounds__(NTO) void lp_adj_out(LinGeom g, const cplx<T>* __re
                                                  const T* __restrict__ tent,
                                                  T* __restrict__ gx, T scale,
                                                  const T* __restrict__ sky,
-                                                 const T* __restrict__ dnu,
-                                                 cplx<T>* __restrict__ u) {
+                                                 T* __restrict__ u) {
   using OT = OutTile<T, L>;
   constexpr int V = OT::V, TPT = OT::TPT;
   constexpr int VS = lds_stride(L, V);
@@ -889,9 +892,9 @@ __global__ __launch_bounds__(NTO) void lp_adj_out(LinGeom g, const cplx<T>* __re
     const int e = tid + uu * NTO;
     if (e < g.w) tnr[uu] = tent[e];
   }
-  T acc[JPT], acc2[JPT], tmp[JPT];
+  T acc[JPT], tmp[JPT];
 #pragma unroll
-  for (int t = 0; t < JPT; ++t) acc[t] = acc2[t] = T(0);
+  for (int t = 0; t < JPT; ++t) acc[t] = T(0);
   const int npair = (g.n + 1) / 2;
 
   // EPI 0 with a shared input and EPI 1: the jobs (s, mod, e) of image
@@ -967,16 +970,10 @@ __global__ __launch_bounds__(NTO) void lp_adj_out(LinGeom g, const cplx<T>* __re
 #pragma unroll
       for (int t = 0; t < JPT; ++t) acc[t] += tmp[t];
     } else {
-      const int e = job % g.E;
-      const T dn = dnu ? dnu[e] : T(0);
 #pragma unroll
       for (int t = 0; t < JPT; ++t) {
         const int j = tid + t * NTO;
-        if (j < g.W) {
-          const T val = scale * sky_v[t] * tmp[t];
-          acc[t] += val;
-          acc2[t] += dn * val;
-        }
+        if (j < g.W) acc[t] += scale * sky_v[t] * tmp[t];
       }
     }
   }
@@ -992,7 +989,7 @@ __global__ __launch_bounds__(NTO) void lp_adj_out(LinGeom g, const cplx<T>* __re
 #pragma unroll
     for (int t = 0; t < JPT; ++t) {
       const int j = tid + t * NTO;
-      if (j < g.W) u[row + j] = cplx<T>{acc[t], acc2[t]};
+      if (j < g.W) u[row + j] = acc[t];
     }
   }
 }
@@ -1150,11 +1147,11 @@ struct LinOps {
 
   template <int EPI>
   static void adj_out(const LinGeom& g, const LinBuffers& b, int nimg, T* gx, double scale,
-                      const T* sky, const T* dnu, C* u, hipStream_t s, const char* name) {
+                      const T* sky, T* u, hipStream_t s, const char* name) {
     TimedLaunch _t(name, s);
     with_jpt(g, [&](auto jpt) {
       lp_adj_out<T, L, EPI, decltype(jpt)::value><<<dim3(g.H, nimg), NTO, 0, s>>>(
-          g, (const C*)b.R2, (const C*)b.tw, (const T*)b.tent, gx, T(scale), sky, dnu, u);
+          g, (const C*)b.R2, (const C*)b.tw, (const T*)b.tent, gx, T(scale), sky, u);
     });
   }
 
@@ -1179,12 +1176,12 @@ struct LinOps {
       case LinCall::kAdjoint:
         adj_front(g, b, c.in, s);
         adj_out<0>(g, b, g.S * (g.x_shared ? g.E : g.J), (T*)c.out, c.scale, nullptr, nullptr,
-                   nullptr, s, "lp_adj_out");
+                   s, "lp_adj_out");
         return launch_status("linpatch_adjoint");
       case LinCall::kAdjointGp:
         adj_front(g, b, c.in, s, c.fin);
-        adj_out<1>(g, b, g.S * g.E, nullptr, c.scale, (const T*)c.sky, (const T*)c.dnu,
-                   (C*)c.out, s, "lp_adj_out_gp");
+        adj_out<1>(g, b, g.S * g.E, nullptr, c.scale, (const T*)c.sky, (T*)c.out, s,
+                   "lp_adj_out_gp");
         return launch_status("linpatch_adjoint_gp");
       default:
         return fail(kInvalidArgument, "unknown linpatch operation");

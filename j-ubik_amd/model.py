@@ -67,6 +67,9 @@ class XrayModel:
         self.J = J
         self.col_div, self.row_div = 0, 0
         self._point_version = None  # sky version of the metric base point
+        self._batch_S = 0           # samples of the last batched gradient
+        self._batch_version = None
+        self._solver = None
 
     def __del__(self):
         m = getattr(self, "_model", None)
@@ -218,6 +221,8 @@ class XrayModel:
                   dev.ptr(s.amp_alpha), dev.ptr(xr), dev.ptr(xa), s.zm, s.mu, dev.ptr(s.dnu_dev),
                   dev.ptr(nll), dev.ptr(g_ref), dev.ptr(g_alpha), dev.stream())
         self.check_domain()
+        self._batch_S = S
+        self._batch_version = s.version
         return nll.cpu().numpy(), dev.like_input(g_ref, xi_ref), (
             dev.like_input(g_alpha, xi_ref) if g_alpha is not None else None)
 
@@ -255,6 +260,81 @@ class XrayModel:
                   dev.ptr(o_alpha), dev.stream())
         return dev.like_input(o_ref, noise), (
             dev.like_input(o_alpha, noise) if o_alpha is not None else None)
+
+    # ------------------------------------------------------------ solvers
+    def reserve_solver(self):
+        """Allocate the device-resident CG workspace (set-up call)."""
+        _lib.call("jubik_model_reserve_solver", self._model)
+        return self
+
+    def metric_samples(self, v_ref, v_alpha=None, samples=None):
+        """sum_s J_s^T diag(1/lam_s) J_s v for one tangent (N, N) at the S
+        sample positions of the last nll_and_grad_batch call."""
+        self._check_samples()
+        s = self.sky
+        S = int(samples if samples is not None else self._batch_S)
+        vr = dev.to_device(v_ref, self.dtype)
+        va = dev.to_device(v_alpha, self.dtype) if s.has_alpha else None
+        o_ref, o_alpha = self._grads()
+        _lib.call("jubik_model_metric_samples", self._model, S, dev.ptr(s.amp_ref),
+                  dev.ptr(s.amp_alpha), dev.ptr(s.dnu_dev), dev.ptr(vr), dev.ptr(va),
+                  dev.ptr(o_ref), dev.ptr(o_alpha), dev.stream())
+        return dev.like_input(o_ref, v_ref), (
+            dev.like_input(o_alpha, v_ref) if o_alpha is not None else None)
+
+    def _solve(self, name, S, b_ref, b_alpha, shape, rtol, absdelta, maxiter, check_every):
+        s = self.sky
+        if not self._solver:
+            self.reserve_solver()
+            self._solver = True
+        br = dev.to_device(b_ref, self.dtype).contiguous()
+        ba = dev.to_device(b_alpha, self.dtype).contiguous() if s.has_alpha else None
+        for b in (br, ba):
+            if b is not None and tuple(b.shape) != shape:
+                raise ValueError(f"right-hand sides must have shape {shape}")
+        x_ref = torch.empty_like(br)
+        x_alpha = torch.empty_like(ba) if ba is not None else None
+        nsys = shape[0] if len(shape) == 3 else 1
+        info = torch.zeros(2 * nsys, dtype=torch.float64, device="cuda")
+        _lib.call(name, self._model, S, dev.ptr(s.amp_ref), dev.ptr(s.amp_alpha),
+                  dev.ptr(s.dnu_dev), dev.ptr(br), dev.ptr(ba), dev.ptr(x_ref), dev.ptr(x_alpha),
+                  int(maxiter), float(rtol), float(absdelta or 0.0), int(check_every),
+                  dev.ptr(info), dev.stream())
+        return (x_ref, x_alpha), info
+
+    def metric_solve(self, b_ref, b_alpha=None, rtol=1e-6, absdelta=None, maxiter=100,
+                     check_every=10):
+        """x_s = (I + J^T diag(1/lam) J)^-1 b_s at the base point for (S, N, N)
+        right-hand sides, by the device-resident CG; returns ((x_ref,
+        x_alpha), info) with info the per-system iteration counts and
+        relative residuals (numpy)."""
+        self._check_point()
+        N = self.sky.N
+        S = int(b_ref.shape[0])
+        self._batch_size(b_ref)
+        (xr, xa), info = self._solve("jubik_model_metric_solve", S, b_ref, b_alpha, (S, N, N),
+                                     rtol, absdelta, maxiter, check_every)
+        inf = info.cpu().numpy().reshape(S, 2)
+        return (xr, xa), {"iterations": inf[:, 0].astype(np.int64), "rel_residual": inf[:, 1]}
+
+    def _check_samples(self):
+        if not self._batch_S:
+            raise RuntimeError("no sample positions (evaluate nll_and_grad_batch first)")
+        if self._batch_version != self.sky.version:
+            raise RuntimeError("the sky parameters changed since the last batched gradient")
+
+    def kl_solve(self, b_ref, b_alpha=None, rtol=1e-6, absdelta=None, maxiter=100,
+                 check_every=10):
+        """x = (I + (1/S) sum_s J_s^T diag(1/lam_s) J_s)^-1 b at the S sample
+        positions of the last nll_and_grad_batch (the KL metric of the
+        standardised latents), by the device-resident CG."""
+        self._check_samples()
+        N = self.sky.N
+        S = int(self._batch_S)
+        (xr, xa), info = self._solve("jubik_model_kl_solve", S, b_ref, b_alpha, (N, N), rtol,
+                                     absdelta, maxiter, check_every)
+        inf = info.cpu().numpy()
+        return (xr, xa), {"iterations": int(inf[0]), "rel_residual": float(inf[1])}
 
     def capture_grad(self, samples=0):
         """Capture one gradient evaluation into an executable HIP graph.
@@ -309,6 +389,68 @@ def capture_grad_batch(entries):
     positions into ``graph.slots[i].xi_ref`` / ``xi_alpha``, call
     ``graph.launch()``, read ``slots[i].nll`` / ``g_ref`` / ``g_alpha``."""
     return BatchGradGraph([_GradSlot(m, int(S)) for m, S in entries])
+
+
+class _MetricSlot:
+    """Fixed device buffers of one captured metric batch: S tangents in,
+    S products out."""
+
+    def __init__(self, model, samples):
+        s = model.sky
+        model._check_point()
+        if not 1 <= samples <= model.max_samples:
+            raise ValueError(f"samples must be in [1, max_samples={model.max_samples}]")
+        shape = (samples, s.N, s.N)
+        self.model, self.samples = model, samples
+        self.v_ref = torch.zeros(shape, dtype=model.dtype, device="cuda")
+        self.v_alpha = torch.zeros(shape, dtype=model.dtype, device="cuda") if s.has_alpha else None
+        self.out_ref = torch.empty(shape, dtype=model.dtype, device="cuda")
+        self.out_alpha = torch.empty(shape, dtype=model.dtype, device="cuda") if s.has_alpha else None
+        self.version = s.version
+
+    def desc(self):
+        s = self.model.sky
+        return _lib.MetricDesc(self.model._model.value, self.samples, dev.ptr(s.amp_ref),
+                               dev.ptr(s.amp_alpha), dev.ptr(s.dnu_dev), dev.ptr(self.v_ref),
+                               dev.ptr(self.v_alpha), dev.ptr(self.out_ref),
+                               dev.ptr(self.out_alpha))
+
+
+def capture_metric_batch(entries):
+    """Capture the metric products of several models (one per observation,
+    uneven grids allowed) into one HIP graph whose batches run as parallel
+    branches: ``entries`` are (model, S) pairs, each model at its current
+    base point.  Write tangents into ``graph.slots[i].v_ref`` / ``v_alpha``,
+    call ``graph.launch()``, read ``out_ref`` / ``out_alpha``."""
+    return MetricBatchGraph([_MetricSlot(m, int(S)) for m, S in entries])
+
+
+class MetricBatchGraph:
+    """Several captured metric batches in one executable graph."""
+
+    def __init__(self, slots):
+        self.slots = slots
+        descs = (_lib.MetricDesc * len(slots))(*[sl.desc() for sl in slots])
+        self._graph = ctypes.c_void_p()
+        _lib.call("jubik_graph_capture_metric_batch", ctypes.byref(self._graph),
+                  ctypes.cast(descs, ctypes.c_void_p), len(slots))
+
+    def launch(self):
+        """Enqueue one replay on the current stream (no synchronisation)."""
+        for sl in self.slots:
+            if sl.model.sky.version != sl.version:
+                raise RuntimeError("the sky parameters changed since this graph was captured: "
+                                   "capture it again")
+        _lib.call("jubik_graph_launch", self._graph, dev.stream())
+
+    def __del__(self):
+        g = getattr(self, "_graph", None)
+        if g is not None and g.value:
+            try:
+                _lib.load().jubik_graph_destroy(g)
+            except Exception:
+                pass
+            self._graph = None
 
 
 class BatchGradGraph:

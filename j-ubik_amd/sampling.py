@@ -10,9 +10,11 @@ latent space of the GP is
     (I + J^T diag(1/lam) J) x = b,        samples = xi +- x,
 
 so x ~ N(0, M^-1) for the MGVI metric M.  Every operator application is one
-batched pass of the HIP kernels over all S systems
-(XrayModel.metric_batch / sqrt_metric_batch); the CG recurrences run per
-system with FP64 dot products.
+batched pass of the HIP kernels over all S systems, and the CG recurrences
+run on the device (XrayModel.metric_solve, jubik_model_metric_solve).
+``conjugate_gradient`` is the same recurrence for an arbitrary operator
+given as a Python callable (host-driven, one synchronisation per
+iteration).
 """
 import numpy as np
 import torch
@@ -84,18 +86,15 @@ def conjugate_gradient(apply, b, rtol=1e-6, absdelta=None, maxiter=100):
     return x, info
 
 
-def metric_solve(model: XrayModel, b_ref, b_alpha=None, rtol=1e-6, absdelta=None, maxiter=100):
+def metric_solve(model: XrayModel, b_ref, b_alpha=None, rtol=1e-6, absdelta=None, maxiter=100,
+                 check_every=10):
     """x = (I + J^T diag(1/lam) J)^-1 b at the last gradient position, for
-    (S, N, N) right-hand sides; returns ((x_ref, x_alpha), info)."""
-    dtype = model.dtype
-    b = (torch.as_tensor(b_ref, device="cuda").to(dtype),
-         None if b_alpha is None else torch.as_tensor(b_alpha, device="cuda").to(dtype))
-
-    def apply(v):
-        m_ref, m_alpha = model.metric_batch(v[0], v[1])
-        return (v[0] + m_ref, None if v[1] is None else v[1] + m_alpha)
-
-    return conjugate_gradient(apply, b, rtol=rtol, absdelta=absdelta, maxiter=maxiter)
+    (S, N, N) right-hand sides, by the library's device-resident CG (FP64
+    dot products in fixed-order reductions, no host synchronisation between
+    checks every ``check_every`` iterations); returns ((x_ref, x_alpha),
+    info)."""
+    return model.metric_solve(b_ref, b_alpha, rtol=rtol, absdelta=absdelta, maxiter=maxiter,
+                              check_every=check_every)
 
 
 def draw_linear_samples(model: XrayModel, xi_ref, xi_alpha=None, n_samples=2, seed=0,
