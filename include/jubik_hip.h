@@ -148,6 +148,17 @@ int jubik_poisson_sample(int dtype, const void* lam, int64_t n, uint64_t seed, i
  * NULL).  ``work``: JUBIK_POISSON_WORK_BYTES of device memory. */
 int jubik_poisson_nll_grad(int dtype, const void* lam, const void* data, int64_t n, double* nll,
                            void* grad, void* work, void* stream);
+/* jubik_poisson_nll_grad without a workspace argument and without any
+ * allocation: no counterpart in the reference; for callers that cannot
+ * allocate device memory (code under HIP graph capture, serving loops,
+ * harnesses that forbid hipMalloc).  The FP64 partial sums live in a
+ * static device array of JUBIK_POISSON_WORK_BYTES (1024 partials, one per
+ * reduction block; any n is reduced in at most 1024 blocks) that each GPU
+ * holds until the process ends.  Every call on one GPU shares that array:
+ * calls that overlap on one GPU (concurrent streams or host threads) race
+ * and must be serialised by the caller. */
+int jubik_poisson_nll_grad_noalloc(int dtype, const void* lam, const void* data, int64_t n,
+                                   double* nll, void* grad, void* stream);
 /* Fisher metric of that likelihood in data space, applied to t:
  * out = t / lam (sqrt_form = 0, nifty.re Poissonian.metric) or its left
  * square root out = t / sqrt(lam) (sqrt_form = 1, left_sqrt_metric), the

@@ -41,6 +41,7 @@ SIGNATURES = {
     "jubik_readout_scatter": [_int, _vp, _vp, _i64, _i64, _vp, _vp],
     "jubik_poisson_nll_grad": [_int, _vp, _vp, _i64, _vp, _vp, _vp, _vp],
     "jubik_poisson_metric": [_int, _vp, _vp, _i64, _int, _vp, _vp],
+    "jubik_poisson_nll_grad_noalloc": [_int, _vp, _vp, _i64, _vp, _vp, _vp],
     "jubik_gp_create": [C.POINTER(_vp), _int, _i64, _i64, _i64, _i64, _int],
     "jubik_gp_destroy": [_vp],
     "jubik_gp_forward": [_vp, _vp, _vp, _vp, _vp, _dbl, _dbl, _vp, _vp, _vp],

@@ -652,6 +652,14 @@ int jubik_poisson_nll_grad(int dtype, const void* lam, const void* data, int64_t
   return poisson_nll_grad(dtype, lam, data, n, nll, grad, (double*)work, as_stream(stream));
 }
 
+int jubik_poisson_nll_grad_noalloc(int dtype, const void* lam, const void* data, int64_t n,
+                                   double* nll, void* grad, void* stream) {
+  if (dtype != 0 && dtype != 1) return fail(kInvalidArgument, "dtype must be 0 or 1");
+  if (n < 0) return fail(kInvalidArgument, "negative size");
+  if (!nll || (n && (!lam || !data))) return fail(kInvalidArgument, "NULL argument");
+  return poisson_nll_grad_noalloc(dtype, lam, data, n, nll, grad, as_stream(stream));
+}
+
 int jubik_poisson_metric(int dtype, const void* lam, const void* t, int64_t n, int sqrt_form,
                          void* out, void* stream) {
   if (dtype != 0 && dtype != 1) return fail(kInvalidArgument, "dtype must be 0 or 1");

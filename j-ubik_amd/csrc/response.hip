@@ -2,6 +2,8 @@
 // (jubik/response.py:12-112; the NLL formula of nifty.re's Poissonian as
 // documented at demos/x-ray-imaging.py:312-313).  Reductions run in FP64 in
 // a fixed order: per-block partial sums, then one block sums the partials.
+#include <mutex>
+#include <string>
 #include <type_traits>
 
 #include "response.hpp"
@@ -265,6 +267,56 @@ int poisson_nll_grad(int dtype, const void* lam, const void* d, int64_t n, doubl
                                              (double*)grad, work);
   k_reduce<<<1, RNT, 0, s>>>(work, blocks, nll);
   return check_hip(hipGetLastError(), "poisson_nll_grad");
+}
+
+// ---------------------------------------------------------------- no-allocation variant
+// The partials of the Poisson reduction in a file-scope device array, for
+// callers that cannot allocate (graph capture, serving loops): one copy per
+// GPU, held until the process ends.  Host code never passes ``scratch``
+// itself to a kernel: its device address comes from hipGetSymbolAddress,
+// once per device, with that device current.
+static_assert(RBLOCKS <= kPoissonWorkDoubles, "the reduction's partials fit the scratch");
+static __device__ __align__(256) double scratch[kPoissonWorkDoubles];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {};
+}  // namespace
+
+// Device address of this device's scratch array (looked up on first use).
+static int scratch_address(double** out) {
+  *out = nullptr;
+  int dev = -1;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return check_hip(e, "hipGetDevice");
+  if (dev < 0 || dev >= kMaxDevices)
+    return fail(kUnsupported, "device index " + std::to_string(dev) + " beyond " +
+                                  std::to_string(kMaxDevices));
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return check_hip(e, "hipGetSymbolAddress(scratch)");
+    g_scratch_ptr[dev] = static_cast<double*>(p);
+  }
+  *out = g_scratch_ptr[dev];
+  return kOk;
+}
+
+int poisson_nll_grad_noalloc(int dtype, const void* lam, const void* d, int64_t n, double* nll,
+                             void* grad, hipStream_t s) {
+  const int blocks = poisson_blocks(n);
+  if (blocks > kPoissonWorkDoubles)
+    return fail(kInvalidArgument, "the reduction needs " + std::to_string(blocks) +
+                                      " partials; the scratch holds " +
+                                      std::to_string(kPoissonWorkDoubles));
+  double* work = nullptr;
+  const int st = scratch_address(&work);
+  if (st) return st;
+  // Every partial the reduction reads is written by this call's first
+  // kernel, so the scratch needs no clearing.
+  return poisson_nll_grad(dtype, lam, d, n, nll, grad, work, s);
 }
 
 }  // namespace jubik

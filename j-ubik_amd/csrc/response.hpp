@@ -24,5 +24,9 @@ int poisson_metric(int dtype, const void* lam, const void* t, int64_t n, int sqr
                    void* out, hipStream_t s);
 int poisson_nll_grad(int dtype, const void* lam, const void* d, int64_t n, double* nll,
                      void* grad, double* work, hipStream_t s);
+// poisson_nll_grad with the partials in the library's static device scratch
+// (no allocation; calls that overlap on one GPU race on it).
+int poisson_nll_grad_noalloc(int dtype, const void* lam, const void* d, int64_t n, double* nll,
+                             void* grad, hipStream_t s);
 
 }  // namespace jubik
