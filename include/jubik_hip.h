@@ -56,6 +56,11 @@ int jubik_timing_report(char* buf, int64_t len);
  * kernel), so one kernel can be timed without events between the other
  * launches of an evaluation. */
 int jubik_timing_select(const char* name);
+/* Fault injection for tests of the error paths (no counterpart in the
+ * reference): the nth next device allocation the library makes fails with
+ * JUBIK_ENOMEM as if the device were full (nth = 0 disarms).  Creation calls
+ * release what they allocated before the failure. */
+int jubik_debug_fail_alloc(int64_t nth);
 
 /* ---------------------------------------------------------------- linpatch
  * Replaces jubik/convolve.py:66-145 linpatch_convolve(x, domain, kernel,

@@ -26,6 +26,7 @@ SIGNATURES = {
     "jubik_timing_enable": [_int],
     "jubik_timing_report": [C.c_char_p, _i64],
     "jubik_timing_select": [C.c_char_p],
+    "jubik_debug_fail_alloc": [_i64],
     "jubik_linpatch_create": [C.POINTER(_vp), _int, _i64, _i64, _i64, _i64, _i64, _i64, _int,
                               _int],
     "jubik_linpatch_destroy": [_vp],

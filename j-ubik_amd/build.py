@@ -59,5 +59,47 @@ def build(verbose=True):
     return LIB
 
 
+ASAN_LIB = OUT / "libjubik_hip_asan.so"
+
+
+def _asan_runtime():
+    """clang's AddressSanitizer runtime of the hipcc toolchain."""
+    root = Path(_hipcc()).resolve().parent.parent
+    for base in (root / "llvm" / "lib" / "clang", root / "lib" / "llvm" / "lib" / "clang"):
+        for cand in sorted(base.glob("*/lib/linux/libclang_rt.asan-x86_64.so")):
+            return cand
+    raise RuntimeError("clang's AddressSanitizer runtime not found")
+
+
+def build_asan(verbose=True):
+    """Host-side AddressSanitizer variant of the library for the argument
+    validation tests: capi.hip (the C ABI's checks and host logic) is
+    compiled with ASan on the host side only, the device code and the other
+    translation units are the regular objects of ``build``.  Returns
+    (library path, ASan runtime to preload)."""
+    objs = sorted(OUT.glob("*.o")) if OUT.exists() else []
+    wanted = {src.stem for src in SRC.glob("*.hip")}
+    objs = [o for o in objs if o.stem in wanted and o.stem != "capi"]
+    if len(objs) != len(wanted) - 1:
+        build(verbose=verbose)
+        objs = [o for o in sorted(OUT.glob("*.o")) if o.stem in wanted and o.stem != "capi"]
+    asan_obj = OUT / "capi_asan.o"
+    cmd = [_hipcc(), *CFLAGS, "-g", "-Xarch_host", "-fsanitize=address", "-Xarch_host",
+           "-fno-omit-frame-pointer", "-c", str(SRC / "capi.hip"), "-o", str(asan_obj)]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"hipcc (host ASan) failed for capi.hip:\n{res.stderr[-6000:]}")
+    tmp = ASAN_LIB.with_suffix(".so.tmp")
+    cmd = [_hipcc(), "-shared", "-fPIC", f"--offload-arch={ARCH}", *map(str, objs), str(asan_obj),
+           "-o", str(tmp)]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"link (host ASan) failed:\n{res.stderr[-6000:]}")
+    os.replace(tmp, ASAN_LIB)
+    if verbose:
+        print(f"built {ASAN_LIB}", file=sys.stderr)
+    return ASAN_LIB, _asan_runtime()
+
+
 if __name__ == "__main__":
     build()

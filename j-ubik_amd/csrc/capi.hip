@@ -72,8 +72,14 @@ struct DeviceGuard {
 
 size_t elem_size(int dtype) { return dtype == 0 ? sizeof(float) : sizeof(double); }
 
+// Fault injection (jubik_debug_fail_alloc): the n-th next device allocation
+// fails as if the device were out of memory; 0 = off.
+std::atomic<int64_t> g_fail_alloc{0};
+
 int dev_alloc(void** p, size_t bytes, const char* what) {
   *p = nullptr;
+  if (g_fail_alloc.load() > 0 && g_fail_alloc.fetch_sub(1) == 1)
+    return fail(kOutOfMemory, std::string("allocating ") + what + ": injected failure");
   if (bytes == 0) bytes = 256;
   hipError_t e = hipMalloc(p, bytes);
   if (e != hipSuccess) {
@@ -261,6 +267,12 @@ int jubik_abi_version(void) { return 1; }
 
 int jubik_timing_enable(int on) {
   g_timing.store(on != 0);
+  return kOk;
+}
+
+int jubik_debug_fail_alloc(int64_t nth) {
+  if (nth < 0) return fail(kInvalidArgument, "nth must be non-negative");
+  g_fail_alloc.store(nth);
   return kOk;
 }
 
