@@ -53,6 +53,9 @@ def parse():
                          "(profiling runs repeat the pick without the tuner's own launches)")
     ap.add_argument("--sweep", action="store_true",
                     help="time every benchmark configuration instead of the 1024^2 x 4E line")
+    ap.add_argument("--transform", default="fft", choices=("fft", "mfma_bf16x3"),
+                    help="column-pass transform of the timed FP32 path (with --profile-only); "
+                         "the full run times both at FFT length 384")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="with --profile-only: first copy a buffer of this many MiB "
                          "(a known-size reference for the HBM byte counters)")
@@ -355,6 +358,26 @@ def sweep(args):
             t_g = time_steps(torch, lambda: model.nll_and_grad_batch(xb, xab), max(3, steps // 8), 1)
             rec.update({"samples": S, "metric_batch_ms_per_product": 1e3 * t_m / S,
                         "grad_batch_ms_per_eval": 1e3 * t_g / S})
+        gmf = None
+        if psf.L == 384:
+            # the split-BF16 matrix-core column transform on the same model
+            psf.set_transform("mfma_bf16x3")
+            nmf, gmr, gma = model.nll_and_grad(xr_t, xa_t)
+            gmf = np.concatenate([gmr.double().cpu().numpy().ravel()] +
+                                 ([gma.double().cpu().numpy().ravel()] if spectral else []))
+            mrec = {"grad_rel_err_vs_fp64": rel(gmf, g64), "grad_nmax_err_vs_fp64": nmax(gmf, g64),
+                    "nll_rel_err_vs_fp64": abs(nmf - n64) / abs(n64),
+                    "grad_eval_ms": 1e3 * time_steps(
+                        torch, lambda: model.nll_and_grad_device(xr_t, xa_t, out=out), steps, 2)}
+            if c["S"]:
+                S = c["S"]
+                xb = xr_t[None] + 0.05 * torch.randn((S, H, H), device="cuda")
+                xab = (xa_t[None] + 0.05 * torch.randn((S, H, H), device="cuda")
+                       if spectral else None)
+                mrec["grad_batch_ms_per_eval"] = 1e3 * time_steps(
+                    torch, lambda: model.nll_and_grad_batch(xb, xab), max(3, steps // 8), 1) / S
+            psf.set_transform("fft")
+            rec["mfma_bf16x3"] = mrec
         if not args.no_cpu:
             # one evaluation of the float64 CPU oracle: timing and the error
             # of both GPU builds against it
@@ -373,6 +396,9 @@ def sweep(args):
                         "grad_nmax_err_fp32_vs_oracle": nmax(g32, go),
                         "grad_rel_err_fp64_vs_oracle": rel(g64, go),
                         "nll_rel_err_fp32_vs_oracle": abs(n32 - no) / abs(no)})
+            if gmf is not None:
+                rec["mfma_bf16x3"].update({"grad_rel_err_vs_oracle": rel(gmf, go),
+                                           "grad_nmax_err_vs_oracle": nmax(gmf, go)})
         print(json.dumps(rec), flush=True)
         if name.startswith("grid_"):
             # kept for the uneven-grid graph below, with its single-evaluation tile
@@ -481,6 +507,7 @@ def main():
     out32 = mixed._grads()
     step = lambda: mixed.nll_and_grad_device(*xi32, out=out32)
     if args.profile_only:
+        mixed.psf.set_transform(args.transform)
         if args.calib_copy_mib:
             src = torch.ones(args.calib_copy_mib << 18, dtype=torch.float32, device="cuda")
             dst = torch.empty_like(src)
@@ -529,6 +556,28 @@ def main():
     torch.cuda.synchronize()
     g32 = np.concatenate([g32r.double().cpu().numpy().ravel(), g32a.double().cpu().numpy().ravel()])
     nll32 = float(nll32.item())
+
+    # the same evaluation with the split-BF16 matrix-core column transform
+    mfma = None
+    if mixed.psf.L == 384:
+        mixed.psf.set_transform("mfma_bf16x3")
+        t_mf = time_steps(torch, step, args.steps, args.warmup)
+        nmf, gmr, gma = mixed.nll_and_grad_device(*xi32)
+        torch.cuda.synchronize()
+        gmf = np.concatenate([gmr.double().cpu().numpy().ravel(),
+                              gma.double().cpu().numpy().ravel()])
+        _lib.timing_enable(True)
+        for _ in range(max(10, min(args.steps, 30))):
+            step()
+        torch.cuda.synchronize()
+        rep_mf = _lib.timing_report()
+        _lib.timing_enable(False)
+        mixed.psf.set_transform("fft")
+        mfma = {"ms_per_step": t_mf * 1e3, "evals_per_s": 1.0 / t_mf,
+                "nll": float(nmf.item()), "grad": gmf,
+                "column_pass_us_bracketed": {
+                    k: 1e3 * tot / c for k, (c, tot, lo, hi) in rep_mf.items()
+                    if k.startswith("lp_cols")}}
 
     # Per-kernel HIP-event timing on the launch stream (outside the timed
     # region).  With every launch bracketed the per-kernel times carry the
@@ -663,6 +712,14 @@ def main():
         "batched_samples": batched,
         "kernels": kernels,
     }
+    if mfma is not None:
+        gmf = mfma.pop("grad")
+        mfma.update({"grad_rel_err_vs_fp64": rel(gmf, g64), "grad_nmax_err_vs_fp64": nmax(gmf, g64),
+                     "nll_rel_err_vs_fp64": abs(mfma["nll"] - nll64) / abs(nll64)})
+        if not args.no_cpu:
+            mfma.update({"grad_rel_err_vs_oracle": rel(gmf, go),
+                         "grad_nmax_err_vs_oracle": nmax(gmf, go)})
+        line["column_transform_mfma_bf16x3"] = mfma
     print(json.dumps(line))
 
 

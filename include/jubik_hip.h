@@ -107,6 +107,17 @@ int jubik_linpatch_adjoint(jubik_linpatch* plan, const void* gy, void* gx, doubl
 int jubik_linpatch_set_tile(jubik_linpatch* plan, int col_div, int row_div);
 int jubik_linpatch_autotune(jubik_linpatch* plan, int reps, int* col_div, int* row_div,
                             double* ms, void* stream);
+/* Column-pass transform (no counterpart in the reference, whose FFTs XLA
+ * runs in the input precision): kind 0 = the plan's FFT (default), kind 1
+ * = each 384-point column DFT as three 128-point DFTs on the matrix cores
+ * (v_mfma_f32_32x32x16_bf16, every product split into BF16 hi and lo
+ * halves, hi*hi + hi*lo + lo*hi, FP32 accumulation) and an FP32 radix-3
+ * step.  Kind 1 needs a float32 plan with window + 2 * margin = 384
+ * (JUBIK_EUNSUPPORTED otherwise) and allocates 128 KiB of DFT matrices on
+ * first use.  Row passes and the overlap-add are unchanged.  It applies to
+ * every later call of the plan and of the models built on it; graphs
+ * captured earlier keep the transform they were captured with. */
+int jubik_linpatch_set_transform(jubik_linpatch* plan, int kind);
 
 /* ---------------------------------------------------------------- primitives
  * slice_patches of jubik/convolve.py:28-63 on x (B, H, W): out (n^2, B,

@@ -56,6 +56,7 @@ SIGNATURES = {
     "jubik_model_metric": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
     "jubik_model_check_domain": [_vp, _vp, _vp],
     "jubik_linpatch_set_tile": [_vp, _int, _int],
+    "jubik_linpatch_set_transform": [_vp, _int],
     "jubik_linpatch_autotune": [_vp, _int, C.POINTER(_int), C.POINTER(_int), _dblp, _vp],
     "jubik_model_set_tile": [_vp, _int, _int],
     "jubik_model_autotune": [_vp, _i64, _int, C.POINTER(_int), C.POINTER(_int), _dblp, _vp],

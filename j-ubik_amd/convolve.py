@@ -61,6 +61,7 @@ class LinpatchOperator:
         self.workspace_bytes, self.spectra_bytes = int(info[6]), int(info[7])
         self._kernel_ref = None
         self.col_div, self.row_div = 1, 1
+        self.transform = "fft"
 
     def __del__(self):
         plan = getattr(self, "_plan", None)
@@ -145,6 +146,20 @@ class LinpatchOperator:
         to rounding)."""
         _lib.call("jubik_linpatch_set_tile", self._plan, int(col_div), int(row_div))
         self.col_div, self.row_div = int(col_div), int(row_div)
+        return self
+
+    TRANSFORMS = ("fft", "mfma_bf16x3")
+
+    def set_transform(self, kind):
+        """Transform of the column passes: "fft" (the plan's FFT, default) or
+        "mfma_bf16x3" (float32 plans at FFT length 384: each column DFT as
+        three 128-point DFTs on the matrix cores with BF16 hi/lo split
+        products, hi*hi + hi*lo + lo*hi, FP32 accumulation, and an FP32
+        radix-3 step).  Applies to the models built on this operator too."""
+        if kind not in self.TRANSFORMS:
+            raise ValueError(f"transform must be one of {self.TRANSFORMS}")
+        _lib.call("jubik_linpatch_set_transform", self._plan, self.TRANSFORMS.index(kind))
+        self.transform = kind
         return self
 
     def autotune(self, reps=20, cache=None):

@@ -161,8 +161,13 @@ struct jubik_linpatch {
   bool ready = false;
   int col_div = 1;  // column- and row-pass tile divisors (set by the autotuner)
   int row_div = 1;
-  LinBuffers bufs() const { return LinBuffers{tw, tent, KT, R1, R2, KB, col_div, row_div}; }
+  int transform = 0;       // column-pass transform: 0 FFT, 1 split-BF16 matrix-core DFT
+  void* frag = nullptr;    // DFT fragments of transform 1 (allocated on first use)
+  LinBuffers bufs() const {
+    return LinBuffers{tw, tent, KT, R1, R2, KB, col_div, row_div, transform == 1 ? frag : nullptr};
+  }
   void release() {
+    dev_free(frag);
     dev_free(tw);
     dev_free(tent);
     dev_free(KT);
@@ -573,6 +578,29 @@ int jubik_linpatch_autotune(jubik_linpatch* plan, int reps, int* col_div, int* r
   plan->row_div = br;
   *col_div = bc;
   *row_div = br;
+  return kOk;
+}
+
+int jubik_linpatch_set_transform(jubik_linpatch* plan, int kind) {
+  if (!plan) return fail(kInvalidArgument, "NULL argument");
+  if (kind != 0 && kind != 1) return fail(kInvalidArgument, "transform must be 0 or 1");
+  if (kind == 1) {
+    if (!linpatch_mfma_supported(plan->dtype, plan->g))
+      return fail(kUnsupported, "the split-BF16 matrix-core transform needs a float32 plan with "
+                                "FFT length 384 (window + 2 * margin = 384); this plan has " +
+                                    std::string(plan->dtype ? "float64" : "float32") +
+                                    " and length " + std::to_string(plan->g.Lw));
+    if (!plan->frag) {
+      DeviceGuard dg(plan->device);
+      std::vector<unsigned short> host;
+      dft128_fragments(host);
+      const size_t bytes = host.size() * sizeof(unsigned short);
+      const int st = dev_alloc(&plan->frag, bytes, "matrix-core DFT fragments");
+      if (st) return st;
+      JUBIK_HIP(hipMemcpy(plan->frag, host.data(), bytes, hipMemcpyHostToDevice));
+    }
+  }
+  plan->transform = kind;
   return kOk;
 }
 

@@ -16,21 +16,8 @@
 
 namespace jubik {
 
-typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
-typedef float f32x16 __attribute__((ext_vector_type(16)));
-
 constexpr int DNT = 512;   // 8 waves, 32 vectors each
 constexpr int DVT = 256;   // vectors per tile
-
-// float -> bf16 with round-to-nearest-even (finite inputs)
-__host__ __device__ inline unsigned short bf16_bits(float f) {
-  unsigned u = __builtin_bit_cast(unsigned, f);
-  u += 0x7FFFu + ((u >> 16) & 1u);
-  return (unsigned short)(u >> 16);
-}
-__host__ __device__ inline float bf16_value(unsigned short h) {
-  return __builtin_bit_cast(float, (unsigned)h << 16);
-}
 
 struct Split8 {
   bf16x8 hi, lo;
@@ -131,7 +118,6 @@ __global__ __launch_bounds__(256) void k_dft128_fft(const cplx<float>* __restric
   for (int e = tid; e < V * L; e += 256) y[base + e] = buf[(e / L) * VS + (e % L)];
 }
 
-// Fragments of C = cos(2 pi r k / 128) and S = sin(2 pi r k / 128), split.
 void dft128_fragments(std::vector<unsigned short>& out) {
   out.assign((size_t)4 * 4 * 8 * 64 * 8, 0);
   const long double tau = 6.283185307179586476925286766559005768L;

@@ -14,6 +14,10 @@ struct LinBuffers {
   void* KB;          // beta of the packed DC/Nyquist column, (M, P, Ek) x L
   int col_div;       // column-pass tile: 1, 2 or 4 (a full, half or quarter tile)
   int row_div;       // row-pass tile: 1, 2 or 4
+  // Split-BF16 fragments of the 128-point DFT matrices (dft128_fragments);
+  // set: the forward and adjoint column passes run on the matrix cores
+  // (FP32 plans at L = 384 outside fold mode, linpatch_mfma_supported).
+  const void* frag = nullptr;
 };
 
 // FFT lengths compiled into the library, in groups of one translation unit
@@ -82,6 +86,14 @@ struct NllFinal {
 int linpatch_adjoint_gp(int dtype, const LinGeom& g, const LinBuffers& b, const void* gy,
                         double scale, const void* sky, const void* dnu, void* u, hipStream_t s,
                         NllFinal fin = NllFinal{});
+
+// Matrix-core column pass (lp_mfma.hip): the forward (adjoint = 0) or
+// adjoint column pass of an FP32 plan at L = 384 with each column DFT as
+// three split-BF16 128-point DFT GEMMs and a radix-3 step; b.frag must be
+// set.  Same inputs and outputs as the FFT column pass.
+bool linpatch_mfma_supported(int dtype, const LinGeom& g);
+int linpatch_cols_mfma(const LinGeom& g, const LinBuffers& b, int adjoint, NllFinal fin,
+                       hipStream_t s);
 
 // Arguments of one linpatch operation (the entry points above), passed to
 // the per-length instantiations (lp_run<T, L> in linpatch_impl.hpp).

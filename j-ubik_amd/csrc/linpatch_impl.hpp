@@ -1045,8 +1045,12 @@ struct LinOps {
       f(std::false_type{});
   }
 
-  static void fwd_front(const LinGeom& g, const LinBuffers& b, const void* x, double scale,
-                        hipStream_t s) {
+  // The split-BF16 matrix-core column pass replaces the FFT one where the
+  // plan holds its fragments (FP32, L = 384, outside fold mode).
+  static constexpr bool kMfma = std::is_same<T, float>::value && L == 384;
+
+  static int fwd_front(const LinGeom& g, const LinBuffers& b, const void* x, double scale,
+                       hipStream_t s) {
     {
       TimedLaunch _t("lp_rows_fwd", s);
       with_rows(g, b, [&](auto vr) {
@@ -1056,6 +1060,8 @@ struct LinOps {
                                                          (const C*)b.tw, (C*)b.R1, T(scale), g.w);
       });
     }
+    if constexpr (kMfma)
+      if (b.frag && g.Lw == L) return linpatch_cols_mfma(g, b, 0, NllFinal{}, s);
     TimedLaunch _t("lp_cols_fwd", s);
     with_fold(g, [&](auto fold) {
       constexpr bool FOLD = decltype(fold)::value;
@@ -1067,10 +1073,12 @@ struct LinOps {
                                                            NllFinal{});
       });
     });
+    return kOk;
   }
 
-  static void adj_front(const LinGeom& g, const LinBuffers& b, const void* gy, hipStream_t s,
-                        NllFinal fin = NllFinal{}) {
+  static int adj_front(const LinGeom& g, const LinBuffers& b, const void* gy, hipStream_t s,
+                       NllFinal fin = NllFinal{}) {
+    int st = kOk;
     with_fold(g, [&](auto fold) {
       constexpr bool FOLD = decltype(fold)::value;
       {
@@ -1082,6 +1090,12 @@ struct LinOps {
               g, (const T*)gy, (const T*)b.tent, (const C*)b.tw, (C*)b.R1, T(1), g.Lw);
         });
       }
+      if constexpr (kMfma && !FOLD) {
+        if (b.frag) {
+          st = linpatch_cols_mfma(g, b, 1, fin, s);
+          return;
+        }
+      }
       TimedLaunch _t("lp_cols_adj", s);
       with_cols(g, b, [&](auto vc) {
         constexpr int VC = decltype(vc)::value;
@@ -1091,6 +1105,7 @@ struct LinOps {
                                                            fin);
       });
     });
+    return st;
   }
 
   // Row tile of the signal and cotangent row passes: VR / b.row_div row
@@ -1160,26 +1175,26 @@ struct LinOps {
       case LinCall::kPrepare:
         return prepare(g, b, c.in, c.layout, s);
       case LinCall::kForward:
-        fwd_front(g, b, c.in, c.scale, s);
+        if (int st = fwd_front(g, b, c.in, c.scale, s)) return st;
         fwd_out<0>(g, b, (T*)c.out, nullptr, nullptr, nullptr, nullptr, s, "lp_fwd_out");
         return launch_status("linpatch_forward");
       case LinCall::kForwardPoisson:
-        fwd_front(g, b, c.in, c.scale, s);
+        if (int st = fwd_front(g, b, c.in, c.scale, s)) return st;
         fwd_out<1>(g, b, (T*)c.lam, (const T*)c.expo, c.counts, (T*)c.resid, c.partial, s,
                    "lp_fwd_out_poisson");
         return launch_status("linpatch_forward_poisson");
       case LinCall::kForwardMetric:
-        fwd_front(g, b, c.in, c.scale, s);
+        if (int st = fwd_front(g, b, c.in, c.scale, s)) return st;
         fwd_out<2>(g, b, (T*)c.lam, (const T*)c.expo, c.counts, (T*)c.out, nullptr, s,
                    "lp_fwd_out_metric");
         return launch_status("linpatch_forward_metric");
       case LinCall::kAdjoint:
-        adj_front(g, b, c.in, s);
+        if (int st = adj_front(g, b, c.in, s)) return st;
         adj_out<0>(g, b, g.S * (g.x_shared ? g.E : g.J), (T*)c.out, c.scale, nullptr, nullptr,
                    s, "lp_adj_out");
         return launch_status("linpatch_adjoint");
       case LinCall::kAdjointGp:
-        adj_front(g, b, c.in, s, c.fin);
+        if (int st = adj_front(g, b, c.in, s, c.fin)) return st;
         adj_out<1>(g, b, g.S * g.E, nullptr, c.scale, (const T*)c.sky, (T*)c.out, s,
                    "lp_adj_out_gp");
         return launch_status("linpatch_adjoint_gp");
