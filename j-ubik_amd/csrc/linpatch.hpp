@@ -9,7 +9,9 @@ struct LinBuffers {
   const void* tw;    // L twiddles exp(-2 pi i e / L)
   const void* tent;  // w tent weights (jubik/convolve.py:18-25)
   void* KT;          // prepared spectra, (M, P, Ek) x L x L2p (alpha in column 0)
-  void* R1;          // row-pass workspace, S x J x P x L x L2p
+  void* R1;          // row-pass workspace: shared row spectra (S x J x n x H x L2p) of the
+                     // signal and cotangent passes, or (M, P, Ek) x w x L2p of the kernel
+                     // preparation; allocated S x J x P x L x L2p (no smaller)
   void* R2;          // column-pass workspace, S x J x P x L x L2p
   void* KB;          // beta of the packed DC/Nyquist column, (M, P, Ek) x L
   int col_div;       // column-pass tile: 1, 2 or 4 (a full, half or quarter tile)

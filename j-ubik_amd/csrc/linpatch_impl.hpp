@@ -51,7 +51,16 @@ __device__ __forceinline__ int cot_col(const LinGeom& g, int f) {
 }
 
 // ------------------------------------------------------------------ rows
-// grid (ceil(nrows/2 / V), P, Z); Z = jobs (signal modes) or M*Ek (kernel).
+// Kernel modes: grid (ceil(w/2 / V), P, M*Ek), one spectrum's w window rows
+// per (patch, kernel).  Signal and cotangent modes: grid (ceil(H/2 / V), n,
+// jobs), the row spectra of image rows shared by every patch row: for patch
+// column iy and image row i, the row FFT of the window columns of iy
+//   signal:    tent(c) * scale * x[i, sy + c - h], c < w, at frame f = c + mf;
+//   cotangent: y[i, sy + cot_col(f) - h - m] at frame f < L,
+// stored at R1[(job * n + iy) * H + i].  A window's rows then differ from
+// these only by the row tent weight (signal), which the column pass applies
+// on load: each image row is transformed once per patch column instead of
+// once per covering patch (two patch rows forward, L / d in the adjoint).
 // VRC row pairs per work-group: RowTile<T, L>::V, or a half or quarter of it
 // (chosen by the autotuner for grids with too few row tiles).
 // FOLD (cotangent mode): the row of L frame positions f holds the periodic
@@ -68,24 +77,16 @@ __global__ __launch_bounds__(NT) void lp_rows(LinGeom g, const T* __restrict__ s
   __shared__ cplx<T> buf[V * VS];
   __shared__ cplx<T> tw[L];
   const int tid = threadIdx.x;
-  const int p = blockIdx.y;
+  const int p = blockIdx.y;  // patch (kernel modes) or patch column (signal modes)
   const int z = blockIdx.z;
   const int q0 = blockIdx.x * V;
-  const int ix = p % g.n, iy = p / g.n;
-  const int sx = start_x(g, ix), sy = start_y(g, iy);
-  // Cotangent windows: rows whose image row sx + r - h - m lies outside the
-  // image are zero; the adjoint column pass never reads them, so they are
-  // neither transformed nor stored (the whole tile returns when all are).
-  const int row0 = (MODE == ROW_COTANGENT) ? sx - g.h - g.m : 0;
-  if (MODE == ROW_COTANGENT &&
-      (row0 + 2 * q0 + 2 * V <= 0 || row0 + 2 * q0 >= g.H))
-    return;
+  const int sy = start_y(g, p < g.n ? p : 0);
 
   int64_t r1base;
   const T* img;
   if constexpr (MODE == ROW_SIGNAL || MODE == ROW_COTANGENT) {
     const int job = z;
-    r1base = (int64_t)job * g.P + p;
+    r1base = (int64_t)job * g.n + p;
     int im = job;  // shared input: image (sample, energy) of the sky stack
     if (MODE == ROW_SIGNAL && g.x_shared) im = (job / g.J) * g.E + job % g.E;
     img = src + (int64_t)im * g.H * g.W;
@@ -116,19 +117,8 @@ __global__ __launch_bounds__(NT) void lp_rows(LinGeom g, const T* __restrict__ s
   const Gather gi = make_gather(img, (int64_t)g.H * g.W);
   if constexpr (kColumns) {
     T av[TPT][V], bv[TPT][V];
-    // Tent weights from the plan's table (the same IEEE double values in T):
-    // the row weights are uniform (scalar loads), the column weights one
-    // zero-reading gather per column.
-    T tr0[V], tr1[V];
-#pragma unroll
-    for (int v = 0; v < V; ++v) {
-      const int r0 = 2 * (q0 + v);
-      tr0[v] = tr1[v] = T(1);
-      if (MODE == ROW_SIGNAL && r0 < nrows) {
-        tr0[v] = tent[r0];
-        tr1[v] = tent[r0 + 1];
-      }
-    }
+    // Column tent weights from the plan's table (the same IEEE double values
+    // in T), one zero-reading gather per column.
     const Gather gt = make_gather(tent, MODE == ROW_SIGNAL ? (int64_t)g.w : 0);
 #pragma unroll
     for (int u = 0; u < TPT; ++u) {
@@ -146,17 +136,15 @@ __global__ __launch_bounds__(NT) void lp_rows(LinGeom g, const T* __restrict__ s
         cok = j >= 0 && j < g.W;
       }
       if (L % NT != 0 && f >= L) cok = false;
-      const int ioff = MODE == ROW_SIGNAL ? sx - g.h : sx - g.h - g.m;
       // Elements outside the image read zero through the gather, so the
       // loads of all rows are in flight together.
 #pragma unroll
       for (int v = 0; v < V; ++v) {
-        const int r0 = 2 * (q0 + v);
-        const int i0 = ioff + r0;
-        const bool ra = cok && r0 < nrows && i0 >= 0 && i0 < g.H;
-        const bool rb = cok && r0 < nrows && i0 + 1 >= 0 && i0 + 1 < g.H;
-        av[u][v] = tr0[v] * wc * gather_load<T>(gi, gather_off<T>(gi, ra, i0 * g.W + j));
-        bv[u][v] = tr1[v] * wc * gather_load<T>(gi, gather_off<T>(gi, rb, (i0 + 1) * g.W + j));
+        const int i0 = 2 * (q0 + v);  // image row
+        const bool ra = cok && i0 < g.H;
+        const bool rb = cok && i0 + 1 < g.H;
+        av[u][v] = wc * gather_load<T>(gi, gather_off<T>(gi, ra, i0 * g.W + j));
+        bv[u][v] = wc * gather_load<T>(gi, gather_off<T>(gi, rb, (i0 + 1) * g.W + j));
       }
     }
 #pragma unroll
@@ -184,16 +172,14 @@ __global__ __launch_bounds__(NT) void lp_rows(LinGeom g, const T* __restrict__ s
         // Zero-reading gathers, as in the column mapping above.
         const int c = f - g.mf;
         const int j = MODE == ROW_SIGNAL ? sy + c - g.h : sy + cot_col<FOLD>(g, f) - g.h - g.m;
-        const int i0 = MODE == ROW_SIGNAL ? sx + r0 - g.h : sx + r0 - g.h - g.m;
+        const int i0 = r0;  // image row
         const bool cok = r0 < nrows && (MODE != ROW_SIGNAL || (c >= 0 && c < g.w)) && j >= 0 &&
                          j < g.W;
-        const bool ra = cok && i0 >= 0 && i0 < g.H;
-        const bool rb = cok && i0 + 1 >= 0 && i0 + 1 < g.H;
+        const bool ra = cok && i0 < g.H;
+        const bool rb = cok && i0 + 1 < g.H;
         const T wc = MODE == ROW_SIGNAL && cok ? tent_weight<T>(g.d, step, c) * scale : T(1);
-        const T wa = MODE == ROW_SIGNAL ? tent_weight<T>(g.d, step, r0) * wc : T(1);
-        const T wb = MODE == ROW_SIGNAL ? tent_weight<T>(g.d, step, r0 + 1) * wc : T(1);
-        av[u] = wa * gather_load<T>(gi, gather_off<T>(gi, ra, i0 * g.W + j));
-        bv[u] = wb * gather_load<T>(gi, gather_off<T>(gi, rb, (i0 + 1) * g.W + j));
+        av[u] = wc * gather_load<T>(gi, gather_off<T>(gi, ra, i0 * g.W + j));
+        bv[u] = wc * gather_load<T>(gi, gather_off<T>(gi, rb, (i0 + 1) * g.W + j));
       } else if (r0 < nrows) {
         if constexpr (MODE == ROW_KERNEL_WIN) {
           const int c = f - g.mf;
@@ -243,8 +229,8 @@ __global__ __launch_bounds__(NT) void lp_rows(LinGeom g, const T* __restrict__ s
     if constexpr (kWaveUniform) v = __builtin_amdgcn_readfirstlane(v);
     const int r0 = 2 * (q0 + v);
     if (r0 >= nrows) continue;
-    const bool st0 = MODE != ROW_COTANGENT || (row0 + r0 >= 0 && row0 + r0 < g.H);
-    const bool st1 = MODE != ROW_COTANGENT || (row0 + r0 + 1 >= 0 && row0 + r0 + 1 < g.H);
+    const bool st0 = true;
+    const bool st1 = r0 + 1 < nrows;  // an odd image height leaves one row
     const cplx<T> zk = buf[v * VS + k];
     cplx<T> A, B;
     if (k == 0) {
@@ -308,6 +294,12 @@ __device__ __forceinline__ void share_packed(cplx<T>* buf, typename FL::Regs& la
 // FOLD: fold mode of LinGeom.  The forward folds the L output rows into the
 // Lw window rows (through LDS); the adjoint loads the periodic extension of
 // the Lw cotangent rows.
+// Signal modes read the shared row spectra of lp_rows (R1[(job * n + iy) *
+// H + image row]); the patches of one patch column that share rows (two
+// forward, about L / d in the adjoint) run close together on one XCD (the
+// grid order below) and meet them in its L2.  The forward multiplies each
+// window row by its tent weight on load, through a zero-reading gather so
+// that the multiply does not pull each load into a branch of its own.
 template <typename T, int L, int MODE, int VC = Tile<T, L>::V, bool FOLD = false>
 __global__ __launch_bounds__(NT) void lp_cols(LinGeom g, const cplx<T>* __restrict__ R1,
                                               const cplx<T>* __restrict__ KT,
@@ -344,8 +336,11 @@ __global__ __launch_bounds__(NT) void lp_cols(LinGeom g, const cplx<T>* __restri
     spec = spec_index(g, z, p);
   }
   const int lw = FOLD ? g.Lw : L;  // window rows of the cotangent / output
-  const int nin = (MODE == COL_ADJ) ? lw : g.w;
-  const cplx<T>* in = R1 + idx * L2P * nin;
+  const cplx<T>* in;
+  if constexpr (MODE == COL_KERNEL)
+    in = R1 + idx * L2P * g.w;
+  else
+    in = R1 + ((int64_t)(smp * g.J + z) * g.n + p / g.n) * g.H * L2P;
   const int m = g.mf, w = g.w;
 
   // Twiddles are fetched into registers now and stored to LDS once the tile
@@ -393,16 +388,25 @@ __global__ __launch_bounds__(NT) void lp_cols(LinGeom g, const cplx<T>* __restri
   // are zero and were never stored; the forward's output rows there are
   // never read.
   const int row0 = start_x(g, p % g.n) - g.h - g.m;
+  const Gather gin = make_gather(in, MODE == COL_FWD ? (int64_t)g.H * L2P : 0);
+  const double tstep = tent_step(g.d);
   auto load_col = [&](int vec, int pos) -> cplx<T> {
     const int k = k0 + vec;
-    if (k >= LH) return cplx<T>{T(0), T(0)};
+    const cplx<T> zero = {T(0), T(0)};
     if constexpr (MODE == COL_ADJ) {
-      const int q = cot_col<FOLD>(g, pos);
-      const int ir = row0 + q;
-      return (ir >= 0 && ir < g.H) ? ld_stream(&in[q * L2P + k]) : cplx<T>{T(0), T(0)};
-    } else {
+      if (k >= LH) return zero;
+      const int ir = row0 + cot_col<FOLD>(g, pos);
+      return (ir >= 0 && ir < g.H) ? ld_stream(&in[ir * L2P + k]) : zero;
+    } else if constexpr (MODE == COL_FWD) {
       const int rr = pos - m;
-      return (rr >= 0 && rr < w) ? ld_stream(&in[rr * L2P + k]) : cplx<T>{T(0), T(0)};
+      const int ir = row0 + g.m + rr;
+      const bool ok = k < LH && rr >= 0 && rr < w && ir >= 0 && ir < g.H;
+      const cplx<T> v = gather_load<cplx<T>>(gin, gather_off<cplx<T>>(gin, ok, ir * L2P + k));
+      return cscale(v, tent_weight<T>(g.d, tstep, rr));
+    } else {
+      if (k >= LH) return zero;
+      const int rr = pos - m;
+      return (rr >= 0 && rr < w) ? ld_stream(&in[rr * L2P + k]) : zero;
     }
   };
   static_assert(F0::Ns == 1, "the first stage takes no twiddles");
@@ -1055,9 +1059,9 @@ struct LinOps {
       TimedLaunch _t("lp_rows_fwd", s);
       with_rows(g, b, [&](auto vr) {
         constexpr int VRC = decltype(vr)::value;
-        dim3 gr((g.w / 2 + VRC - 1) / VRC, g.P, g.S * g.J);
+        dim3 gr(((g.H + 1) / 2 + VRC - 1) / VRC, g.n, g.S * g.J);
         lp_rows<T, L, ROW_SIGNAL, VRC><<<gr, NT, 0, s>>>(g, (const T*)x, (const T*)b.tent,
-                                                         (const C*)b.tw, (C*)b.R1, T(scale), g.w);
+                                                         (const C*)b.tw, (C*)b.R1, T(scale), g.H);
       });
     }
     if constexpr (kMfma)
@@ -1085,9 +1089,9 @@ struct LinOps {
         TimedLaunch _t("lp_rows_adj", s);
         with_rows(g, b, [&](auto vr) {
           constexpr int VRC = decltype(vr)::value;
-          dim3 gr((g.Lw / 2 + VRC - 1) / VRC, g.P, g.S * g.J);
+          dim3 gr(((g.H + 1) / 2 + VRC - 1) / VRC, g.n, g.S * g.J);
           lp_rows<T, L, ROW_COTANGENT, VRC, FOLD><<<gr, NT, 0, s>>>(
-              g, (const T*)gy, (const T*)b.tent, (const C*)b.tw, (C*)b.R1, T(1), g.Lw);
+              g, (const T*)gy, (const T*)b.tent, (const C*)b.tw, (C*)b.R1, T(1), g.H);
         });
       }
       if constexpr (kMfma && !FOLD) {

@@ -202,7 +202,8 @@ __global__ __launch_bounds__(MNT) void lp_cols_mfma(LinGeom g, const bf16x8* __r
                                                    const cplx<float>* __restrict__ twg,
                                                    cplx<float>* __restrict__ out,
                                                    const cplx<float>* __restrict__ KB,
-                                                   NllFinal fin, int nitems) {
+                                                   NllFinal fin, int nitems,
+                                                   const float* __restrict__ tent) {
   constexpr int L2P = half_stride(ML);
   constexpr int PT = MV * ML / MNT;  // tile elements per thread
   __shared__ bf16x8 F[NFRAG_LDS];
@@ -240,7 +241,7 @@ __global__ __launch_bounds__(MNT) void lp_cols_mfma(LinGeom g, const bf16x8* __r
   const cplx<float> w1 = twg[tid & 127], w2 = twg[2 * (tid & 127)];
   const int slot = xcd_tile(blockIdx.x, gridDim.x);
   struct Item {
-    int64_t idx, spec;
+    int64_t idx, spec, rows;  // output slab, spectrum, shared row spectra of the input
     int k0, row0;
   };
   auto locate = [&](int item) {
@@ -252,6 +253,7 @@ __global__ __launch_bounds__(MNT) void lp_cols_mfma(LinGeom g, const bf16x8* __r
     const int p = item % g.P;
     const int z = item / g.P;
     it.idx = (int64_t)(smp * g.J + z) * g.P + p;
+    it.rows = ((int64_t)(smp * g.J + z) * g.n + p / g.n) * g.H;
     it.spec = spec_index(g, z, p);
     // image row of frame row 0: rows outside the image are zero on input
     // (never stored) and never read on output
@@ -261,7 +263,10 @@ __global__ __launch_bounds__(MNT) void lp_cols_mfma(LinGeom g, const bf16x8* __r
   // element e = tid + u * MNT of a tile: column e % MV, frame row e / MV
   cplx<float> xin[PT], kv[PT], be[PT];
   auto fetch = [&](const Item& it) {
-    const cplx<float>* in = R1 + it.idx * L2P * (ADJ ? ML : g.w);
+    // the row spectra lp_rows shares between the patches of a patch column
+    // (plain loads: neighbouring patches read the same rows); the forward
+    // applies the window row's tent weight
+    const cplx<float>* in = R1 + it.rows * L2P;
     const cplx<float>* ks = KT + it.spec * L2P * ML;
 #pragma unroll
     for (int u = 0; u < PT; ++u) {
@@ -270,10 +275,12 @@ __global__ __launch_bounds__(MNT) void lp_cols_mfma(LinGeom g, const bf16x8* __r
       cplx<float> v = {0.f, 0.f};
       if (ADJ) {
         const int ir = it.row0 + pos;
-        if (ir >= 0 && ir < g.H) v = ld_stream(&in[pos * L2P + it.k0 + c]);
+        if (ir >= 0 && ir < g.H) v = in[(int64_t)ir * L2P + it.k0 + c];
       } else {
         const int rr = pos - g.m;
-        if (rr >= 0 && rr < g.w) v = ld_stream(&in[rr * L2P + it.k0 + c]);
+        const int ir = it.row0 + g.m + rr;
+        if (rr >= 0 && rr < g.w && ir >= 0 && ir < g.H)
+          v = cscale(in[(int64_t)ir * L2P + it.k0 + c], tent[rr]);
       }
       xin[u] = v;
       kv[u] = ks[pos * L2P + it.k0 + c];
@@ -363,12 +370,14 @@ int linpatch_cols_mfma(const LinGeom& g, const LinBuffers& b, int adjoint, NllFi
   const auto* tw = static_cast<const cplx<float>*>(b.tw);
   auto* R2 = static_cast<cplx<float>*>(b.R2);
   const auto* KB = static_cast<const cplx<float>*>(b.KB);
+  const auto* tent = static_cast<const float*>(b.tent);
   if (adjoint) {
     TimedLaunch _t("lp_cols_adj_mfma", s);
-    lp_cols_mfma<true><<<grid, MNT, 0, s>>>(g, F, R1, KT, tw, R2, KB, fin, nitems);
+    lp_cols_mfma<true><<<grid, MNT, 0, s>>>(g, F, R1, KT, tw, R2, KB, fin, nitems, tent);
   } else {
     TimedLaunch _t("lp_cols_fwd_mfma", s);
-    lp_cols_mfma<false><<<grid, MNT, 0, s>>>(g, F, R1, KT, tw, R2, KB, NllFinal{}, nitems);
+    lp_cols_mfma<false><<<grid, MNT, 0, s>>>(g, F, R1, KT, tw, R2, KB, NllFinal{}, nitems,
+                                              tent);
   }
   return check_hip(hipGetLastError(), "linpatch_cols_mfma");
 }
