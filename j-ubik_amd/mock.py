@@ -1,12 +1,16 @@
 """Mock data: Poisson counts drawn on the GPU from the expected counts.
 
 jubik/data.py:81-151 (create_mock_data) evaluates lam = R(sky(position)),
-read out per key, and draws jax.random.poisson per key (data.py:139-145),
-casting to int.  Here the draw is libjubik_hip's counter-based Poisson
+read out per key, draws jax.random.poisson per key (data.py:139-145),
+casting to int, and saves the data and the position as pickles of their
+key -> array dicts (data.py:146-149, utils.py:35-47, 170-182).  Here the draw is libjubik_hip's counter-based Poisson
 sampler: each count is a pure function of (seed, flat index, lam), so the
 same seed reproduces the data bit for bit and a sub-array draws the same
 counts as the full array.
 """
+import os
+import pickle
+
 import numpy as np
 import torch
 
@@ -45,3 +49,28 @@ def create_mock_counts(lam, keep, seed=0, keys=None):
     counts = poisson_sample(lam, seed)
     counts = counts.cpu().numpy() if isinstance(counts, torch.Tensor) else counts
     return Vector({k: counts[i][keep[i]].astype(np.int64) for i, k in enumerate(keys)})
+
+
+def _host_tree(tree):
+    out = {}
+    for k, v in dict(tree).items():
+        out[k] = v.detach().cpu().numpy() if isinstance(v, torch.Tensor) else np.asarray(v)
+    return out
+
+
+def save_mock_data(data, position, file_info):
+    """Write mock data and the position it was drawn at as the reference
+    does (data.py:146-149): the data Vector's key -> counts dict to
+    file_info["res_dir"]/file_info["data_dict"] and the position's key ->
+    array dict to file_info["res_dir"]/file_info["pos_dict"], each with
+    pickle.dump, creating the directory if needed.  Returns both paths."""
+    res_dir = file_info["res_dir"]
+    os.makedirs(res_dir, exist_ok=True)
+    paths = (os.path.join(res_dir, file_info["data_dict"]),
+             os.path.join(res_dir, file_info["pos_dict"]))
+    tree = data.tree if isinstance(data, Vector) else data
+    pos = position.tree if isinstance(position, Vector) else position
+    for obj, path in ((tree, paths[0]), (pos, paths[1])):
+        with open(path, "wb") as f:
+            pickle.dump(_host_tree(obj), f)
+    return paths

@@ -225,3 +225,20 @@ def test_compute_path_requires_a_device(ja):
 
 def test_integrate_matches_reference_fixture(ja):
     assert ja.integrate(np.ones((3, 4)), ja.Domain((3, 4), (0.5, 0.25)), (0, 1)) == 1.5
+
+
+def test_save_mock_data_writes_the_reference_pickles(ja, tmp_path):
+    """data.py:146-149: the data Vector's dict and the position's dict are
+    pickled under res_dir with the file names of file_info."""
+    import pickle
+    data = ja.Vector({"tm_1": np.array([0, 3, 7]), "tm_2": np.array([1, 1])})
+    pos = {"xi_ref": np.arange(6.0).reshape(2, 3), "xi_alpha": np.ones((2, 3))}
+    info = {"res_dir": str(tmp_path / "mock"), "data_dict": "data.pkl", "pos_dict": "pos.pkl"}
+    dpath, ppath = ja.save_mock_data(data, pos, info)
+    assert dpath == str(tmp_path / "mock" / "data.pkl") and ppath.endswith("pos.pkl")
+    with open(dpath, "rb") as f:
+        d = pickle.load(f)
+    with open(ppath, "rb") as f:
+        p = pickle.load(f)
+    assert set(d) == {"tm_1", "tm_2"} and np.array_equal(d["tm_1"], [0, 3, 7])
+    assert np.array_equal(p["xi_ref"], pos["xi_ref"]) and np.array_equal(p["xi_alpha"], np.ones((2, 3)))
