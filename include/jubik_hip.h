@@ -128,13 +128,17 @@ int jubik_slice_patches(int dtype, const void* x, int64_t B, int64_t H, int64_t 
                         int64_t margin, void* out, void* stream);
 int jubik_slice_patches_adjoint(int dtype, const void* patches, int64_t B, int64_t H, int64_t W,
                                 int64_t n, int64_t margin, void* x, void* stream);
-/* convolve of jubik/convolve.py:148-176 over the last two axes of real
- * N x N arrays: out[b] = dvol * Re(IFFT2(FFT2(x[b / (nby / nbx)]) *
- * FFT2(y[b]))).  ``work`` holds jubik_convolve_work_bytes(dtype, nby, N)
- * bytes of device memory. */
-int64_t jubik_convolve_work_bytes(int dtype, int64_t nby, int64_t N);
-int jubik_convolve(int dtype, const void* x, const void* y, int64_t nbx, int64_t nby, int64_t N,
-                   double dvol, void* work, void* out, void* stream);
+/* convolve of jubik/convolve.py:148-176 over two axes of real H x W
+ * arrays (the last two after the caller moves them there): out[b] = dvol *
+ * Re(IFFT2(FFT2(x[b / (nby / nbx)]) * FFT2(y[b]))), the circular
+ * convolution of the reference's fftn / ifftn.  Any H, W up to 2048: a
+ * length without its own FFT plan runs zero-padded at a compiled length
+ * >= 2 n - 1 and is folded back (the same values).  ``work`` holds
+ * jubik_convolve_work_bytes(dtype, nby, H, W) bytes of device memory (-1:
+ * unsupported sizes). */
+int64_t jubik_convolve_work_bytes(int dtype, int64_t nby, int64_t H, int64_t W);
+int jubik_convolve(int dtype, const void* x, const void* y, int64_t nbx, int64_t nby, int64_t H,
+                   int64_t W, double dvol, void* work, void* out, void* stream);
 
 /* ---------------------------------------------------------------- response
  * Exposure closure of jubik/response.py:12-45 (``exposures * x`` with

@@ -36,7 +36,7 @@ SIGNATURES = {
     "jubik_linpatch_adjoint": [_vp, _vp, _vp, _dbl, _vp],
     "jubik_slice_patches": [_int, _vp, _i64, _i64, _i64, _i64, _i64, _vp, _vp],
     "jubik_slice_patches_adjoint": [_int, _vp, _i64, _i64, _i64, _i64, _i64, _vp, _vp],
-    "jubik_convolve": [_int, _vp, _vp, _i64, _i64, _i64, _dbl, _vp, _vp, _vp],
+    "jubik_convolve": [_int, _vp, _vp, _i64, _i64, _i64, _i64, _dbl, _vp, _vp, _vp],
     "jubik_exposure_apply": [_int, _vp, _i64, _vp, _i64, _vp, _i64, _vp],
     "jubik_readout_gather": [_int, _vp, _vp, _i64, _vp, _vp],
     "jubik_readout_scatter": [_int, _vp, _vp, _i64, _i64, _vp, _vp],
@@ -122,7 +122,7 @@ def load():
             fn = getattr(lib, name)
             fn.argtypes = args
             fn.restype = C.c_int
-        lib.jubik_convolve_work_bytes.argtypes = [_int, _i64, _i64]
+        lib.jubik_convolve_work_bytes.argtypes = [_int, _i64, _i64, _i64]
         lib.jubik_convolve_work_bytes.restype = _i64
         _lib = lib
         return lib

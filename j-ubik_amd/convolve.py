@@ -338,47 +338,66 @@ def slice_patches_transpose(patches, shape, n_patches_per_axis, additional_margi
 
 
 def convolve(x, y, domain, axes):
-    """FFT convolution over the last two axes, jubik/convolve.py:148-176.
+    """FFT convolution, jubik/convolve.py:148-176.
 
-    ``x`` is the kernel, ``y`` the signal; returns dvol * Re(ifft(fft(x)
-    fft(y))).  When y has one more axis than x, x is broadcast over it and
-    the reference's notice is printed.  Square transforms of the compiled
-    sizes (4 ... 1024) run on the GPU; others raise NotImplementedError.
+    ``x`` is the kernel, ``y`` the signal; returns dvol * Re(ifftn(fftn(x)
+    fftn(y))) over ``axes``, the circular convolution.  One or two axes of
+    any lengths up to 2048; arrays of equal rank may convolve over any two
+    of their axes (they are moved to the end on the device and back).  When
+    y has one more axis than x, x is broadcast over it (the last two axes
+    only) and the reference's notice is printed.
     """
     nd_x, nd_y = len(x.shape), len(y.shape)
     ax = tuple(a % nd_y for a in axes)
-    if ax != (nd_y - 2, nd_y - 1):
-        raise NotImplementedError("convolve runs over the last two axes only")
+    if len(set(ax)) != len(ax) or not 1 <= len(ax) <= 2:
+        raise NotImplementedError("convolve runs over one or two distinct axes")
     dvol = float(math.prod(domain.distances[a] for a in axes))
-    if nd_y > nd_x:
+    broadcast = nd_y > nd_x
+    if broadcast:
         print("kernel_shape:", tuple(x.shape))
         print("signal_shape:", tuple(y.shape))
         print("Dimension Inconsistency. Broadcasting PSFs")
-    N = int(y.shape[-1])
-    if int(y.shape[-2]) != N or tuple(x.shape[-2:]) != (N, N):
-        raise NotImplementedError("convolve needs square arrays of equal size")
+        if ax != (nd_y - 2, nd_y - 1):
+            raise NotImplementedError("the broadcast branch convolves over the last two axes")
     dev.require_device()
     dtype = dev.float_dtype(x, y)
     xt, yt = dev.to_device(x, dtype), dev.to_device(y, dtype)
-    if nd_y > nd_x:
+    if broadcast:
+        H, W = int(yt.shape[-2]), int(yt.shape[-1])
+        if tuple(xt.shape[-2:]) != (H, W):
+            raise ValueError(f"kernel {tuple(xt.shape)} and signal {tuple(yt.shape)} differ on "
+                             "the convolution axes")
         if tuple(yt.shape[:-3]) != tuple(xt.shape[:-2]):
-            xt = xt.unsqueeze(-3).expand(yt.shape[:-3] + (1, N, N)).contiguous()
+            xt = xt.unsqueeze(-3).expand(yt.shape[:-3] + (1, H, W)).contiguous()
             nbx = int(np.prod(yt.shape[:-3])) if yt.dim() > 3 else 1
         else:
             nbx = int(np.prod(xt.shape[:-2])) if xt.dim() > 2 else 1
+        back = None
     else:
         shape = torch.broadcast_shapes(tuple(xt.shape), tuple(yt.shape))
-        if tuple(xt.shape) != tuple(shape):
-            xt = xt.expand(shape).contiguous()
-        if tuple(yt.shape) != tuple(shape):
-            yt = yt.expand(shape).contiguous()
-        nbx = int(np.prod(shape[:-2])) if len(shape) > 2 else 1
+        xt, yt = xt.expand(shape), yt.expand(shape)
+        if len(ax) == 1:  # a unit axis in front of the convolution axis
+            xt, yt = xt.unsqueeze(-1), yt.unsqueeze(-1)
+            ax = (len(shape), ax[0])
+        back = ax
+        xt = torch.movedim(xt, ax, (-2, -1)).contiguous()
+        yt = torch.movedim(yt, ax, (-2, -1)).contiguous()
+        H, W = int(yt.shape[-2]), int(yt.shape[-1])
+        nbx = int(np.prod(xt.shape[:-2])) if xt.dim() > 2 else 1
     nby = int(np.prod(yt.shape[:-2])) if yt.dim() > 2 else 1
     out = torch.empty(tuple(yt.shape), dtype=dtype, device="cuda")
-    work = torch.empty(int(_lib.load().jubik_convolve_work_bytes(dev.code(dtype), nby, N)),
-                       dtype=torch.uint8, device="cuda")
-    _lib.call("jubik_convolve", dev.code(dtype), dev.ptr(xt), dev.ptr(yt), nbx, nby, N, dvol,
-              dev.ptr(work), dev.ptr(out), dev.stream())
+    if out.numel():
+        nbytes = int(_lib.load().jubik_convolve_work_bytes(dev.code(dtype), nby, H, W))
+        if nbytes < 0:
+            raise NotImplementedError(f"convolve axes of lengths {H} x {W}: at most 2048")
+        work = torch.empty(nbytes, dtype=torch.uint8, device="cuda")
+        _lib.call("jubik_convolve", dev.code(dtype), dev.ptr(xt), dev.ptr(yt), nbx, nby, H, W,
+                  dvol, dev.ptr(work), dev.ptr(out), dev.stream())
+    if back is not None:
+        out = torch.movedim(out, (-2, -1), back)
+        if len(axes) == 1:
+            out = out.squeeze(-1)
+        out = out.contiguous()
     return dev.like_input(out, y)
 
 

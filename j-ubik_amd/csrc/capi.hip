@@ -619,34 +619,43 @@ int jubik_slice_patches_adjoint(int dtype, const void* patches, int64_t B, int64
   return slice_patches_adjoint(dtype, patches, B, H, W, n, margin, x, as_stream(stream));
 }
 
-int64_t jubik_convolve_work_bytes(int dtype, int64_t nby, int64_t N) {
-  return (nby * N * N + N) * 2 * (int64_t)elem_size(dtype);
+int64_t jubik_convolve_work_bytes(int dtype, int64_t nby, int64_t H, int64_t W) {
+  const int64_t LH = convolve_fft_length(H), LW = convolve_fft_length(W);
+  if (nby < 1 || W < 1 || !LH || !LW) return -1;
+  return (nby * LH * W + LH + LW) * 2 * (int64_t)elem_size(dtype);
 }
 
-int jubik_convolve(int dtype, const void* x, const void* y, int64_t nbx, int64_t nby, int64_t N,
-                   double dvol, void* work, void* out, void* stream) {
+int jubik_convolve(int dtype, const void* x, const void* y, int64_t nbx, int64_t nby, int64_t H,
+                   int64_t W, double dvol, void* work, void* out, void* stream) {
   if (dtype != 0 && dtype != 1) return fail(kInvalidArgument, "dtype must be 0 or 1");
   if (!x || !y || !work || !out) return fail(kInvalidArgument, "NULL argument");
-  if (!convolve_length_supported((int)N))
-    return fail(kUnsupported, "convolve size " + std::to_string(N) + " is not compiled in");
-  // twiddles go behind the complex work planes
+  if (H < 1 || W < 1) return fail(kInvalidArgument, "convolve axes must not be empty");
+  const int64_t LH = convolve_fft_length(H), LW = convolve_fft_length(W);
+  if (!LH || !LW)
+    return fail(kUnsupported, "convolve axis of length " + std::to_string(!LH ? H : W) +
+                                  " exceeds 2048");
+  // twiddles of both FFT lengths go behind the complex work planes
   const size_t es = elem_size(dtype);
   static thread_local std::vector<unsigned char> host;
-  host.resize(2 * es * N);
-  for (int64_t e = 0; e < N; ++e) {
-    const long double ang = -2.0L * 3.141592653589793238462643383279502884L * e / N;
-    const double c = (double)cosl(ang), sn = (double)sinl(ang);
-    if (dtype == 0) {
-      reinterpret_cast<float*>(host.data())[2 * e] = (float)c;
-      reinterpret_cast<float*>(host.data())[2 * e + 1] = (float)sn;
-    } else {
-      reinterpret_cast<double*>(host.data())[2 * e] = c;
-      reinterpret_cast<double*>(host.data())[2 * e + 1] = sn;
+  host.resize(2 * es * (LH + LW));
+  int64_t at = 0;
+  for (int64_t L : {LH, LW})
+    for (int64_t e = 0; e < L; ++e, ++at) {
+      const long double ang = -2.0L * 3.141592653589793238462643383279502884L * e / L;
+      const double c = (double)cosl(ang), sn = (double)sinl(ang);
+      if (dtype == 0) {
+        reinterpret_cast<float*>(host.data())[2 * at] = (float)c;
+        reinterpret_cast<float*>(host.data())[2 * at + 1] = (float)sn;
+      } else {
+        reinterpret_cast<double*>(host.data())[2 * at] = c;
+        reinterpret_cast<double*>(host.data())[2 * at + 1] = sn;
+      }
     }
-  }
-  char* tw = reinterpret_cast<char*>(work) + (size_t)nby * N * N * 2 * es;
-  JUBIK_HIP(hipMemcpyAsync(tw, host.data(), host.size(), hipMemcpyHostToDevice, as_stream(stream)));
-  return convolve2d(dtype, x, y, nbx, nby, N, dvol, tw, work, out, as_stream(stream));
+  char* twh = reinterpret_cast<char*>(work) + (size_t)nby * LH * W * 2 * es;
+  char* tww = twh + (size_t)LH * 2 * es;
+  JUBIK_HIP(hipMemcpyAsync(twh, host.data(), host.size(), hipMemcpyHostToDevice,
+                           as_stream(stream)));
+  return convolve2d(dtype, x, y, nbx, nby, H, W, dvol, twh, tww, work, out, as_stream(stream));
 }
 
 // ------------------------------------------------------------------ response

@@ -119,12 +119,22 @@ int slice_patches_adjoint(int dtype, const void* patches, int64_t B, int64_t H, 
 }
 
 // ------------------------------------------------------------ convolve
-// Real N x N arrays: c = x_kb + i y_b is transformed with one complex 2-D
-// FFT; the spectra of x and y are separated with Z(-k), multiplied, and
-// transformed back.  Batch b uses kernel kb = b / (nby / nbx).
+// Real arrays over two axes of lengths H x W: c = x_kb + i y_b is
+// transformed with one complex 2-D FFT, the spectra of x and y are separated
+// with Z(-k), multiplied, and transformed back.  Batch b uses kernel kb = b /
+// (nby / nbx).  Each axis runs at FFT length LH (LW): its own length when it
+// has a compiled plan, else the smallest compiled length >= 2 n - 1 with the
+// arrays zero-padded.  The size-L circular convolution of the padded arrays
+// is then their linear convolution g (support 2 n - 1), and the size-n
+// circular convolution the reference computes is the fold g[u] + g[u + n]:
+// the row pass folds the columns after its inverse row FFT, the inverse
+// column pass folds the rows (the fold commutes with the transform along the
+// other axis).
 struct ConvGeom {
-  int N, nbx, nby;
-  double scale;  // dvol / N^2
+  int H, W;      // array lengths of the two axes
+  int LH, LW;    // their FFT lengths
+  int nbx, nby;
+  double scale;  // dvol / (LH LW)
 };
 
 template <typename T, int N>
@@ -132,8 +142,10 @@ struct ConvTile {
   static constexpr int V = Tile<T, N>::V < N ? Tile<T, N>::V : N;
 };
 
-// PASS 0: forward column FFT of x + iy into Tw; PASS 1: inverse column FFT of
-// Tw, real part times scale into out.  grid (N / V, nby).
+// Column FFTs of length N = LH over the W columns, V per work-group.  PASS 0:
+// forward FFT of x + i y (rows >= H zero) into the work planes (nby, LH, W);
+// PASS 1: inverse FFT of the work planes, fold rows r + H onto r, real part
+// times scale into out.  grid (ceil(W / V), nby).
 template <typename T, int N, int PASS>
 __global__ __launch_bounds__(PNT) void k_conv_cols(ConvGeom g, const T* __restrict__ x,
                                                    const T* __restrict__ y,
@@ -147,27 +159,42 @@ __global__ __launch_bounds__(PNT) void k_conv_cols(ConvGeom g, const T* __restri
   const int b = blockIdx.y;
   const int kb = b / (g.nby / g.nbx);
   const int c0 = blockIdx.x * V;
-  const int64_t plane = (int64_t)N * N;
+  const int64_t plane = (int64_t)g.H * g.W;
+  const int64_t wplane = (int64_t)N * g.W;
   load_twiddles(tw, twg, N, tid, PNT);
   for (int e = tid; e < V * N; e += PNT) {
     const int r = e / V, v = e - (e / V) * V;
-    const int64_t o = (int64_t)r * N + c0 + v;
-    buf[v * VS + r] = PASS == 0 ? cplx<T>{x[kb * plane + o], y[b * plane + o]} : Tw[b * plane + o];
+    const int c = c0 + v;
+    cplx<T> val = {T(0), T(0)};
+    if (c < g.W) {
+      const int64_t o = (int64_t)r * g.W + c;
+      if (PASS == 1)
+        val = Tw[b * wplane + o];
+      else if (r < g.H)
+        val = cplx<T>{x[kb * plane + o], y[b * plane + o]};
+    }
+    buf[v * VS + r] = val;
   }
   __syncthreads();
   fft_lds<T, N, V, VS, PNT, PASS == 1, kFewVectorPlan<N, V, PNT>>(buf, tw, tid);
   for (int e = tid; e < V * N; e += PNT) {
     const int r = e / V, v = e - (e / V) * V;
-    const int64_t o = (int64_t)r * N + c0 + v;
-    if (PASS == 0)
-      Tw[b * plane + o] = buf[v * VS + r];
-    else
-      out[b * plane + o] = T(g.scale) * buf[v * VS + r].re;
+    const int c = c0 + v;
+    if (c >= g.W) continue;
+    const int64_t o = (int64_t)r * g.W + c;
+    if (PASS == 0) {
+      Tw[b * wplane + o] = buf[v * VS + r];
+    } else if (r < g.H) {
+      T s = buf[v * VS + r].re;
+      if (N != g.H && r + g.H < N) s += buf[v * VS + r + g.H].re;
+      out[b * plane + o] = T(g.scale) * s;
+    }
   }
 }
 
-// Row pair (r, -r): forward row FFTs, spectrum product X Y, inverse row FFTs.
-// grid (N / 2 + 1, nby).
+// Row pair (r, -r) of length N = LW: forward row FFTs (columns >= W zero),
+// spectrum product X Y, inverse row FFTs, columns c + W folded onto c, the W
+// folded columns written back.  grid (LH / 2 + 1, nby).
 template <typename T, int N>
 __global__ __launch_bounds__(PNT) void k_conv_rows(ConvGeom g, const cplx<T>* __restrict__ twg,
                                                    cplx<T>* __restrict__ Tw) {
@@ -175,12 +202,12 @@ __global__ __launch_bounds__(PNT) void k_conv_rows(ConvGeom g, const cplx<T>* __
   __shared__ cplx<T> buf[2 * VS];
   __shared__ cplx<T> tw[N];
   const int tid = threadIdx.x;
-  const int r0 = blockIdx.x, r1 = (N - r0) % N;
-  cplx<T>* base = Tw + (int64_t)blockIdx.y * N * N;
+  const int r0 = blockIdx.x, r1 = (g.LH - r0) % g.LH;
+  cplx<T>* base = Tw + (int64_t)blockIdx.y * g.LH * g.W;
   load_twiddles(tw, twg, N, tid, PNT);
   for (int e = tid; e < 2 * N; e += PNT) {
     const int v = e / N, k = e - (e / N) * N;
-    buf[v * VS + k] = base[(int64_t)(v ? r1 : r0) * N + k];
+    buf[v * VS + k] = k < g.W ? base[(int64_t)(v ? r1 : r0) * g.W + k] : cplx<T>{T(0), T(0)};
   }
   __syncthreads();
   fft_lds<T, N, 2, VS, PNT, false, kFewVectorPlan<N, 2, PNT>>(buf, tw, tid);
@@ -207,52 +234,87 @@ __global__ __launch_bounds__(PNT) void k_conv_rows(ConvGeom g, const cplx<T>* __
   __syncthreads();
   fft_lds<T, N, 2, VS, PNT, true, kFewVectorPlan<N, 2, PNT>>(buf, tw, tid);
   const int nrows = r0 == r1 ? 1 : 2;
-  for (int e = tid; e < nrows * N; e += PNT) {
-    const int v = e / N, k = e - (e / N) * N;
-    base[(int64_t)(v ? r1 : r0) * N + k] = buf[v * VS + k];
+  for (int e = tid; e < nrows * g.W; e += PNT) {
+    const int v = e / g.W, k = e - v * g.W;
+    cplx<T> z = buf[v * VS + k];
+    if (N != g.W && k + g.W < N) z = cadd(z, buf[v * VS + k + g.W]);
+    base[(int64_t)(v ? r1 : r0) * g.W + k] = z;
   }
 }
 
-#define JUBIK_CONV_LENGTHS(X) X(4) X(8) X(16) X(32) X(64) X(96) X(128) X(192) X(256) X(384) X(512) X(1024)
+#define JUBIK_CONV_LENGTHS(X) \
+  X(4) X(8) X(16) X(32) X(64) X(96) X(128) X(192) X(256) X(384) X(512) X(768) X(1024) X(2048)
 
-bool convolve_length_supported(int N) {
-  switch (N) {
-#define JUBIK_CASE(l) case l:
-    JUBIK_CONV_LENGTHS(JUBIK_CASE)
+int convolve_fft_length(int64_t n) {
+  if (n < 1) return 0;
+  int best = 0;
+#define JUBIK_CASE(l) \
+  if (n == l) return l; \
+  if (best == 0 && l >= 2 * n - 1) best = l;
+  JUBIK_CONV_LENGTHS(JUBIK_CASE)
 #undef JUBIK_CASE
-    return true;
-    default:
-      return false;
-  }
+  return best;
 }
 
-template <typename T, int N>
-static int conv_run(const ConvGeom& g, const void* x, const void* y, const void* tw, void* work,
-                    void* out, hipStream_t s) {
-  constexpr int V = ConvTile<T, N>::V;
+template <typename T>
+static void conv_cols(const ConvGeom& g, int pass, const void* x, const void* y, const void* tw,
+                      void* work, void* out, hipStream_t s) {
   using C = cplx<T>;
-  dim3 gc(N / V, g.nby);
-  k_conv_cols<T, N, 0><<<gc, PNT, 0, s>>>(g, (const T*)x, (const T*)y, (const C*)tw, (C*)work,
-                                          nullptr);
-  k_conv_rows<T, N><<<dim3(N / 2 + 1, g.nby), PNT, 0, s>>>(g, (const C*)tw, (C*)work);
-  k_conv_cols<T, N, 1><<<gc, PNT, 0, s>>>(g, nullptr, nullptr, (const C*)tw, (C*)work, (T*)out);
-  return check_hip(hipGetLastError(), "convolve");
-}
-
-int convolve2d(int dtype, const void* x, const void* y, int64_t nbx, int64_t nby, int64_t N,
-               double dvol, const void* tw, void* work, void* out, hipStream_t s) {
-  if (nbx < 1 || nby < 1 || nby % nbx) return fail(kInvalidArgument, "batch sizes do not broadcast");
-  ConvGeom g{(int)N, (int)nbx, (int)nby, dvol / ((double)N * (double)N)};
-  switch (N) {
-#define JUBIK_CASE(l)                                                          \
-  case l:                                                                      \
-    return dtype == 0 ? conv_run<float, l>(g, x, y, tw, work, out, s)          \
-                      : conv_run<double, l>(g, x, y, tw, work, out, s);
+  switch (g.LH) {
+#define JUBIK_CASE(l)                                                                        \
+  case l: {                                                                                  \
+    constexpr int V = ConvTile<T, l>::V;                                                     \
+    const dim3 gc((g.W + V - 1) / V, g.nby);                                                 \
+    if (pass == 0)                                                                           \
+      k_conv_cols<T, l, 0><<<gc, PNT, 0, s>>>(g, (const T*)x, (const T*)y, (const C*)tw,     \
+                                               (C*)work, nullptr);                           \
+    else                                                                                     \
+      k_conv_cols<T, l, 1><<<gc, PNT, 0, s>>>(g, nullptr, nullptr, (const C*)tw, (C*)work,   \
+                                               (T*)out);                                     \
+    break;                                                                                   \
+  }
     JUBIK_CONV_LENGTHS(JUBIK_CASE)
 #undef JUBIK_CASE
     default:
-      return fail(kUnsupported, "convolve size " + std::to_string(N) + " is not compiled in");
+      break;
   }
+}
+
+template <typename T>
+static void conv_rows(const ConvGeom& g, const void* tw, void* work, hipStream_t s) {
+  using C = cplx<T>;
+  const dim3 gr(g.LH / 2 + 1, g.nby);
+  switch (g.LW) {
+#define JUBIK_CASE(l)                                                                   \
+  case l:                                                                               \
+    k_conv_rows<T, l><<<gr, PNT, 0, s>>>(g, (const C*)tw, (C*)work);                    \
+    break;
+    JUBIK_CONV_LENGTHS(JUBIK_CASE)
+#undef JUBIK_CASE
+    default:
+      break;
+  }
+}
+
+int convolve2d(int dtype, const void* x, const void* y, int64_t nbx, int64_t nby, int64_t H,
+               int64_t W, double dvol, const void* twh, const void* tww, void* work, void* out,
+               hipStream_t s) {
+  if (nbx < 1 || nby < 1 || nby % nbx) return fail(kInvalidArgument, "batch sizes do not broadcast");
+  const int LH = convolve_fft_length(H), LW = convolve_fft_length(W);
+  if (!LH || !LW)
+    return fail(kUnsupported, "convolve axis of length " + std::to_string(!LH ? H : W) +
+                                  " exceeds 2048");
+  ConvGeom g{(int)H, (int)W, LH, LW, (int)nbx, (int)nby, dvol / ((double)LH * (double)LW)};
+  if (dtype == 0) {
+    conv_cols<float>(g, 0, x, y, twh, work, out, s);
+    conv_rows<float>(g, tww, work, s);
+    conv_cols<float>(g, 1, x, y, twh, work, out, s);
+  } else {
+    conv_cols<double>(g, 0, x, y, twh, work, out, s);
+    conv_rows<double>(g, tww, work, s);
+    conv_cols<double>(g, 1, x, y, twh, work, out, s);
+  }
+  return check_hip(hipGetLastError(), "convolve");
 }
 
 }  // namespace jubik

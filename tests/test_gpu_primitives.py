@@ -53,3 +53,38 @@ def test_convolve_matches_oracle(ja, N, dtype):
     got = ja.convolve(k, z, ja.Domain((N, N), (0.5, 2.0)), axes=(-2, -1))
     err = np.linalg.norm(got - want) / np.linalg.norm(want)
     assert err <= (1e-13 if dtype == np.float64 else 2e-6)
+
+
+# Rectangular arrays and lengths without their own plan (zero-padded to a
+# compiled length >= 2 n - 1 and folded), against numpy's fftn in float64.
+@pytest.mark.parametrize("kshape,yshape,axes", [
+    ((50, 70), (50, 70), (-2, -1)),          # both axes folded (128 x 192)
+    ((2, 33, 128), (2, 33, 128), (-2, -1)),   # one folded, one own length
+    ((3, 100, 7), (3, 100, 7), (1, 2)),       # plot.py:513's axes [1, 2]
+    ((6, 5, 9), (6, 5, 9), (0, 2)),           # two non-adjacent axes
+    ((4, 37), (4, 37), (-1,)),                # one axis
+    ((1, 1), (1, 1), (-2, -1)),               # single element
+    ((96, 1024), (96, 1024), (-2, -1)),       # own lengths, rectangular
+    ((3, 700, 40), (3, 2, 700, 40), (-2, -1)),  # broadcast branch, folded to 2048 x 96
+])
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_convolve_general_shapes(ja, kshape, yshape, axes, dtype, capsys):
+    rng = np.random.default_rng(len(kshape) + kshape[-1])
+    k = rng.normal(size=kshape).astype(dtype)
+    z = rng.normal(size=yshape).astype(dtype)
+    dist = tuple(0.5 + 0.25 * i for i in range(len(yshape)))
+    want = cv.convolve(k.astype(np.float64), z.astype(np.float64), ODomain(yshape, dist), axes,
+                       verbose=False)
+    got = ja.convolve(k, z, ja.Domain(yshape, dist), axes=axes)
+    assert got.shape == want.shape and got.dtype == dtype
+    err = np.linalg.norm(got - want) / np.linalg.norm(want)
+    assert err <= (1e-12 if dtype == np.float64 else 3e-6), err
+
+
+def test_convolve_size_limit(ja):
+    with pytest.raises(NotImplementedError, match="2048"):
+        ja.convolve(np.ones((4, 3000)), np.ones((4, 3000)), ja.Domain((4, 3000), (1.0, 1.0)),
+                    axes=(-2, -1))
+    with pytest.raises(NotImplementedError):
+        ja.convolve(np.ones((2, 2, 2)), np.ones((2, 2, 2)), ja.Domain((2, 2, 2), (1.0,) * 3),
+                    axes=(0, 1, 2))
