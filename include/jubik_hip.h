@@ -305,6 +305,35 @@ int jubik_model_grad_batch(jubik_model* model, int64_t S, const void* amp_ref,
                            const void* amp_alpha, const void* xi_ref, const void* xi_alpha,
                            double zm, double mu, const void* dnu, double* nll, void* g_ref,
                            void* g_alpha, void* stream);
+/* Point sources added to the diffuse sky: jubik/sky_models.py:376-441 (the
+ * SkyModel point-source component, sky = diffuse + points at 202) and
+ * jubik/sky_model/multifrequency/spectral_product_mf_point_sources.py:99-287
+ * (MultiFrequencyInvGammaSky without spectral deviations).  Per pixel
+ *   p_e = I(xi_ps) exp(alpha dnu_e),  alpha = alpha_mean + alpha_std xi_ps_alpha,
+ * I = invgamma_prior(a, q)(xi_ps): the inverse-gamma variable of shape a and
+ * scale q with the quantile of the standard normal xi_ps.
+ * jubik_model_set_point_sources sets the priors (enable = 0 switches them
+ * off) and allocates the model's point-source buffers.
+ * jubik_model_grad_points evaluates S positions (xi_ps, xi_ps_alpha, g_ps,
+ * g_ps_alpha are (S, H, W) stacks on the sky grid, the rest as in
+ * jubik_model_grad_batch): S = 1 sets the metric base point like
+ * jubik_model_grad, S > 1 uses the batch slots like jubik_model_grad_batch.
+ * The metric entry points then apply to the diffuse latents at lam of the
+ * total sky.  Without a spectral index (single-energy sky) xi_ps_alpha and
+ * g_ps_alpha may be NULL and p = I. */
+int jubik_model_set_point_sources(jubik_model* model, int enable, double a, double q,
+                                  double alpha_mean, double alpha_std);
+int jubik_model_grad_points(jubik_model* model, int64_t S, const void* amp_ref,
+                            const void* amp_alpha, const void* xi_ref, const void* xi_alpha,
+                            double zm, double mu, const void* dnu, const void* xi_ps,
+                            const void* xi_ps_alpha, double* nll, void* g_ref, void* g_alpha,
+                            void* g_ps, void* g_ps_alpha, void* stream);
+/* The inverse-gamma prior transform of nifty.re's invgamma_prior(a, q) as
+ * used at jubik/sky_models.py:386-391: log_i[k] = log I(xi[k]) and (if not
+ * NULL) dlog_i[k] = d log I / d xi at n standard-normal values xi (device
+ * arrays of the dtype). */
+int jubik_invgamma_prior(int dtype, const void* xi, int64_t n, double a, double q, void* log_i,
+                         void* dlog_i, void* stream);
 /* S metric products at the base point in one pass: v and out are (S, N, N)
  * stacks (the S conjugate-gradient systems of an MGVI sample draw). */
 int jubik_model_metric_batch(jubik_model* model, int64_t S, const void* amp_ref,

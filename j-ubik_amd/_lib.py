@@ -51,6 +51,10 @@ SIGNATURES = {
     "jubik_model_destroy": [_vp],
     "jubik_model_set_data": [_vp, _vp, _vp],
     "jubik_model_grad": [_vp, _vp, _vp, _vp, _vp, _dbl, _dbl, _vp, _vp, _vp, _vp, _vp],
+    "jubik_model_set_point_sources": [_vp, _int, _dbl, _dbl, _dbl, _dbl],
+    "jubik_model_grad_points": [_vp, _i64, _vp, _vp, _vp, _vp, _dbl, _dbl, _vp, _vp, _vp, _vp,
+                                _vp, _vp, _vp, _vp, _vp],
+    "jubik_invgamma_prior": [_int, _vp, _i64, _dbl, _dbl, _vp, _vp, _vp],
     "jubik_model_grad_hyper": [_vp, _vp, _vp, _vp, _vp, _dbl, _dbl, _vp, _vp, _vp, _vp, _vp, _vp,
                                _vp, _vp],
     "jubik_model_metric": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],

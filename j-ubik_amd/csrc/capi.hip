@@ -11,6 +11,7 @@
 #include "../../include/jubik_hip.h"
 #include "cg.hpp"
 #include "gp.hpp"
+#include "points.hpp"
 #include "linpatch.hpp"
 #include "dft128.hpp"
 #include "primitives.hpp"
@@ -220,7 +221,16 @@ struct jubik_model {
   bool have_point = false;
   int col_div = 0;  // column- and row-pass tile divisors; 0 = the plan's
   int row_div = 0;
+  // point sources (jubik_model_set_point_sources)
+  bool ps_on = false;
+  PointSources ps{};
+  void* skyt = nullptr;     // max_samples x E x H x W: diffuse + point sources (the PSF input)
+  void* ct = nullptr;       // max_samples x E x H x W: cotangent of that total sky
+  void* psw = nullptr;      // max_samples x 3 x H x W: log I, d log I / d xi, alpha
   void release() {
+    dev_free(skyt);
+    dev_free(ct);
+    dev_free(psw);
     dev_free(sky);
     dev_free(dsky);
     dev_free(resid);
@@ -981,9 +991,20 @@ namespace {
 
 // S gradient evaluations at the positions xi[s]; sky and lam go to slots
 // slot0 .. slot0 + S - 1, nll[s] is the NLL of position s.
+// With point sources (xi_ps set; the model's point-source prior) the PSF
+// sees the diffuse sky plus p_e = I(xi_ps) exp(alpha dnu_e), and g_ps,
+// g_ps_alpha receive the gradients with respect to xi_ps and xi_ps_alpha.
+struct PsArgs {
+  const void* xi = nullptr;
+  const void* xa = nullptr;
+  void* g_xi = nullptr;
+  void* g_xa = nullptr;
+};
+
 int model_grad(jubik_model* model, int S, int slot0, const void* amp_ref, const void* amp_alpha,
                const void* xi_ref, const void* xi_alpha, double zm, double mu, const void* dnu,
-               double* nll, void* g_ref, void* g_alpha, hipStream_t s, GpHyper hy = GpHyper{}) {
+               double* nll, void* g_ref, void* g_alpha, hipStream_t s, GpHyper hy = GpHyper{},
+               PsArgs pa = PsArgs{}) {
   jubik_linpatch* lp = model->lp;
   jubik_gp* gp = model->gp;
   GpGeom gg = gp->g;
@@ -998,8 +1019,19 @@ int model_grad(jubik_model* model, int S, int slot0, const void* amp_ref, const 
   int st = gp_forward(gp->dtype, gg, model->gbufs(), amp_ref, amp_alpha, xi_ref, xi_alpha, dnu,
                       sky, s);
   if (st) return st;
-  st = linpatch_forward_poisson(lp->dtype, g, model->lbufs(), sky, model->pixel_area, model->expo,
-                                model->counts, model->resid, model->lam_slot(slot0),
+  const bool ps = pa.xi != nullptr;
+  const int64_t npix = (int64_t)g.H * g.W;
+  const void* psf_in = sky;
+  if (ps) {
+    st = ps_prepare(lp->dtype, model->ps, S, npix, pa.xi, pa.xa, model->psw, s);
+    if (!st)
+      st = ps_add(lp->dtype, S, g.E, npix, sky, model->psw, gp->g.has_alpha ? dnu : nullptr,
+                  model->skyt, s);
+    if (st) return st;
+    psf_in = model->skyt;
+  }
+  st = linpatch_forward_poisson(lp->dtype, g, model->lbufs(), psf_in, model->pixel_area,
+                                model->expo, model->counts, model->resid, model->lam_slot(slot0),
                                 (double*)model->partial, s);
   if (st) return st;
   NllFinal fin;
@@ -1011,11 +1043,15 @@ int model_grad(jubik_model* model, int S, int slot0, const void* amp_ref, const 
   model->last_S = S;
   if (slot0 == 1) model->batch_S = S;
   st = linpatch_adjoint_gp(lp->dtype, g, model->lbufs(), model->resid, model->pixel_area, sky, dnu,
-                           model->u, s, fin);
+                           model->u, s, fin, ps ? model->ct : nullptr);
   if (st) return st;
   gg.uplanes = g.E;
-  return gp_adjoint_u(gp->dtype, gg, model->gbufs(), amp_ref, amp_alpha, dnu, model->u, g_ref, g_alpha,
-                      s, hy);
+  st = gp_adjoint_u(gp->dtype, gg, model->gbufs(), amp_ref, amp_alpha, dnu, model->u, g_ref,
+                    g_alpha, s, hy);
+  if (st || !ps) return st;
+  return ps_grad(lp->dtype, model->ps, S, g.E, npix, model->ct, model->psw,
+                 gp->g.has_alpha ? dnu : nullptr, pa.g_xi, gp->g.has_alpha ? pa.g_xa : nullptr,
+                 s);
 }
 
 int check_model(jubik_model* model) {
@@ -1032,6 +1068,77 @@ int check_alpha(const jubik_model* model, const void* amp_alpha, const void* a, 
 }
 
 }  // namespace
+
+int jubik_model_set_point_sources(jubik_model* model, int enable, double a, double q,
+                                  double alpha_mean, double alpha_std) {
+  if (!model) return fail(kInvalidArgument, "NULL argument");
+  if (!enable) {
+    model->ps_on = false;
+    return kOk;
+  }
+  if (!(a > 0.0) || !(q > 0.0))
+    return fail(kInvalidArgument, "the inverse-gamma prior needs a > 0 and q > 0");
+  if (!(alpha_std > 0.0))
+    return fail(kInvalidArgument, "the spectral-index prior needs std > 0");
+  DeviceGuard dg(model->lp->device);
+  const size_t plane = model->img_bytes();
+  const size_t S = (size_t)model->max_samples;
+  int st = kOk;
+  if (!model->skyt) st = dev_alloc(&model->skyt, S * model->lp->g.E * plane, "point-source sky");
+  if (!st && !model->ct) st = dev_alloc(&model->ct, S * model->lp->g.E * plane, "sky cotangent");
+  if (!st && !model->psw) st = dev_alloc(&model->psw, S * 3 * plane, "point-source parameters");
+  if (st) {
+    dev_free(model->skyt);
+    dev_free(model->ct);
+    dev_free(model->psw);
+    return st;
+  }
+  model->ps = PointSources{a, q, alpha_mean, alpha_std};
+  model->ps_on = true;
+  model->have_point = false;
+  return kOk;
+}
+
+int jubik_model_grad_points(jubik_model* model, int64_t S, const void* amp_ref,
+                            const void* amp_alpha, const void* xi_ref, const void* xi_alpha,
+                            double zm, double mu, const void* dnu, const void* xi_ps,
+                            const void* xi_ps_alpha, double* nll, void* g_ref, void* g_alpha,
+                            void* g_ps, void* g_ps_alpha, void* stream) {
+  if (!model || !nll || !g_ref || !amp_ref || !xi_ref || !xi_ps || !g_ps)
+    return fail(kInvalidArgument, "NULL argument");
+  if (!model->ps_on)
+    return fail(kNotReady, "no point-source prior set (call jubik_model_set_point_sources)");
+  int st = check_model(model);
+  if (!st) st = check_alpha(model, amp_alpha, xi_alpha, g_alpha, dnu);
+  if (!st && model->gp->g.has_alpha && (!xi_ps_alpha || !g_ps_alpha))
+    st = fail(kInvalidArgument, "spectral-index inputs are NULL");
+  if (st) return st;
+  if (S < 1 || S > model->max_samples)
+    return fail(kInvalidArgument, "samples " + std::to_string(S) + " outside [1, " +
+                                      std::to_string(model->max_samples) + "]");
+  DeviceGuard dg(model->lp->device);
+  PsArgs pa;
+  pa.xi = xi_ps;
+  pa.xa = xi_ps_alpha;
+  pa.g_xi = g_ps;
+  pa.g_xa = g_ps_alpha;
+  // one position: the metric base point (slot 0); a batch: slots 1 .. S
+  const int slot0 = S == 1 ? 0 : 1;
+  if (S == 1) model->have_point = false;
+  st = model_grad(model, (int)S, slot0, amp_ref, amp_alpha, xi_ref, xi_alpha, zm, mu, dnu, nll,
+                  g_ref, g_alpha, as_stream(stream), GpHyper{}, pa);
+  if (st == kOk && S == 1) model->have_point = true;
+  return st;
+}
+
+int jubik_invgamma_prior(int dtype, const void* xi, int64_t n, double a, double q, void* log_i,
+                         void* dlog_i, void* stream) {
+  if (dtype != 0 && dtype != 1) return fail(kInvalidArgument, "dtype must be 0 or 1");
+  if (n < 0 || (n > 0 && (!xi || !log_i))) return fail(kInvalidArgument, "NULL argument");
+  if (!(a > 0.0) || !(q > 0.0))
+    return fail(kInvalidArgument, "the inverse-gamma prior needs a > 0 and q > 0");
+  return invgamma_prior(dtype, xi, n, a, q, log_i, dlog_i, as_stream(stream));
+}
 
 int jubik_model_grad_hyper(jubik_model* model, const void* amp_ref, const void* amp_alpha,
                            const void* xi_ref, const void* xi_alpha, double zm, double mu,

@@ -123,8 +123,9 @@ int linpatch_adjoint(int dtype, const LinGeom& g, const LinBuffers& b, const voi
 }
 int linpatch_adjoint_gp(int dtype, const LinGeom& g, const LinBuffers& b, const void* gy,
                         double scale, const void* sky, const void* dnu, void* u, hipStream_t s,
-                        NllFinal fin) {
+                        NllFinal fin, void* ct) {
   LinCall c;
+  c.ct = ct;
   c.op = LinCall::kAdjointGp;
   c.in = gy;
   c.scale = scale;

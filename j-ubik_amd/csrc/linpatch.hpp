@@ -84,10 +84,12 @@ struct NllFinal {
 
 // Adjoint chained into the GP: u[s, e] = sum_mod scale * sky[e] *
 // PSF^T(gy)_(s, mod, e), real (S, E, H, W) planes (dnu is unused: the GP
-// adjoint applies it); optionally finalises the NLL.
+// adjoint applies it); optionally finalises the NLL.  With ct set it also
+// receives sum_mod scale * PSF^T(gy)_(s, mod, e), the cotangent of the sky
+// the PSF saw (point sources added to the diffuse sky).
 int linpatch_adjoint_gp(int dtype, const LinGeom& g, const LinBuffers& b, const void* gy,
                         double scale, const void* sky, const void* dnu, void* u, hipStream_t s,
-                        NllFinal fin = NllFinal{});
+                        NllFinal fin = NllFinal{}, void* ct = nullptr);
 
 // Matrix-core column pass (lp_mfma.hip): the forward (adjoint = 0) or
 // adjoint column pass of an FP32 plan at L = 384 with each column DFT as
@@ -114,6 +116,7 @@ struct LinCall {
   const void* dnu = nullptr;
   int layout = 0;
   NllFinal fin{};
+  void* ct = nullptr;        // adjoint into the GP: cotangent of the total sky
 };
 
 }  // namespace jubik

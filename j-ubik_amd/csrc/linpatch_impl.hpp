@@ -896,9 +896,9 @@ __global__ __launch_bounds__(NTO) void lp_adj_out(LinGeom g, const cplx<T>* __re
     const int e = tid + uu * NTO;
     if (e < g.w) tnr[uu] = tent[e];
   }
-  T acc[JPT], tmp[JPT];
+  T acc[JPT], tmp[JPT], tot[JPT];
 #pragma unroll
-  for (int t = 0; t < JPT; ++t) acc[t] = T(0);
+  for (int t = 0; t < JPT; ++t) acc[t] = tot[t] = T(0);
   const int npair = (g.n + 1) / 2;
 
   // EPI 0 with a shared input and EPI 1: the jobs (s, mod, e) of image
@@ -977,7 +977,10 @@ __global__ __launch_bounds__(NTO) void lp_adj_out(LinGeom g, const cplx<T>* __re
 #pragma unroll
       for (int t = 0; t < JPT; ++t) {
         const int j = tid + t * NTO;
-        if (j < g.W) acc[t] += scale * sky_v[t] * tmp[t];
+        if (j < g.W) {
+          acc[t] += scale * sky_v[t] * tmp[t];
+          tot[t] += scale * tmp[t];
+        }
       }
     }
   }
@@ -994,6 +997,15 @@ __global__ __launch_bounds__(NTO) void lp_adj_out(LinGeom g, const cplx<T>* __re
     for (int t = 0; t < JPT; ++t) {
       const int j = tid + t * NTO;
       if (j < g.W) u[row + j] = acc[t];
+    }
+    // with point sources: the cotangent of the total sky (u without the
+    // diffuse sky factor) into gx
+    if (gx) {
+#pragma unroll
+      for (int t = 0; t < JPT; ++t) {
+        const int j = tid + t * NTO;
+        if (j < g.W) gx[row + j] = tot[t];
+      }
     }
   }
 }
@@ -1199,7 +1211,7 @@ struct LinOps {
         return launch_status("linpatch_adjoint");
       case LinCall::kAdjointGp:
         if (int st = adj_front(g, b, c.in, s, c.fin)) return st;
-        adj_out<1>(g, b, g.S * g.E, nullptr, c.scale, (const T*)c.sky, (T*)c.out, s,
+        adj_out<1>(g, b, g.S * g.E, (T*)c.ct, c.scale, (const T*)c.sky, (T*)c.out, s,
                    "lp_adj_out_gp");
         return launch_status("linpatch_adjoint_gp");
       default:

@@ -126,6 +126,75 @@ class XrayModel:
         self._point_version = s.version
         return self._nll, g_ref, g_alpha
 
+    # ------------------------------------------------------------ point sources
+    def set_point_sources(self, a, q, alpha_mean=0.0, alpha_std=1.0):
+        """Add point sources to the diffuse sky (jubik/sky_models.py:376-441,
+        spectral_product_mf_point_sources.py:99-287): per pixel p_e =
+        I(xi_ps) exp(alpha dnu_e) with I = invgamma_prior(a, q)(xi_ps) and
+        alpha = alpha_mean + alpha_std xi_ps_alpha.  ``a=None`` removes them."""
+        if a is None:
+            _lib.call("jubik_model_set_point_sources", self._model, 0, 0.0, 0.0, 0.0, 1.0)
+            self.point_sources = None
+            return self
+        _lib.call("jubik_model_set_point_sources", self._model, 1, float(a), float(q),
+                  float(alpha_mean), float(alpha_std))
+        self.point_sources = dict(a=float(a), q=float(q), alpha_mean=float(alpha_mean),
+                                  alpha_std=float(alpha_std))
+        self._point_version = None
+        return self
+
+    def nll_and_grad_points(self, xi_ref, xi_alpha, xi_ps, xi_ps_alpha=None):
+        """NLL and gradients with point sources: (nll, g_ref, g_alpha, g_ps,
+        g_ps_alpha).  xi_ps (H, W) evaluates one position and sets the metric
+        base point; a stack (S, H, W) with (S, N, N) latents evaluates S
+        positions in one pass (nll then an array of S)."""
+        if getattr(self, "point_sources", None) is None:
+            raise RuntimeError("no point-source prior set (call set_point_sources)")
+        s = self.sky
+        H, W = self.psf.H, self.psf.W
+        xp = dev.to_device(xi_ps, self.dtype)
+        batch = xp.dim() == 3
+        S = int(xp.shape[0]) if batch else 1
+        if batch:
+            S = self._batch_size(xp)
+            xr = self._stack(xi_ref, S, "xi_ref")
+            xa = self._stack(xi_alpha, S, "xi_alpha") if s.has_alpha else None
+        else:
+            xr = dev.to_device(xi_ref, self.dtype)
+            xa = dev.to_device(xi_alpha, self.dtype) if s.has_alpha else None
+        want = (S, H, W) if batch else (H, W)
+        if tuple(xp.shape) != want:
+            raise ValueError(f"xi_ps must have shape {want}, got {tuple(xp.shape)}")
+        xpa = None
+        if s.has_alpha:
+            if xi_ps_alpha is None:
+                raise ValueError("a spectral sky needs xi_ps_alpha")
+            xpa = dev.to_device(xi_ps_alpha, self.dtype)
+            if tuple(xpa.shape) != want:
+                raise ValueError(f"xi_ps_alpha must have shape {want}")
+            xpa = xpa.contiguous()
+        xp = xp.contiguous()
+        gshape = (S, s.N, s.N) if batch else (s.N, s.N)
+        g_ref = torch.empty(gshape, dtype=self.dtype, device="cuda")
+        g_alpha = torch.empty_like(g_ref) if s.has_alpha else None
+        g_ps = torch.empty(want, dtype=self.dtype, device="cuda")
+        g_psa = torch.empty_like(g_ps) if s.has_alpha else None
+        nll = torch.empty(S, dtype=torch.float64, device="cuda")
+        _lib.call("jubik_model_grad_points", self._model, S, dev.ptr(s.amp_ref),
+                  dev.ptr(s.amp_alpha), dev.ptr(xr), dev.ptr(xa), s.zm, s.mu,
+                  dev.ptr(s.dnu_dev), dev.ptr(xp), dev.ptr(xpa), dev.ptr(nll), dev.ptr(g_ref),
+                  dev.ptr(g_alpha), dev.ptr(g_ps), dev.ptr(g_psa), dev.stream())
+        self.check_domain()
+        if batch:
+            self._batch_S = S
+            self._batch_version = s.version
+            out_nll = nll.cpu().numpy()
+        else:
+            self._point_version = s.version
+            out_nll = float(nll.item())
+        like = lambda t: None if t is None else dev.like_input(t, xi_ps)
+        return out_nll, like(g_ref), like(g_alpha), like(g_ps), like(g_psa)
+
     def check_domain(self, raise_error=True):
         """Counts of read-out pixels with lam <= 0 per sample of the last
         gradient evaluation (synchronises).  Raises FloatingPointError when

@@ -29,9 +29,11 @@ The prior's own gradient of 1/2 z^2 is z; ``latent_energy_and_grad`` adds it.
 """
 import math
 
+import numpy as np
 import torch
 
 from . import _device as dev
+from . import _lib
 
 KEYS = ("spatial_fluctuations", "spectral_index_fluctuations", "spectral_index_mean",
         "zero_mode")
@@ -65,6 +67,32 @@ class normal_prior:
 
     def __call__(self, z):
         return self.mean + self.std * float(z)
+
+
+class invgamma_prior:
+    """xi -> the inverse-gamma variable of shape a and scale q with the same
+    quantile as the standard normal xi (nifty.re's invgamma_prior(a, q), the
+    point-source prior of jubik/sky_models.py:386-391): I = q / y with
+    Q(a, y) = Phi(xi), Q the regularised upper incomplete gamma function.
+    Evaluated on the device (jubik_invgamma_prior)."""
+
+    def __init__(self, a, q):
+        if not (a > 0 and q > 0):
+            raise ValueError("invgamma_prior needs a > 0 and q > 0")
+        self.a, self.q = float(a), float(q)
+
+    def log_and_dlog(self, xi):
+        """(log I(xi), d log I / d xi), same shape and kind as xi."""
+        dev.require_device()
+        t = dev.to_device(xi, dev.float_dtype(xi)).contiguous()
+        logi, dlog = torch.empty_like(t), torch.empty_like(t)
+        _lib.call("jubik_invgamma_prior", dev.code(t.dtype), dev.ptr(t), int(t.numel()), self.a,
+                  self.q, dev.ptr(logi), dev.ptr(dlog), dev.stream())
+        return dev.like_input(logi, xi), dev.like_input(dlog, xi)
+
+    def __call__(self, xi):
+        logi, _ = self.log_and_dlog(xi)
+        return torch.exp(logi) if isinstance(logi, torch.Tensor) else np.exp(logi)
 
 
 class SpectralSkyLatents:
