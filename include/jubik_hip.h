@@ -334,6 +334,35 @@ int jubik_model_grad_points(jubik_model* model, int64_t S, const void* amp_ref,
  * arrays of the dtype). */
 int jubik_invgamma_prior(int dtype, const void* xi, int64_t n, double a, double q, void* log_i,
                          void* dlog_i, void* stream);
+/* geoVI nonlinear sample updates (nifty.re's nonlinearly_update, the
+ * "nonlinear_resample" / "nonlinear_update" sample modes of
+ * demos/erosita_demo.py:304-316 and demos/configs/eROSITA_demo.yaml:148-166):
+ * a sample x near the base point p (the last jubik_model_grad) is moved to
+ * minimise 1/2 |r(x)|^2 with
+ *   r(x) = (x - p) + J(p)^T D_p 2 (sqrt lam(x) - sqrt lam(p)) - m,
+ * D = diag(lam^-1/2) on read-out pixels and m the metric sample of the
+ * linear draw (2 sqrt lam is the Poisson likelihood's transformation, in
+ * whose coordinates its Fisher metric is the identity).  Its Gauss-Newton
+ * metric is (I + K)^T (I + K), K = J(p)^T D_p D_x J(x).
+ * jubik_model_geovi_point evaluates the model at x (into the batch slot 1,
+ * the base point stays) and prepares the scalings; nll receives the NLL at
+ * x (one device double).  jubik_model_geovi_apply: mode 0 out = K v, mode
+ * 1 out = K^T v, mode 2 out = J(p)^T D_p 2 (sqrt lam(x) - sqrt lam(p)) (v
+ * unused), mode 3 out = (K + K^T + K^T K) v.  jubik_model_geovi_solve: CG on
+ * (I + K + K^T + K^T K) x = b, arguments as jubik_model_metric_solve.  A
+ * gradient evaluation moves the points: call jubik_model_geovi_point again
+ * (JUBIK_ENOTREADY otherwise).  No counterpart in the reference beyond the
+ * nifty.re call it configures. */
+int jubik_model_geovi_point(jubik_model* model, const void* amp_ref, const void* amp_alpha,
+                            const void* x_ref, const void* x_alpha, double zm, double mu,
+                            const void* dnu, double* nll, void* stream);
+int jubik_model_geovi_apply(jubik_model* model, int mode, const void* amp_ref,
+                            const void* amp_alpha, const void* dnu, const void* v_ref,
+                            const void* v_alpha, void* out_ref, void* out_alpha, void* stream);
+int jubik_model_geovi_solve(jubik_model* model, const void* amp_ref, const void* amp_alpha,
+                            const void* dnu, const void* b_ref, const void* b_alpha, void* x_ref,
+                            void* x_alpha, int maxiter, double rtol, double absdelta,
+                            int check_every, double* info, void* stream);
 /* S metric products at the base point in one pass: v and out are (S, N, N)
  * stacks (the S conjugate-gradient systems of an MGVI sample draw). */
 int jubik_model_metric_batch(jubik_model* model, int64_t S, const void* amp_ref,

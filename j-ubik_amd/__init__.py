@@ -15,10 +15,11 @@ from .sky import CorrelatedFieldSky, powerlaw_amplitude, powerlaw_amplitude_grad
 from .model import XrayModel, capture_grad_batch, capture_metric_batch, dense_counts
 from .sampling import conjugate_gradient, draw_linear_samples, metric_solve
 from .kl import kl_value_and_grad, newton_cg, optimize_kl
+from .geovi import draw_nonlinear_samples, nonlinear_update
 from .mock import create_mock_counts, poisson_sample, save_mock_data
 from .psf import psf_interpolator
 from .dft import DFT128
-from . import priors, synthetic
+from . import geovi, priors, synthetic
 from .autograd import linpatch_apply, xray_nll
 
 __all__ = [
@@ -27,6 +28,6 @@ __all__ = [
     "linpatch_convolve_modules", "build_exposure_function", "build_readout_function",
     "Poissonian", "CorrelatedFieldSky", "powerlaw_amplitude", "XrayModel", "dense_counts",
     "conjugate_gradient", "draw_linear_samples", "metric_solve", "kl_value_and_grad",
-    "newton_cg", "optimize_kl", "create_mock_counts", "save_mock_data",
+    "newton_cg", "optimize_kl", "draw_nonlinear_samples", "nonlinear_update", "create_mock_counts", "save_mock_data",
     "poisson_sample", "psf_interpolator", "DFT128",
 ]

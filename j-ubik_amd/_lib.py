@@ -55,6 +55,10 @@ SIGNATURES = {
     "jubik_model_grad_points": [_vp, _i64, _vp, _vp, _vp, _vp, _dbl, _dbl, _vp, _vp, _vp, _vp,
                                 _vp, _vp, _vp, _vp, _vp],
     "jubik_invgamma_prior": [_int, _vp, _i64, _dbl, _dbl, _vp, _vp, _vp],
+    "jubik_model_geovi_point": [_vp, _vp, _vp, _vp, _vp, _dbl, _dbl, _vp, _vp, _vp],
+    "jubik_model_geovi_apply": [_vp, _int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
+    "jubik_model_geovi_solve": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _int, _dbl, _dbl, _int,
+                                _vp, _vp],
     "jubik_model_grad_hyper": [_vp, _vp, _vp, _vp, _vp, _dbl, _dbl, _vp, _vp, _vp, _vp, _vp, _vp,
                                _vp, _vp],
     "jubik_model_metric": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],

@@ -227,7 +227,15 @@ struct jubik_model {
   void* skyt = nullptr;     // max_samples x E x H x W: diffuse + point sources (the PSF input)
   void* ct = nullptr;       // max_samples x E x H x W: cotangent of that total sky
   void* psw = nullptr;      // max_samples x 3 x H x W: log I, d log I / d xi, alpha
+  // geoVI nonlinear sample updates (jubik_model_geovi_*): the sample point x
+  // lives in slot 1; geo holds sqrt(lam_x lam_p) and eta = 2 (sqrt lam_x -
+  // sqrt lam_p) (2 x J x H x W); geo_vec three latent vectors (2 x N^2 each)
+  void* geo = nullptr;
+  void* geo_vec = nullptr;
+  bool geo_ready = false;
   void release() {
+    dev_free(geo);
+    dev_free(geo_vec);
     dev_free(skyt);
     dev_free(ct);
     dev_free(psw);
@@ -1007,6 +1015,7 @@ int model_grad(jubik_model* model, int S, int slot0, const void* amp_ref, const 
                PsArgs pa = PsArgs{}) {
   jubik_linpatch* lp = model->lp;
   jubik_gp* gp = model->gp;
+  model->geo_ready = false;  // slots 0 / 1 change
   GpGeom gg = gp->g;
   gg.zm = zm;
   gg.mu = mu;
@@ -1419,6 +1428,61 @@ int model_metric(jubik_model* model, int S, bool per_sample, const void* amp_ref
                       out_alpha, s);
 }
 
+// The cross operators of geoVI at the base point p (slot 0) and the sample
+// point x (slot 1): to_p = true: K v = J(p)^T D_p D_x J(x) v, to_p = false:
+// K^T v = J(x)^T D_x D_p J(p) v, D = diag(lam^-1/2) on read-out pixels: the
+// metric pass with the tangent taken at one sky, the lam scaling
+// sqrt(lam_x lam_p) and the adjoint at the other sky.
+int model_cross(jubik_model* model, bool to_p, const void* amp_ref, const void* amp_alpha,
+                const void* dnu, const void* v_ref, const void* v_alpha, void* out_ref,
+                void* out_alpha, hipStream_t s) {
+  jubik_linpatch* lp = model->lp;
+  jubik_gp* gp = model->gp;
+  GpGeom gg = gp->g;
+  gg.S = 1;
+  gg.sky_shared = 1;
+  LinGeom g = lp->g;
+  g.S = 1;
+  g.sky_shared = 1;
+  const void* sky_t = model->sky_slot(to_p ? 1 : 0);
+  const void* sky_a = model->sky_slot(to_p ? 0 : 1);
+  int st = gp_tangent(gp->dtype, gg, model->gbufs(), amp_ref, amp_alpha, v_ref, v_alpha, dnu,
+                      sky_t, model->dsky, s);
+  if (st) return st;
+  st = linpatch_forward_metric(lp->dtype, g, model->lbufs(), model->dsky, model->pixel_area,
+                               model->expo, model->counts, model->geo, model->resid, s);
+  if (st) return st;
+  st = linpatch_adjoint_gp(lp->dtype, g, model->lbufs(), model->resid, model->pixel_area, sky_a,
+                           dnu, model->u, s);
+  if (st) return st;
+  gg.uplanes = g.E;
+  return gp_adjoint_u(gp->dtype, gg, model->gbufs(), amp_ref, amp_alpha, dnu, model->u, out_ref,
+                      out_alpha, s);
+}
+
+// out = K v + K^T v + K^T K v: the Gauss-Newton metric of the geoVI residual
+// r(x) = (x - p) + J(p)^T D_p 2 (sqrt lam(x) - sqrt lam(p)) - m is
+// (I + K)^T (I + K) = I + this.
+int geo_metric(jubik_model* model, const void* amp_ref, const void* amp_alpha, const void* dnu,
+               const void* v_ref, const void* v_alpha, void* out_ref, void* out_alpha,
+               hipStream_t s) {
+  const int dtype = model->lp->dtype;
+  const int64_t n = (int64_t)model->gp->g.N * model->gp->g.N;
+  const size_t es = elem_size(dtype);
+  char* gv = static_cast<char*>(model->geo_vec);
+  void* kv[2] = {gv, v_alpha ? gv + n * es : nullptr};              // K v
+  void* kkv[2] = {gv + 2 * n * es, v_alpha ? gv + 3 * n * es : nullptr};  // K^T K v
+  int st = model_cross(model, true, amp_ref, amp_alpha, dnu, v_ref, v_alpha, kv[0], kv[1], s);
+  if (!st) st = model_cross(model, false, amp_ref, amp_alpha, dnu, v_ref, v_alpha, out_ref,
+                            out_alpha, s);
+  if (!st) st = model_cross(model, false, amp_ref, amp_alpha, dnu, kv[0], kv[1], kkv[0], kkv[1],
+                            s);
+  for (int f = 0; f < 2 && !st; ++f)
+    if (f == 0 || v_alpha)
+      st = vec_sum3(dtype, n, f ? out_alpha : out_ref, kv[f], kkv[f], f ? out_alpha : out_ref, s);
+  return st;
+}
+
 int check_batch(const jubik_model* model, int64_t S) {
   if (S < 1 || S > model->max_samples)
     return fail(kInvalidArgument, "batch of " + std::to_string(S) +
@@ -1440,12 +1504,13 @@ int check_samples_point(const jubik_model* model, int64_t S) {
 // Conjugate gradients on (I + M) x = b, M the metric of model_metric: S
 // systems at the base point (per_sample = false) or one system with M the
 // mean over the S sample positions (per_sample = true).
+// geo = true: one system (I + K + K^T + K^T K) x = b of geoVI (geo_metric).
 int model_solve(jubik_model* model, int S, bool per_sample, const void* amp_ref,
                 const void* amp_alpha, const void* dnu, const void* b_ref, const void* b_alpha,
                 void* x_ref, void* x_alpha, int maxiter, double rtol, double absdelta,
-                int check_every, double* info, hipStream_t s) {
+                int check_every, double* info, hipStream_t s, bool geo = false) {
   const int dtype = model->lp->dtype;
-  const int nsys = per_sample ? 1 : S;
+  const int nsys = per_sample || geo ? 1 : S;
   const bool alpha = model->gp->g.has_alpha != 0;
   const int64_t n = (int64_t)model->gp->g.N * model->gp->g.N;
   const size_t es = elem_size(dtype);
@@ -1467,9 +1532,10 @@ int model_solve(jubik_model* model, int S, bool per_sample, const void* amp_ref,
   double* work = static_cast<double*>(model->cg_work);
   int st = cg_init(dtype, d, b, x, r, p, work, s);
   for (int it = 0; it < maxiter && !st; ++it) {
-    st = model_metric(model, S, per_sample, amp_ref, amp_alpha, dnu, p.f[0], p.f[1], Mv.f[0],
-                      Mv.f[1], s);
-    if (!st) st = cg_matvec(dtype, d, p, Mv, per_sample ? 1.0 / S : 1.0, Ap, work, s);
+    st = geo ? geo_metric(model, amp_ref, amp_alpha, dnu, p.f[0], p.f[1], Mv.f[0], Mv.f[1], s)
+             : model_metric(model, S, per_sample, amp_ref, amp_alpha, dnu, p.f[0], p.f[1],
+                            Mv.f[0], Mv.f[1], s);
+    if (!st) st = cg_matvec(dtype, d, p, Mv, per_sample && !geo ? 1.0 / S : 1.0, Ap, work, s);
     if (!st) st = cg_update(dtype, d, b, x, r, p, Ap, work, s);
     if (!st && check_every > 0 && (it + 1) % check_every == 0 && it + 1 < maxiter) {
       int active = 0;
@@ -1503,6 +1569,91 @@ int jubik_model_metric_batch(jubik_model* model, int64_t S, const void* amp_ref,
   DeviceGuard dg(model->lp->device);
   return model_metric(model, (int)S, false, amp_ref, amp_alpha, dnu, v_ref, v_alpha, out_ref,
                       out_alpha, as_stream(stream));
+}
+
+int jubik_model_geovi_point(jubik_model* model, const void* amp_ref, const void* amp_alpha,
+                            const void* x_ref, const void* x_alpha, double zm, double mu,
+                            const void* dnu, double* nll, void* stream) {
+  if (!model || !amp_ref || !x_ref || !nll) return fail(kInvalidArgument, "NULL argument");
+  int st = check_model(model);
+  if (!st) st = check_alpha(model, amp_alpha, x_alpha, x_alpha, dnu);
+  if (st) return st;
+  if (!model->have_point) return fail(kNotReady, "no base point (call jubik_model_grad first)");
+  if (model->max_samples < 1) return fail(kInvalidArgument, "model has no sample slot");
+  DeviceGuard dg(model->lp->device);
+  const size_t es = elem_size(model->lp->dtype);
+  const int64_t npix = (int64_t)model->lp->g.J * model->lp->g.H * model->lp->g.W;
+  const int64_t n = (int64_t)model->gp->g.N * model->gp->g.N;
+  if (!model->geo) st = dev_alloc(&model->geo, 2 * npix * es, "geoVI scalings");
+  if (!st && !model->geo_vec) st = dev_alloc(&model->geo_vec, 4 * n * es, "geoVI vectors");
+  if (st) return st;
+  model->geo_ready = false;
+  // the evaluation at x into slot 1 (its gradient lands in geo_vec and is
+  // not used); the base point stays in slot 0
+  char* gv = static_cast<char*>(model->geo_vec);
+  st = model_grad(model, 1, 1, amp_ref, amp_alpha, x_ref, x_alpha, zm, mu, dnu, nll, gv,
+                  model->gp->g.has_alpha ? gv + n * es : nullptr, as_stream(stream));
+  if (!st)
+    st = geo_prep(model->lp->dtype, model->counts, model->lam_slot(0), model->lam_slot(1),
+                  model->geo, static_cast<char*>(model->geo) + npix * es, npix,
+                  as_stream(stream));
+  if (!st) model->geo_ready = true;
+  return st;
+}
+
+int jubik_model_geovi_apply(jubik_model* model, int mode, const void* amp_ref,
+                            const void* amp_alpha, const void* dnu, const void* v_ref,
+                            const void* v_alpha, void* out_ref, void* out_alpha, void* stream) {
+  if (!model || !amp_ref || !out_ref || (mode != 2 && !v_ref))
+    return fail(kInvalidArgument, "NULL argument");
+  if (mode < 0 || mode > 3) return fail(kInvalidArgument, "geoVI mode must be 0 .. 3");
+  int st = check_alpha(model, amp_alpha, mode == 2 ? out_alpha : v_alpha, out_alpha, dnu);
+  if (st) return st;
+  if (!model->geo_ready || !model->have_point)
+    return fail(kNotReady, "no geoVI sample point (call jubik_model_geovi_point)");
+  DeviceGuard dg(model->lp->device);
+  hipStream_t s = as_stream(stream);
+  if (mode == 0 || mode == 1)
+    return model_cross(model, mode == 0, amp_ref, amp_alpha, dnu, v_ref, v_alpha, out_ref,
+                       out_alpha, s);
+  if (mode == 3)
+    return geo_metric(model, amp_ref, amp_alpha, dnu, v_ref, v_alpha, out_ref, out_alpha, s);
+  // mode 2: J(p)^T D_p eta
+  jubik_linpatch* lp = model->lp;
+  jubik_gp* gp = model->gp;
+  LinGeom g = lp->g;
+  g.S = 1;
+  g.sky_shared = 1;
+  GpGeom gg = gp->g;
+  gg.S = 1;
+  gg.sky_shared = 1;
+  gg.uplanes = g.E;
+  const int64_t npix = (int64_t)g.J * g.H * g.W;
+  const void* eta = static_cast<char*>(model->geo) + npix * elem_size(lp->dtype);
+  st = sqrt_metric_cot(lp->dtype, model->expo, model->counts, model->lam_slot(0), eta,
+                       model->resid, npix, 1, s);
+  if (!st)
+    st = linpatch_adjoint_gp(lp->dtype, g, model->lbufs(), model->resid, model->pixel_area,
+                             model->sky_slot(0), dnu, model->u, s);
+  if (!st)
+    st = gp_adjoint_u(gp->dtype, gg, model->gbufs(), amp_ref, amp_alpha, dnu, model->u, out_ref,
+                      out_alpha, s);
+  return st;
+}
+
+int jubik_model_geovi_solve(jubik_model* model, const void* amp_ref, const void* amp_alpha,
+                            const void* dnu, const void* b_ref, const void* b_alpha, void* x_ref,
+                            void* x_alpha, int maxiter, double rtol, double absdelta,
+                            int check_every, double* info, void* stream) {
+  if (!model || !amp_ref || !b_ref || !x_ref) return fail(kInvalidArgument, "NULL argument");
+  int st = check_alpha(model, amp_alpha, b_alpha, x_alpha, dnu);
+  if (!st) st = check_solve_args(model, maxiter, rtol, check_every);
+  if (st) return st;
+  if (!model->geo_ready || !model->have_point)
+    return fail(kNotReady, "no geoVI sample point (call jubik_model_geovi_point)");
+  DeviceGuard dg(model->lp->device);
+  return model_solve(model, 1, false, amp_ref, amp_alpha, dnu, b_ref, b_alpha, x_ref, x_alpha,
+                     maxiter, rtol, absdelta, check_every, info, as_stream(stream), true);
 }
 
 int jubik_model_metric_samples(jubik_model* model, int64_t S, const void* amp_ref,

@@ -227,4 +227,26 @@ int cg_active_count(const CgDims& d, const double* work, hipStream_t s, int* cou
   return kOk;
 }
 
+template <typename T>
+__global__ __launch_bounds__(CNT) void k_vec_sum3(int64_t n, const T* __restrict__ a,
+                                                 const T* __restrict__ b,
+                                                 const T* __restrict__ c, T* __restrict__ out) {
+  for (int64_t i = blockIdx.x * (int64_t)CNT + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * CNT)
+    out[i] = a[i] + b[i] + c[i];
+}
+
+int vec_sum3(int dtype, int64_t n, const void* a, const void* b, const void* c, void* out,
+             hipStream_t s) {
+  if (n <= 0) return kOk;
+  const unsigned grid = (unsigned)((n + CNT - 1) / CNT < 4096 ? (n + CNT - 1) / CNT : 4096);
+  if (dtype == 0)
+    k_vec_sum3<float><<<grid, CNT, 0, s>>>(n, (const float*)a, (const float*)b,
+                                           (const float*)c, (float*)out);
+  else
+    k_vec_sum3<double><<<grid, CNT, 0, s>>>(n, (const double*)a, (const double*)b,
+                                            (const double*)c, (double*)out);
+  return check_hip(hipGetLastError(), "vec_sum3");
+}
+
 }  // namespace jubik

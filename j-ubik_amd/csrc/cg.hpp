@@ -45,6 +45,9 @@ int cg_update(int dtype, const CgDims& d, CgVec b, CgVec x, CgVec r, CgVec p, Cg
               double* work, hipStream_t s);
 // info[2 s] = iterations, info[2 s + 1] = |r| / |b| (device doubles).
 int cg_finish(const CgDims& d, const double* work, double* info, hipStream_t s);
+// out = a + b + c elementwise (n elements; out may alias a).
+int vec_sum3(int dtype, int64_t n, const void* a, const void* b, const void* c, void* out,
+             hipStream_t s);
 // Number of systems still iterating (synchronises the stream).
 int cg_active_count(const CgDims& d, const double* work, hipStream_t s, int* count);
 

@@ -223,6 +223,39 @@ int sqrt_metric_cot(int dtype, const void* expo, const int* counts, const void* 
   return check_hip(hipGetLastError(), "sqrt_metric_cot");
 }
 
+// geoVI: per read-out pixel geo = sqrt(lam_x lam_p) (the metric scaling of
+// the cross operator J(p)^T D_p D_x J(x)) and eta = 2 (sqrt lam_x - sqrt
+// lam_p) (the change of the Poisson transformation 2 sqrt lam); masked
+// pixels get geo = 1, eta = 0.
+template <typename T>
+__global__ __launch_bounds__(RNT) void k_geo_prep(const int* __restrict__ counts,
+                                                  const T* __restrict__ lam_p,
+                                                  const T* __restrict__ lam_x,
+                                                  T* __restrict__ geo, T* __restrict__ eta,
+                                                  int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)RNT + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * RNT) {
+    const bool ok = counts[i] >= 0;
+    const T lp = lam_p[i], lx = lam_x[i];
+    geo[i] = ok ? sqrt(lx * lp) : T(1);
+    eta[i] = ok ? T(2) * (sqrt(lx) - sqrt(lp)) : T(0);
+  }
+}
+
+int geo_prep(int dtype, const int* counts, const void* lam_p, const void* lam_x, void* geo,
+             void* eta, int64_t n, hipStream_t s) {
+  if (n <= 0) return kOk;
+  if (dtype == 0)
+    k_geo_prep<float><<<grid_for(n), RNT, 0, s>>>(counts, (const float*)lam_p,
+                                                  (const float*)lam_x, (float*)geo, (float*)eta,
+                                                  n);
+  else
+    k_geo_prep<double><<<grid_for(n), RNT, 0, s>>>(counts, (const double*)lam_p,
+                                                   (const double*)lam_x, (double*)geo,
+                                                   (double*)eta, n);
+  return check_hip(hipGetLastError(), "geo_prep");
+}
+
 int poisson_sample(int dtype, const void* lam, int64_t n, uint64_t seed, int* counts,
                    hipStream_t s) {
   if (n <= 0) return kOk;

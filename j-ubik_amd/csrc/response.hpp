@@ -19,6 +19,10 @@ int poisson_sample(int dtype, const void* lam, int64_t n, uint64_t seed, int* co
 // (n pixels per sample, S samples).
 int sqrt_metric_cot(int dtype, const void* expo, const int* counts, const void* lam,
                     const void* noise, void* cot, int64_t n, int S, hipStream_t s);
+// geoVI: geo = sqrt(lam_x lam_p), eta = 2 (sqrt lam_x - sqrt lam_p) where
+// counts >= 0 (else geo = 1, eta = 0); n pixels.
+int geo_prep(int dtype, const int* counts, const void* lam_p, const void* lam_x, void* geo,
+             void* eta, int64_t n, hipStream_t s);
 // out = t / lam (sqrt_form 0) or t / sqrt(lam) (sqrt_form 1).
 int poisson_metric(int dtype, const void* lam, const void* t, int64_t n, int sqrt_form,
                    void* out, hipStream_t s);

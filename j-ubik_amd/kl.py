@@ -96,19 +96,32 @@ def newton_cg(model: XrayModel, xbar_ref, xbar_alpha, deltas_ref, deltas_alpha=N
 
 
 def optimize_kl(model: XrayModel, xi_ref, xi_alpha=None, n_samples=4, n_iterations=5, seed=0,
-                xtol=1e-6, maxiter=20, draw_rtol=1e-6, draw_absdelta=None, draw_maxiter=100):
-    """The MGVI loop of optimize_kl with sample_mode "linear_resample":
-    per global iteration, draw ``n_samples`` antithetic linear samples around
-    the current mean, then minimise the KL over the mean with Newton-CG.
+                xtol=1e-6, maxiter=20, draw_rtol=1e-6, draw_absdelta=None, draw_maxiter=100,
+                sample_mode="linear_resample", nonlinear_kwargs=None):
+    """The VI loop of optimize_kl: per global iteration, draw ``n_samples``
+    antithetic samples around the current mean, then minimise the KL over
+    the mean with Newton-CG.  sample_mode "linear_resample" draws MGVI
+    samples; "nonlinear_resample" moves each linear sample to its geoVI
+    sample (geovi.nonlinear_update, options in ``nonlinear_kwargs``).
     Returns (xi_ref, xi_alpha, samples_ref, samples_alpha, history)."""
+    if sample_mode not in ("linear_resample", "nonlinear_resample"):
+        raise ValueError(f"unsupported sample_mode {sample_mode!r}")
     xr = torch.as_tensor(xi_ref, device="cuda").to(model.dtype)
     xa = None if xi_alpha is None else torch.as_tensor(xi_alpha, device="cuda").to(model.dtype)
     history = []
     s_ref = s_alpha = None
     for it in range(n_iterations):
-        s_ref, s_alpha, info = draw_linear_samples(model, xr, xa, n_samples=n_samples,
-                                                   seed=seed + it, rtol=draw_rtol,
-                                                   absdelta=draw_absdelta, maxiter=draw_maxiter)
+        if sample_mode == "nonlinear_resample":
+            from .geovi import draw_nonlinear_samples
+            s_ref, s_alpha, info = draw_nonlinear_samples(
+                model, xr, xa, n_samples=n_samples, seed=seed + it, draw_rtol=draw_rtol,
+                draw_absdelta=draw_absdelta, draw_maxiter=draw_maxiter,
+                **(nonlinear_kwargs or {}))
+        else:
+            s_ref, s_alpha, info = draw_linear_samples(model, xr, xa, n_samples=n_samples,
+                                                       seed=seed + it, rtol=draw_rtol,
+                                                       absdelta=draw_absdelta,
+                                                       maxiter=draw_maxiter)
         d_ref = (s_ref - xr[None]).contiguous()
         d_alpha = None if xa is None else (s_alpha - xa[None]).contiguous()
         xr, xa, hist = newton_cg(model, xr, xa, d_ref, d_alpha, xtol=xtol, maxiter=maxiter)

@@ -405,6 +405,61 @@ class XrayModel:
         inf = info.cpu().numpy()
         return (xr, xa), {"iterations": int(inf[0]), "rel_residual": float(inf[1])}
 
+    # ------------------------------------------------------------ geoVI
+    _GEO_MODES = {"K": 0, "KT": 1, "lh": 2, "metric": 3}
+
+    def geovi_point(self, x_ref, x_alpha=None):
+        """Evaluate the model at a sample point x for the geoVI operators
+        (jubik_model_geovi_point; the metric base point stays where the last
+        nll_and_grad put it).  Returns the NLL at x."""
+        self._check_point()
+        s = self.sky
+        xr = dev.to_device(x_ref, self.dtype).contiguous()
+        xa = dev.to_device(x_alpha, self.dtype).contiguous() if s.has_alpha else None
+        nll = torch.empty(1, dtype=torch.float64, device="cuda")
+        _lib.call("jubik_model_geovi_point", self._model, dev.ptr(s.amp_ref),
+                  dev.ptr(s.amp_alpha), dev.ptr(xr), dev.ptr(xa), s.zm, s.mu,
+                  dev.ptr(s.dnu_dev), dev.ptr(nll), dev.stream())
+        self._batch_S = 0  # the batch slot now holds x
+        self.check_domain()
+        return float(nll.item())
+
+    def geovi_apply(self, mode, v_ref=None, v_alpha=None):
+        """The geoVI operators at the base point p and the last geovi_point x:
+        "K": J(p)^T D_p D_x J(x) v, "KT": its transpose, "lh": J(p)^T D_p 2
+        (sqrt lam(x) - sqrt lam(p)) (no v), "metric": (K + K^T + K^T K) v."""
+        s = self.sky
+        m = self._GEO_MODES[mode]
+        vr = dev.to_device(v_ref, self.dtype).contiguous() if v_ref is not None else None
+        va = (dev.to_device(v_alpha, self.dtype).contiguous()
+              if (s.has_alpha and v_alpha is not None) else None)
+        o_ref, o_alpha = self._grads()
+        _lib.call("jubik_model_geovi_apply", self._model, m, dev.ptr(s.amp_ref),
+                  dev.ptr(s.amp_alpha), dev.ptr(s.dnu_dev), dev.ptr(vr), dev.ptr(va),
+                  dev.ptr(o_ref), dev.ptr(o_alpha), dev.stream())
+        return o_ref, o_alpha
+
+    def geovi_solve(self, b_ref, b_alpha=None, rtol=1e-6, absdelta=None, maxiter=100,
+                    check_every=10):
+        """x = (I + K + K^T + K^T K)^-1 b (the geoVI Gauss-Newton system at the
+        last geovi_point) by the device-resident CG; returns ((x_ref,
+        x_alpha), info)."""
+        s = self.sky
+        if not self._solver:
+            self.reserve_solver()
+            self._solver = True
+        br = dev.to_device(b_ref, self.dtype).contiguous()
+        ba = dev.to_device(b_alpha, self.dtype).contiguous() if s.has_alpha else None
+        x_ref = torch.empty_like(br)
+        x_alpha = torch.empty_like(ba) if ba is not None else None
+        info = torch.zeros(2, dtype=torch.float64, device="cuda")
+        _lib.call("jubik_model_geovi_solve", self._model, dev.ptr(s.amp_ref),
+                  dev.ptr(s.amp_alpha), dev.ptr(s.dnu_dev), dev.ptr(br), dev.ptr(ba),
+                  dev.ptr(x_ref), dev.ptr(x_alpha), int(maxiter), float(rtol),
+                  float(absdelta or 0.0), int(check_every), dev.ptr(info), dev.stream())
+        inf = info.cpu().numpy()
+        return (x_ref, x_alpha), {"iterations": int(inf[0]), "rel_residual": float(inf[1])}
+
     def capture_grad(self, samples=0):
         """Capture one gradient evaluation into an executable HIP graph.
 

@@ -98,13 +98,15 @@ def metric_solve(model: XrayModel, b_ref, b_alpha=None, rtol=1e-6, absdelta=None
 
 
 def draw_linear_samples(model: XrayModel, xi_ref, xi_alpha=None, n_samples=2, seed=0,
-                        rtol=1e-6, absdelta=None, maxiter=100):
+                        rtol=1e-6, absdelta=None, maxiter=100, return_metric_samples=False):
     """n_samples antithetic MGVI samples around (xi_ref, xi_alpha).
 
     Returns (samples_ref (n_samples, N, N), samples_alpha or None, info);
-    samples 2k and 2k + 1 are xi + x_k and xi - x_k.  The base point is
-    set by a gradient evaluation at xi; noise comes from a seeded torch
-    generator on the device.
+    samples 2k and 2k + 1 are xi + x_k and xi - x_k, x_k = (I + M)^-1 m_k
+    with m_k the metric samples.  With return_metric_samples the tuple ends
+    with (m_ref, m_alpha) (n_samples / 2, N, N) as well (geoVI needs them).
+    The base point is set by a gradient evaluation at xi; noise comes from a
+    seeded torch generator on the device.
     """
     if n_samples < 2 or n_samples % 2:
         raise ValueError("n_samples must be a positive even number (antithetic pairs)")
@@ -128,4 +130,6 @@ def draw_linear_samples(model: XrayModel, xi_ref, xi_alpha=None, n_samples=2, se
     s_ref = torch.stack([xr + x_ref, xr - x_ref], dim=1).reshape((2 * S, N, N))
     s_alpha = (torch.stack([xa + x_alpha, xa - x_alpha], dim=1).reshape((2 * S, N, N))
                if xa is not None else None)
+    if return_metric_samples:
+        return s_ref, s_alpha, info, b
     return s_ref, s_alpha, info
