@@ -398,27 +398,38 @@ __global__ __launch_bounds__(NT) void lp_cols(LinGeom g, const cplx<T>* __restri
       const int ir = row0 + cot_col<FOLD>(g, pos);
       return (ir >= 0 && ir < g.H) ? ld_stream(&in[ir * L2P + k]) : zero;
     } else if constexpr (MODE == COL_FWD) {
+      // the tent weight is applied after all loads are issued (tent_rows):
+      // a multiply here would make each task group wait for its loads
+      // before the next group's are issued
       const int rr = pos - m;
       const int ir = row0 + g.m + rr;
       const bool ok = k < LH && rr >= 0 && rr < w && ir >= 0 && ir < g.H;
-      const cplx<T> v = gather_load<cplx<T>>(gin, gather_off<cplx<T>>(gin, ok, ir * L2P + k));
-      return cscale(v, tent_weight<T>(g.d, tstep, rr));
+      return gather_load<cplx<T>>(gin, gather_off<cplx<T>>(gin, ok, ir * L2P + k));
     } else {
       if (k >= LH) return zero;
       const int rr = pos - m;
       return (rr >= 0 && rr < w) ? ld_stream(&in[rr * L2P + k]) : zero;
     }
   };
+  // forward: window row pos - m carries its tent weight
+  auto tent_rows = [&](typename F0::Regs& regs) {
+    if constexpr (MODE == COL_FWD)
+      F0::for_each(regs, tid, [&](int, int pos, cplx<T>& val) {
+        val = cscale(val, tent_weight<T>(g.d, tstep, pos - m));
+      });
+  };
   static_assert(F0::Ns == 1, "the first stage takes no twiddles");
   if constexpr (NS == 1) {
     F0::gather(last, tid, load_col);
     store_twiddles();
+    tent_rows(last);
     __syncthreads();
     F0::compute(last, tw, tid);
   } else {
     typename F0::Regs v0;
     F0::gather(v0, tid, load_col);
     store_twiddles();
+    tent_rows(v0);
     F0::compute(v0, tw, tid);
     F0::scatter_lds(buf, v0, tid);
     __syncthreads();
