@@ -81,7 +81,62 @@ __global__ __launch_bounds__(GNT) void gp_cols(GpGeom g, const T* __restrict__ a
   const Gather gar = make_gather(amp_ref, nn), gxr = make_gather(xi_ref, nn);
   const Gather gaa = make_gather(amp_alpha, g.has_alpha ? nn : 0);
   const Gather gxa = make_gather(xi_alpha, g.has_alpha ? nn : 0);
-  for (int u0 = 0; u0 < EPT; u0 += CHK) {
+  // Row-vector gathers: the tile's V adjacent columns of a row are one 8- or
+  // 16-byte load per array, V times fewer loads than one per element (the
+  // adjoint's planes need W to be a multiple of V, so that a tile lies wholly
+  // inside or wholly outside the sky grid).  Same products and sums per
+  // element as the per-element path below.
+  constexpr bool kRowVec = (V * sizeof(T) == 8 || V * sizeof(T) == 16) && N % V == 0;
+  bool row_vec = false;
+  if constexpr (kRowVec) {
+    row_vec = MODE == 0 || g.W % V == 0;
+    if (row_vec) {
+      using VT = T __attribute__((ext_vector_type(V)));
+      constexpr int RPT = (N + GNT - 1) / GNT;  // rows per thread
+      constexpr int CR = RPT < 4 ? RPT : 4;     // rows per chunk
+      for (int r0 = 0; r0 < RPT; r0 += CR) {
+        VT va[CR], vb[CR];
+#pragma unroll
+        for (int c = 0; c < CR; ++c) {
+          const int x1 = tid + (r0 + c) * GNT;
+          if constexpr (MODE == 0) {
+            const bool ok = x1 < N;
+            const int o = x1 * N + c0;
+            const int orf = gather_off<T>(gar, ok, o);
+            const int oal = gather_off<T>(gaa, ok && g.has_alpha, o);
+            va[c] = gather_load<VT>(gar, orf) * gather_load<VT>(gxr, orf);
+            vb[c] = gather_load<VT>(gaa, oal) * gather_load<VT>(gxa, oal);
+          } else {
+            va[c] = VT(T(0));
+            vb[c] = VT(T(0));
+          }
+        }
+        if constexpr (MODE == 1) {
+          for (int pl = 0; pl < g.uplanes; ++pl) {
+            const Gather gu = make_gather(u + pl * plane, plane);
+            const T dn = g.has_alpha ? dnu[pl % g.E] : T(0);
+#pragma unroll
+            for (int c = 0; c < CR; ++c) {
+              const int x1 = tid + (r0 + c) * GNT;
+              const bool ok = x1 < g.H && c0 < g.W;
+              const VT uv = gather_load<VT>(gu, gather_off<T>(gu, ok, x1 * g.W + c0));
+              va[c] += uv;
+              vb[c] += dn * uv;
+            }
+          }
+        }
+#pragma unroll
+        for (int c = 0; c < CR; ++c) {
+          const int x1 = tid + (r0 + c) * GNT;
+          if (x1 < N) {
+#pragma unroll
+            for (int v = 0; v < V; ++v) buf[v * VS + x1] = cplx<T>{va[c][v], vb[c][v]};
+          }
+        }
+      }
+    }
+  }
+  for (int u0 = 0; !row_vec && u0 < EPT; u0 += CHK) {
     cplx<T> val[CHK];
 #pragma unroll
     for (int c = 0; c < CHK; ++c) val[c] = cplx<T>{T(0), T(0)};

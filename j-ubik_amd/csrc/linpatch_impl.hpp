@@ -764,16 +764,11 @@ __global__ __launch_bounds__(NTO) void lp_fwd_out(LinGeom g, const cplx<T>* __re
   // loads pending at its head (twiddles, epilogue operands) would be waited
   // for before the spectrum loads are issued.
   auto pass = [&](int q0, auto first) {
-    sum_pair_spectra<T, L, false>(buf, base, g, lw, R, lo, nc, nullptr, q0, tid);
+    // Epilogue operands are fetched ahead of the spectrum loads: they are
+    // older in the load queue, so they are in registers by the time the
+    // spectra are, and their latency overlaps the spectrum loads instead of
+    // the transform (measured at 1024^2 x 4E, L = 384: 53.2 -> 51.2 us).
     if constexpr (decltype(first)::value) {
-#pragma unroll
-      for (int u = 0; u < TPT; ++u) {
-        const int e = tid + u * NTO;
-        if (e < L) tw[e] = twr[u];
-      }
-      // Epilogue operands are fetched once the spectrum loads are issued
-      // and consumed (fetched earlier, they hold those loads back); their
-      // latency overlaps the transform.
       if constexpr (EPI != 0) {
 #pragma unroll
         for (int t = 0; t < JPT; ++t) {
@@ -782,6 +777,14 @@ __global__ __launch_bounds__(NTO) void lp_fwd_out(LinGeom g, const cplx<T>* __re
           cnt_v[t] = j < g.W ? counts[rowd + j] : -1;
           if (EPI == 2) lam_v[t] = j < g.W ? y[rowl + j] : T(1);
         }
+      }
+    }
+    sum_pair_spectra<T, L, false>(buf, base, g, lw, R, lo, nc, nullptr, q0, tid);
+    if constexpr (decltype(first)::value) {
+#pragma unroll
+      for (int u = 0; u < TPT; ++u) {
+        const int e = tid + u * NTO;
+        if (e < L) tw[e] = twr[u];
       }
     }
     __syncthreads();
