@@ -960,7 +960,9 @@ int jubik_model_create_batched(jubik_model** model, jubik_gp* gp, jubik_linpatch
   if (!st) st = dev_alloc(&m->resid, img * g.J * S * es, "residual");
   if (!st) st = dev_alloc(&m->partial, 2 * (size_t)g.J * g.H * S * sizeof(double), "NLL partials");
   if (!st) st = dev_alloc(&m->nbad, S * sizeof(double), "lambda domain counts");
-  if (!st) st = dev_alloc(&m->u, img * g.E * S * es, "GP cotangent planes");
+  // GP cotangent: E real planes per sample, or one complex plane (energies
+  // folded in) per sample
+  if (!st) st = dev_alloc(&m->u, img * std::max(g.E, 2) * S * es, "GP cotangent planes");
   if (!st && S > 1) {
     st = dev_alloc(&m->R1, lp->ws_bytes * S, "batched workspace R1");
     if (!st) st = dev_alloc(&m->R2, lp->ws_bytes * S, "batched workspace R2");
@@ -1051,10 +1053,14 @@ int model_grad(jubik_model* model, int S, int slot0, const void* amp_ref, const 
   fin.bad_out = (double*)model->nbad;
   model->last_S = S;
   if (slot0 == 1) model->batch_S = S;
-  st = linpatch_adjoint_gp(lp->dtype, g, model->lbufs(), model->resid, model->pixel_area, sky, dnu,
-                           model->u, s, fin, ps ? model->ct : nullptr);
+  st = linpatch_adjoint_gp(lp->dtype, g, model->lbufs(), model->resid, model->pixel_area, sky,
+                           gp->g.has_alpha ? dnu : nullptr, model->u, s, fin,
+                           ps ? model->ct : nullptr);
   if (st) return st;
-  gg.uplanes = g.E;
+  // without point sources the adjoint folded the energies into one complex
+  // plane per sample
+  gg.uplanes = ps ? g.E : 1;
+  gg.ucomplex = ps ? 0 : 1;
   st = gp_adjoint_u(gp->dtype, gg, model->gbufs(), amp_ref, amp_alpha, dnu, model->u, g_ref,
                     g_alpha, s, hy);
   if (st || !ps) return st;
@@ -1414,15 +1420,16 @@ int model_metric(jubik_model* model, int S, bool per_sample, const void* amp_ref
   st = linpatch_forward_metric(lp->dtype, g, model->lbufs(), model->dsky, model->pixel_area,
                                model->expo, model->counts, model->lam_slot(slot), model->resid, s);
   if (st) return st;
-  st = linpatch_adjoint_gp(lp->dtype, g, model->lbufs(), model->resid, model->pixel_area, sky, dnu,
-                           model->u, s);
+  st = linpatch_adjoint_gp(lp->dtype, g, model->lbufs(), model->resid, model->pixel_area, sky,
+                           gp->g.has_alpha ? dnu : nullptr, model->u, s);
   if (st) return st;
+  gg.ucomplex = 1;
   if (per_sample) {
     // one GP adjoint of the cotangent planes of every sample (it is linear)
     gg.S = 1;
-    gg.uplanes = S * g.E;
+    gg.uplanes = S;
   } else {
-    gg.uplanes = g.E;
+    gg.uplanes = 1;
   }
   return gp_adjoint_u(gp->dtype, gg, model->gbufs(), amp_ref, amp_alpha, dnu, model->u, out_ref,
                       out_alpha, s);
@@ -1453,9 +1460,10 @@ int model_cross(jubik_model* model, bool to_p, const void* amp_ref, const void* 
                                model->expo, model->counts, model->geo, model->resid, s);
   if (st) return st;
   st = linpatch_adjoint_gp(lp->dtype, g, model->lbufs(), model->resid, model->pixel_area, sky_a,
-                           dnu, model->u, s);
+                           gp->g.has_alpha ? dnu : nullptr, model->u, s);
   if (st) return st;
-  gg.uplanes = g.E;
+  gg.uplanes = 1;
+  gg.ucomplex = 1;
   return gp_adjoint_u(gp->dtype, gg, model->gbufs(), amp_ref, amp_alpha, dnu, model->u, out_ref,
                       out_alpha, s);
 }
@@ -1627,14 +1635,15 @@ int jubik_model_geovi_apply(jubik_model* model, int mode, const void* amp_ref,
   GpGeom gg = gp->g;
   gg.S = 1;
   gg.sky_shared = 1;
-  gg.uplanes = g.E;
+  gg.uplanes = 1;
+  gg.ucomplex = 1;
   const int64_t npix = (int64_t)g.J * g.H * g.W;
   const void* eta = static_cast<char*>(model->geo) + npix * elem_size(lp->dtype);
   st = sqrt_metric_cot(lp->dtype, model->expo, model->counts, model->lam_slot(0), eta,
                        model->resid, npix, 1, s);
   if (!st)
     st = linpatch_adjoint_gp(lp->dtype, g, model->lbufs(), model->resid, model->pixel_area,
-                             model->sky_slot(0), dnu, model->u, s);
+                             model->sky_slot(0), gp->g.has_alpha ? dnu : nullptr, model->u, s);
   if (!st)
     st = gp_adjoint_u(gp->dtype, gg, model->gbufs(), amp_ref, amp_alpha, dnu, model->u, out_ref,
                       out_alpha, s);
@@ -1742,13 +1751,14 @@ int jubik_model_sqrt_metric_batch(jubik_model* model, int64_t S, const void* amp
   GpGeom gg = gp->g;
   gg.S = (int)S;
   gg.sky_shared = 1;
-  gg.uplanes = g.E;
+  gg.uplanes = 1;
+  gg.ucomplex = 1;
   const int64_t n = (int64_t)g.J * g.H * g.W;
   int st = sqrt_metric_cot(lp->dtype, model->expo, model->counts, model->lam_slot(0), noise,
                            model->resid, n, (int)S, s);
   if (st) return st;
   st = linpatch_adjoint_gp(lp->dtype, g, model->lbufs(), model->resid, model->pixel_area,
-                           model->sky_slot(0), dnu, model->u, s);
+                           model->sky_slot(0), gp->g.has_alpha ? dnu : nullptr, model->u, s);
   if (st) return st;
   return gp_adjoint_u(gp->dtype, gg, model->gbufs(), amp_ref, amp_alpha, dnu, model->u, out_ref,
                       out_alpha, s);

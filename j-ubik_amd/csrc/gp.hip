@@ -36,7 +36,8 @@ struct GpTile {
 // grid (ceil(N / V), S).  MODE 0: c = amp * xi (forward / tangent);  MODE 1:
 // c = sum_p u_p * (1 + i dnu[p mod E]) over the uplanes real (H, W)
 // cotangent planes u_p (one per energy, or per sample and energy), zero
-// outside the sky grid.  Sample blockIdx.y reads xi[s] / u[s] and writes
+// outside the sky grid; with g.ucomplex, c = sum_p u_p over uplanes complex
+// planes that hold the energies folded in already.  Sample blockIdx.y reads xi[s] / u[s] and writes
 // work[s].
 // When V does not divide N (odd grids such as 567) the last tile's columns
 // beyond N read zero and are not stored.
@@ -58,7 +59,7 @@ __global__ __launch_bounds__(GNT) void gp_cols(GpGeom g, const T* __restrict__ a
   const int64_t smp = blockIdx.y;
   if (xi_ref) xi_ref += smp * N * N;
   if (xi_alpha) xi_alpha += smp * N * N;
-  if (u) u += smp * g.uplanes * g.H * g.W;
+  if (u) u += smp * g.uplanes * g.H * g.W * (g.ucomplex ? 2 : 1);
   Tw += smp * N * N;
   // Loads are issued in unrolled chunks of CH elements per thread (their
   // latencies overlap), twiddles first.
@@ -87,9 +88,11 @@ __global__ __launch_bounds__(GNT) void gp_cols(GpGeom g, const T* __restrict__ a
   // inside or wholly outside the sky grid).  Same products and sums per
   // element as the per-element path below.
   constexpr bool kRowVec = (V * sizeof(T) == 8 || V * sizeof(T) == 16) && N % V == 0;
+  // complex planes: a row of the tile is V complex values
+  constexpr bool kRowVecC = 2 * V * sizeof(T) == 16 && N % V == 0;
   bool row_vec = false;
   if constexpr (kRowVec) {
-    row_vec = MODE == 0 || g.W % V == 0;
+    row_vec = MODE == 0 || (g.W % V == 0 && (!g.ucomplex || kRowVecC));
     if (row_vec) {
       using VT = T __attribute__((ext_vector_type(V)));
       constexpr int RPT = (N + GNT - 1) / GNT;  // rows per thread
@@ -112,16 +115,38 @@ __global__ __launch_bounds__(GNT) void gp_cols(GpGeom g, const T* __restrict__ a
           }
         }
         if constexpr (MODE == 1) {
-          for (int pl = 0; pl < g.uplanes; ++pl) {
-            const Gather gu = make_gather(u + pl * plane, plane);
-            const T dn = g.has_alpha ? dnu[pl % g.E] : T(0);
+          if (g.ucomplex) {
+            if constexpr (kRowVecC) {
+              using VT2 = T __attribute__((ext_vector_type(2 * V)));
+              for (int pl = 0; pl < g.uplanes; ++pl) {
+                const cplx<T>* up = reinterpret_cast<const cplx<T>*>(u) + pl * plane;
+                const Gather gu = make_gather(up, plane);
 #pragma unroll
-            for (int c = 0; c < CR; ++c) {
-              const int x1 = tid + (r0 + c) * GNT;
-              const bool ok = x1 < g.H && c0 < g.W;
-              const VT uv = gather_load<VT>(gu, gather_off<T>(gu, ok, x1 * g.W + c0));
-              va[c] += uv;
-              vb[c] += dn * uv;
+                for (int c = 0; c < CR; ++c) {
+                  const int x1 = tid + (r0 + c) * GNT;
+                  const bool ok = x1 < g.H && c0 < g.W;
+                  const VT2 w =
+                      gather_load<VT2>(gu, gather_off<cplx<T>>(gu, ok, x1 * g.W + c0));
+#pragma unroll
+                  for (int v = 0; v < V; ++v) {
+                    va[c][v] += w[2 * v];
+                    vb[c][v] += w[2 * v + 1];
+                  }
+                }
+              }
+            }
+          } else {
+            for (int pl = 0; pl < g.uplanes; ++pl) {
+              const Gather gu = make_gather(u + pl * plane, plane);
+              const T dn = g.has_alpha ? dnu[pl % g.E] : T(0);
+#pragma unroll
+              for (int c = 0; c < CR; ++c) {
+                const int x1 = tid + (r0 + c) * GNT;
+                const bool ok = x1 < g.H && c0 < g.W;
+                const VT uv = gather_load<VT>(gu, gather_off<T>(gu, ok, x1 * g.W + c0));
+                va[c] += uv;
+                vb[c] += dn * uv;
+              }
             }
           }
         }
@@ -153,6 +178,19 @@ __global__ __launch_bounds__(GNT) void gp_cols(GpGeom g, const T* __restrict__ a
         const int orf = gather_off<T>(gar, ok, o), oal = gather_off<T>(gaa, ok && g.has_alpha, o);
         val[c].re = gather_load<T>(gar, orf) * gather_load<T>(gxr, orf);
         val[c].im = gather_load<T>(gaa, oal) * gather_load<T>(gxa, oal);
+      }
+    } else if (g.ucomplex) {
+      for (int pl = 0; pl < g.uplanes; ++pl) {
+        const cplx<T>* up = reinterpret_cast<const cplx<T>*>(u) + pl * plane;
+        const Gather gu = make_gather(up, plane);
+#pragma unroll
+        for (int c = 0; c < CHK; ++c) {
+          const int e = tid + (u0 + c) * GNT;
+          const int x1 = e / V, v = e - (e / V) * V;
+          const int x2 = c0 + v;
+          const bool ok = (V * N % GNT == 0 || e < V * N) && x1 < g.H && x2 < g.W;
+          val[c] = cadd(val[c], gather_load<cplx<T>>(gu, gather_off<cplx<T>>(gu, ok, x1 * g.W + x2)));
+        }
       }
     } else {
       for (int pl = 0; pl < g.uplanes; ++pl) {

@@ -16,6 +16,9 @@ struct GpGeom {
   int sky_shared; // tangent: 1 = one base-point sky for every sample
   int work_shared; // tangent: 1 = one tangent shared by the samples (its column
                    // pass runs once; each sample's row pass reads work[0])
+  int ucomplex = 0; // adjoint input: the uplanes planes are complex (H, W) planes
+                    // with the energies folded in (sum_e u_e (1 + i dnu_e)),
+                    // summed as they are
 };
 
 // Optional outputs of the adjoint: the gradients with respect to the

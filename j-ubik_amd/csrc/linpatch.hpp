@@ -82,11 +82,13 @@ struct NllFinal {
   double* bad_out = nullptr;
 };
 
-// Adjoint chained into the GP: u[s, e] = sum_mod scale * sky[e] *
-// PSF^T(gy)_(s, mod, e), real (S, E, H, W) planes (dnu is unused: the GP
-// adjoint applies it); optionally finalises the NLL.  With ct set it also
-// receives sum_mod scale * PSF^T(gy)_(s, mod, e), the cotangent of the sky
-// the PSF saw (point sources added to the diffuse sky).
+// Adjoint chained into the GP, with v_(s,e) = sum_mod scale * sky[e] *
+// PSF^T(gy)_(s, mod, e); optionally finalises the NLL.  Without ct, u gets one
+// complex (H, W) plane per sample, sum_e v_(s,e) (1 + i dnu_e) (dnu NULL:
+// real part only), read by the GP adjoint with GpGeom::ucomplex.  With ct
+// set, u gets the real (S, E, H, W) planes v (the GP adjoint applies dnu) and
+// ct receives sum_mod scale * PSF^T(gy)_(s, mod, e), the cotangent of the
+// sky the PSF saw (point sources added to the diffuse sky).
 int linpatch_adjoint_gp(int dtype, const LinGeom& g, const LinBuffers& b, const void* gy,
                         double scale, const void* sky, const void* dnu, void* u, hipStream_t s,
                         NllFinal fin = NllFinal{}, void* ct = nullptr);

@@ -876,6 +876,10 @@ __global__ __launch_bounds__(NTO) void lp_fwd_out(LinGeom g, const cplx<T>* __re
 }
 
 // EPI 0: grid (H, nimg); gx[img] = scale * sum over the jobs of img.
+// EPI 3: grid (H, S); one complex plane per sample,
+//        u[s] = sum_e (sum_mod scale * sky[s, e] * xbar_(s,mod,e)) (1 + i dnu_e)
+//        (dnu NULL: real part only), the GP adjoint's column input c with the
+//        energies already folded in, jobs summed in their order (s, mod, e).
 // EPI 1: grid (H, S * E);  u[s, e] = sum_mod scale * sky[s, e] * xbar_(s,mod,e),
 //        one real plane per sample and energy, the cotangent of log s (the GP
 //        adjoint folds the energies, with their dnu, while loading them);
@@ -886,7 +890,8 @@ __global__ __launch_bounds__(NTO) void lp_adj_out(LinGeom g, const cplx<T>* __re
                                                  const T* __restrict__ tent,
                                                  T* __restrict__ gx, T scale,
                                                  const T* __restrict__ sky,
-                                                 T* __restrict__ u) {
+                                                 T* __restrict__ u,
+                                                 const T* __restrict__ dnu) {
   using OT = OutTile<T, L>;
   constexpr int V = OT::V, TPT = OT::TPT;
   constexpr int VS = lds_stride(L, V);
@@ -927,9 +932,16 @@ __global__ __launch_bounds__(NTO) void lp_adj_out(LinGeom g, const cplx<T>* __re
     je = blockIdx.y + 1;
     jstep = 1;
   }
+  if (EPI == 3) {  // every job of sample blockIdx.y
+    jb = blockIdx.y * g.J;
+    je = jb + g.J;
+    jstep = 1;
+  }
 
   // EPI 1: the sky row of this image, fetched once the first spectrum loads
   // are issued and consumed (its latency then overlaps the transform).
+  // EPI 3: the sky row of each job's energy, fetched ahead of its spectrum
+  // loads.
   T sky_v[JPT];
   const float rd = 1.0f / (float)g.d;
   // One batch of V patch-column pairs of one job; the first batch of the
@@ -978,6 +990,18 @@ __global__ __launch_bounds__(NTO) void lp_adj_out(LinGeom g, const cplx<T>* __re
   };
   for (int job = jb; job < je; job += jstep) {
     const cplx<T>* base = R2 + (int64_t)job * g.P * g.w * L2P;
+    T dn = T(0);
+    if constexpr (EPI == 3) {
+      const int e = (job - jb) % g.E;
+      const int simg = g.sky_shared ? e : blockIdx.y * g.E + e;
+      const T* srow = sky + ((int64_t)simg * g.H + i) * g.W;
+#pragma unroll
+      for (int t = 0; t < JPT; ++t) {
+        const int j = tid + t * NTO;
+        sky_v[t] = j < g.W ? srow[j] : T(0);
+      }
+      if (dnu) dn = dnu[e];
+    }
 #pragma unroll
     for (int t = 0; t < JPT; ++t) tmp[t] = T(0);
     int q0 = 0;
@@ -994,13 +1018,27 @@ __global__ __launch_bounds__(NTO) void lp_adj_out(LinGeom g, const cplx<T>* __re
       for (int t = 0; t < JPT; ++t) {
         const int j = tid + t * NTO;
         if (j < g.W) {
-          acc[t] += scale * sky_v[t] * tmp[t];
-          tot[t] += scale * tmp[t];
+          if (EPI == 3) {
+            // folded: real part and dnu-weighted imaginary part (in tot)
+            const T v = scale * sky_v[t] * tmp[t];
+            acc[t] += v;
+            tot[t] += dn * v;
+          } else {
+            acc[t] += scale * sky_v[t] * tmp[t];
+            tot[t] += scale * tmp[t];
+          }
         }
       }
     }
   }
-  if (EPI == 0) {
+  if (EPI == 3) {
+    cplx<T>* urow = reinterpret_cast<cplx<T>*>(u) + ((int64_t)blockIdx.y * g.H + i) * g.W;
+#pragma unroll
+    for (int t = 0; t < JPT; ++t) {
+      const int j = tid + t * NTO;
+      if (j < g.W) urow[j] = cplx<T>{acc[t], tot[t]};
+    }
+  } else if (EPI == 0) {
     const int64_t row = ((int64_t)blockIdx.y * g.H + i) * g.W;
 #pragma unroll
     for (int t = 0; t < JPT; ++t) {
@@ -1194,11 +1232,12 @@ struct LinOps {
 
   template <int EPI>
   static void adj_out(const LinGeom& g, const LinBuffers& b, int nimg, T* gx, double scale,
-                      const T* sky, T* u, hipStream_t s, const char* name) {
+                      const T* sky, T* u, hipStream_t s, const char* name,
+                      const T* dnu = nullptr) {
     TimedLaunch _t(name, s);
     with_jpt(g, [&](auto jpt) {
       lp_adj_out<T, L, EPI, decltype(jpt)::value><<<dim3(g.H, nimg), NTO, 0, s>>>(
-          g, (const C*)b.R2, (const C*)b.tw, (const T*)b.tent, gx, T(scale), sky, u);
+          g, (const C*)b.R2, (const C*)b.tw, (const T*)b.tent, gx, T(scale), sky, u, dnu);
     });
   }
 
@@ -1227,8 +1266,15 @@ struct LinOps {
         return launch_status("linpatch_adjoint");
       case LinCall::kAdjointGp:
         if (int st = adj_front(g, b, c.in, s, c.fin)) return st;
-        adj_out<1>(g, b, g.S * g.E, (T*)c.ct, c.scale, (const T*)c.sky, (T*)c.out, s,
-                   "lp_adj_out_gp");
+        // without point sources: one folded complex plane per sample;
+        // with them: a real plane per sample and energy, and the total-sky
+        // cotangent in c.ct
+        if (c.ct)
+          adj_out<1>(g, b, g.S * g.E, (T*)c.ct, c.scale, (const T*)c.sky, (T*)c.out, s,
+                     "lp_adj_out_gp");
+        else
+          adj_out<3>(g, b, g.S, nullptr, c.scale, (const T*)c.sky, (T*)c.out, s,
+                     "lp_adj_out_gp", (const T*)c.dnu);
         return launch_status("linpatch_adjoint_gp");
       default:
         return fail(kInvalidArgument, "unknown linpatch operation");
