@@ -113,6 +113,12 @@ __device__ __forceinline__ int64_t spec_index(const LinGeom& g, int job, int p) 
   return ((int64_t)mod * g.P + p) * g.Ek + (g.Ek == 1 ? 0 : e);
 }
 
+// Index of the shared row spectra of job z of sample smp: the job, or for a
+// forward signal shared by the modules the image (smp, energy).
+__device__ __forceinline__ int r1_image(const LinGeom& g, int smp, int z, bool fwd) {
+  return (fwd && g.x_shared) ? smp * g.E + z % g.E : smp * g.J + z;
+}
+
 // Tent weight t of a 2d window (jubik/convolve.py:18-25): numpy
 // linspace(0, 1, d) then its mirror image; entry i of linspace is i * step
 // with step = 1 / (d - 1) in double and the last entry exactly 1.  IEEE

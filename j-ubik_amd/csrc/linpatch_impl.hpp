@@ -53,7 +53,9 @@ __device__ __forceinline__ int cot_col(const LinGeom& g, int f) {
 // ------------------------------------------------------------------ rows
 // Kernel modes: grid (ceil(w/2 / V), P, M*Ek), one spectrum's w window rows
 // per (patch, kernel).  Signal and cotangent modes: grid (ceil(H/2 / V), n,
-// jobs), the row spectra of image rows shared by every patch row: for patch
+// jobs, or S * E images for a signal shared by the modules (x_shared): its
+// row spectra do not depend on the module), the row spectra of image rows
+// shared by every patch row: for patch
 // column iy and image row i, the row FFT of the window columns of iy
 //   signal:    tent(c) * scale * x[i, sy + c - h], c < w, at frame f = c + mf;
 //   cotangent: y[i, sy + cot_col(f) - h - m] at frame f < L,
@@ -85,11 +87,10 @@ __global__ __launch_bounds__(NT) void lp_rows(LinGeom g, const T* __restrict__ s
   int64_t r1base;
   const T* img;
   if constexpr (MODE == ROW_SIGNAL || MODE == ROW_COTANGENT) {
-    const int job = z;
-    r1base = (int64_t)job * g.n + p;
-    int im = job;  // shared input: image (sample, energy) of the sky stack
-    if (MODE == ROW_SIGNAL && g.x_shared) im = (job / g.J) * g.E + job % g.E;
-    img = src + (int64_t)im * g.H * g.W;
+    // z: the job, or for a shared signal the image (sample, energy) of the
+    // sky stack; R1 is laid out by the same index
+    r1base = (int64_t)z * g.n + p;
+    img = src + (int64_t)z * g.H * g.W;
   } else {
     const int mod = z / g.Ek, ek = z - mod * g.Ek;
     const int64_t spec = ((int64_t)mod * g.P + p) * g.Ek + ek;
@@ -340,7 +341,7 @@ __global__ __launch_bounds__(NT) void lp_cols(LinGeom g, const cplx<T>* __restri
   if constexpr (MODE == COL_KERNEL)
     in = R1 + idx * L2P * g.w;
   else
-    in = R1 + ((int64_t)(smp * g.J + z) * g.n + p / g.n) * g.H * L2P;
+    in = R1 + ((int64_t)r1_image(g, smp, z, MODE == COL_FWD) * g.n + p / g.n) * g.H * L2P;
   const int m = g.mf, w = g.w;
 
   // Twiddles are fetched into registers now and stored to LDS once the tile
@@ -1125,7 +1126,7 @@ struct LinOps {
       TimedLaunch _t("lp_rows_fwd", s);
       with_rows(g, b, [&](auto vr) {
         constexpr int VRC = decltype(vr)::value;
-        dim3 gr(((g.H + 1) / 2 + VRC - 1) / VRC, g.n, g.S * g.J);
+        dim3 gr(((g.H + 1) / 2 + VRC - 1) / VRC, g.n, g.S * (g.x_shared ? g.E : g.J));
         lp_rows<T, L, ROW_SIGNAL, VRC><<<gr, NT, 0, s>>>(g, (const T*)x, (const T*)b.tent,
                                                          (const C*)b.tw, (C*)b.R1, T(scale), g.H);
       });

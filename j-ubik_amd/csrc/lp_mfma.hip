@@ -253,7 +253,7 @@ __global__ __launch_bounds__(MNT) void lp_cols_mfma(LinGeom g, const bf16x8* __r
     const int p = item % g.P;
     const int z = item / g.P;
     it.idx = (int64_t)(smp * g.J + z) * g.P + p;
-    it.rows = ((int64_t)(smp * g.J + z) * g.n + p / g.n) * g.H;
+    it.rows = ((int64_t)r1_image(g, smp, z, !ADJ) * g.n + p / g.n) * g.H;
     it.spec = spec_index(g, z, p);
     // image row of frame row 0: rows outside the image are zero on input
     // (never stored) and never read on output
