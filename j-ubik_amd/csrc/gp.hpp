@@ -10,7 +10,8 @@ struct GpGeom {
   int E;          // energies
   int has_alpha;  // spectral-index field present
   double zm, mu;  // zero mode and spectral-index mean
-  int uplanes;    // adjoint input: number of real (H, W) planes summed on load
+  int uplanes;    // adjoint input: number of (H, W) planes summed on load (real,
+                  // or complex with ucomplex)
   int S;          // samples per launch (grid axis y); inputs and outputs are
                   // stacked per sample, the amplitude fields are shared
   int sky_shared; // tangent: 1 = one base-point sky for every sample
