@@ -21,15 +21,13 @@ CASES = [
     dict(H=100, n=4, margin=7, E=2, M=2, seed=61, N=128),   # stride 25 odd: clamped window
     dict(H=98, n=3, margin=16, E=3, M=1, seed=62, N=128),   # n does not divide H, odd n
     dict(H=60, n=5, margin=12, E=1, M=3, seed=63, N=64),    # odd n, three modules, L = 48
+    # odd sky width (the GP adjoint reads the folded cotangent element-wise),
+    # clamped last window, spectral
+    dict(H=99, n=3, margin=15, E=2, M=1, seed=64, N=128),
 ]
 
 
-def _ok(c):
-    L = 2 * (c["H"] // c["n"]) + 2 * c["margin"]
-    return L in (16, 24, 32, 48, 64, 96, 128, 192, 224, 256, 324, 384)
-
-
-@pytest.mark.parametrize("cfg", [c for c in CASES if _ok(c)])
+@pytest.mark.parametrize("cfg", CASES)
 @pytest.mark.parametrize("dtype,tol", [(torch.float64, 1e-10), (torch.float32, 1e-5)])
 def test_grad_and_metric_vs_oracle(ja, cfg, dtype, tol):
     prob, xi = make_problem(**cfg)
