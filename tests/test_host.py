@@ -82,6 +82,37 @@ def test_tuning_and_batch_graph_entry_points_validate_arguments(ja):
     assert st == _lib.EINVAL and b"descriptor 0" in lib.jubik_last_error()
 
 
+def test_newer_entry_points_validate_arguments(ja):
+    """The transform switch, the general convolve, the point-source sky, the
+    inverse-gamma prior and the geoVI entry points reject NULL handles and
+    bad arguments with a status and a message, before any device work."""
+    from jubik_amd import _lib
+    lib = _lib.load()
+    assert lib.jubik_linpatch_set_transform(None, 0) == _lib.EINVAL
+    buf = (ctypes.c_float * 16)()
+    # convolve: NULL operands, empty axes, bad dtype
+    assert lib.jubik_convolve(0, None, buf, 1, 1, 4, 4, 1.0, buf, buf, None) == _lib.EINVAL
+    st = lib.jubik_convolve(0, buf, buf, 1, 1, 0, 4, 1.0, buf, buf, None)
+    assert st == _lib.EINVAL and b"empty" in lib.jubik_last_error()
+    assert lib.jubik_convolve(5, buf, buf, 1, 1, 4, 4, 1.0, buf, buf, None) == _lib.EINVAL
+    # point sources and the inverse-gamma prior
+    assert lib.jubik_model_set_point_sources(None, 1, 1.0, 1.0, 0.0, 1.0) == _lib.EINVAL
+    assert lib.jubik_model_grad_points(None, 1, *([None] * 4), 0.0, 0.0,
+                                       *([None] * (len(_lib.SIGNATURES["jubik_model_grad_points"]) - 8))) == _lib.EINVAL
+    st = lib.jubik_invgamma_prior(0, None, 0, -1.0, 1.0, None, None, None)
+    assert st == _lib.EINVAL and b"a > 0" in lib.jubik_last_error()
+    assert lib.jubik_invgamma_prior(0, None, 4, 1.0, 1.0, None, None, None) == _lib.EINVAL
+    assert lib.jubik_invgamma_prior(3, buf, 4, 1.0, 1.0, buf, buf, None) == _lib.EINVAL
+    # geoVI
+    nll = ctypes.c_double()
+    assert lib.jubik_model_geovi_point(None, None, None, None, None, 0.0, 0.0, None, None,
+                                       ctypes.byref(nll)) == _lib.EINVAL
+    assert lib.jubik_model_geovi_apply(None, 0, *([None] * 8)) == _lib.EINVAL
+    nargs = len(_lib.SIGNATURES["jubik_model_geovi_solve"])
+    assert lib.jubik_model_geovi_solve(*([None] * 8), 10, 1e-6, 0.0, 1,
+                                       *([None] * (nargs - 12))) == _lib.EINVAL
+
+
 def test_grad_desc_layout_matches_the_header(ja, tmp_path):
     """The ctypes mirror of jubik_grad_desc has the C struct's size and
     field offsets (checked with the host C compiler)."""
