@@ -479,14 +479,14 @@ def nmax(a, b):
 
 def main():
     args = parse()
-    if args.sweep:
-        sweep(args)
-        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
               "(the hot path is sharded by module/energy/sample across GPUs by design, "
               "but only the one-GPU figure is measured)", file=sys.stderr)
         sys.exit(2)
+    if args.sweep:
+        sweep(args)
+        return
     import torch
     import __graft_entry__ as ge
     ja = ge.load_package()

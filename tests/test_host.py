@@ -273,3 +273,17 @@ def test_save_mock_data_writes_the_reference_pickles(ja, tmp_path):
         p = pickle.load(f)
     assert set(d) == {"tm_1", "tm_2"} and np.array_equal(d["tm_1"], [0, 3, 7])
     assert np.array_equal(p["xi_ref"], pos["xi_ref"]) and np.array_equal(p["xi_alpha"], np.ones((2, 3)))
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py measures one GPU: --gpus N > 1 exits with status 2 and says
+    so, before touching a device (also with --sweep)."""
+    import subprocess
+    import sys
+    from pathlib import Path
+    root = Path(__file__).resolve().parent.parent
+    for extra in ([], ["--sweep"]):
+        res = subprocess.run([sys.executable, str(root / "bench.py"), "--gpus", "2", *extra],
+                             capture_output=True, text=True, timeout=120, cwd=root)
+        assert res.returncode == 2
+        assert "measures one GPU" in res.stderr
