@@ -533,6 +533,12 @@ def main():
     t_graph = time_steps(torch, graph.replay, args.steps, args.warmup)
     del graph
 
+    # host buffers (numpy positions in, numpy gradients out, the NLL read
+    # back): the PCIe-inclusive rate of the drop-in functions, not the value
+    xi_host = [None if v is None else np.ascontiguousarray(v, dtype=np.float32)
+               for v in inp["xi"]]
+    t_host = time_steps(torch, lambda: mixed.nll_and_grad(*xi_host), max(3, args.steps // 5), 2)
+
     # S VI-sample positions per call (jubik_model_grad_batch)
     batched = None
     if args.samples > 1:
@@ -709,6 +715,9 @@ def main():
         "speedup_vs_plain_fp64": (value / best_plain["evals_per_s"]) if best_plain else None,
         "grad_err_vs_cpu_oracle": grad_vs_oracle,
         "hip_graph": {"ms_per_step": t_graph * 1e3, "evals_per_s": 1.0 / t_graph},
+        "host_buffers": {"ms_per_step": t_host * 1e3, "evals_per_s": 1.0 / t_host,
+                         "what": "numpy float32 positions in, numpy gradients out, NLL read "
+                                 "back each call (PCIe both ways); not the bench value"},
         "batched_samples": batched,
         "kernels": kernels,
     }
