@@ -17,21 +17,21 @@ def build_exposure_function(exposures, exposure_cut=None):
     """jubik/response.py:12-45: returns ``x -> exposures * x``.
 
     Entries below ``exposure_cut`` are zeroed in place in ``exposures`` (the
-    reference mutates the caller's array, which Chandra relies on).
+    reference mutates the caller's array, which Chandra relies on).  Like
+    the reference's closure, each call multiplies by the array as it is at
+    that call: later in-place edits of ``exposures`` are seen (a device
+    tensor of the working dtype is used without a copy; a host array is
+    copied to the device per call).
     """
     if exposure_cut is not None:
         if exposure_cut < 0:
             raise ValueError("exposure_cut should be non-negative or None!")
         exposures[exposures < exposure_cut] = 0
-    cache = {}
 
     def exposure(x):
         dtype = dev.float_dtype(exposures, x)
         dev.require_device()
-        key = (dtype, torch.cuda.current_device())
-        if key not in cache:
-            cache[key] = dev.to_device(exposures, dtype)
-        e = cache[key]
+        e = dev.to_device(exposures, dtype)
         xt = dev.to_device(x, dtype)
         try:
             oshape = tuple(torch.broadcast_shapes(tuple(e.shape), tuple(xt.shape)))

@@ -66,6 +66,25 @@ def test_exposure_matches_reference_fixture(ja):
     np.testing.assert_array_equal(got, exposures * x)
 
 
+def test_exposure_sees_later_in_place_edits(ja):
+    """The reference's closure multiplies by the caller's array as it is at
+    each call (jubik/response.py:12-45): an in-place edit after the first call
+    shows in the next one, for host arrays and device tensors alike."""
+    rng = np.random.default_rng(3)
+    x = rng.uniform(0.5, 1.5, size=(2, 16, 16))
+    e = rng.uniform(1.0, 2.0, size=(2, 16, 16))
+    f = ja.build_exposure_function(e)
+    np.testing.assert_array_equal(f(x), e * x)
+    e[0] *= 3.0
+    np.testing.assert_array_equal(f(x), e * x)
+    et = torch.as_tensor(e, device="cuda")
+    ft = ja.build_exposure_function(et)
+    xt = torch.as_tensor(x, device="cuda")
+    assert torch.equal(ft(xt), et * xt)
+    et[1] = 0.0
+    assert torch.equal(ft(xt), et * xt)
+
+
 def test_readout_matches_reference_fixture(ja):
     """jubik test_build_exposure_readout_function.py fixtures."""
     exposures = np.random.default_rng(0).uniform(0.0, 3e3, size=2 * 24 ** 2 * 3).reshape(
