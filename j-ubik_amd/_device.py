@@ -5,6 +5,7 @@ current HIP device as contiguous float32/float64 tensors.  Results are
 returned as the same kind of array as the primary input.
 """
 import ctypes as C
+import threading
 
 import numpy as np
 import torch
@@ -48,11 +49,37 @@ def to_device(a, dtype):
     return t.contiguous()
 
 
+# Pinned (page-locked) staging buffers for device-to-host results, one per
+# (device, dtype, element count), reused across calls: a download from
+# device memory into pinned host memory runs at the DMA rate, several times
+# the rate of a download into pageable memory (measured on MI355X: 2 x 4 MB
+# in 0.18 ms pinned against 1.6 ms pageable).  The caller receives a fresh
+# numpy copy, never a view of the staging buffer.
+_STAGING = {}
+_STAGING_MAX_BYTES = 256 << 20
+_STAGING_LOCK = threading.Lock()
+
+
+def _download(t):
+    t = t.detach()
+    if t.device.type != "cuda" or t.numel() * t.element_size() > _STAGING_MAX_BYTES:
+        return t.cpu().numpy()
+    key = (t.device.index, t.dtype, t.numel())
+    with _STAGING_LOCK:
+        pin = _STAGING.get(key)
+        if pin is None:
+            pin = torch.empty(t.numel(), dtype=t.dtype, pin_memory=True)
+            _STAGING[key] = pin
+        pin.copy_(t.reshape(-1), non_blocking=True)
+        torch.cuda.current_stream(t.device).synchronize()
+        return pin.numpy().reshape(tuple(t.shape)).copy()
+
+
 def like_input(t, ref):
     """Return ``t`` as the kind of array ``ref`` is (torch stays on device)."""
     if isinstance(ref, torch.Tensor):
         return t
-    return t.detach().cpu().numpy()
+    return _download(t)
 
 
 def ptr(t):
