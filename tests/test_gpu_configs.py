@@ -5,7 +5,10 @@ Uneven grids: three observations on 256^2 / 512^2 / 768^2 grids with
     n = 6 / 8 / 12 and margins 22 / 32 / 48 (L = 128 / 192 / 224) and
     metric products.
 Wide: 2048^2 x 4 energies, n = 16, margin 64 (L = 384).
-Small instances are checked against the float64 oracle; the full sizes
+Gradient workload: 1024^2 x 4 energies, n = 8, margin 64 (L = 384), spectral
+sky, Gaussian+Moffat PSFs, exposure ramp, 5 % mask, against the float64
+oracle at full size.
+Small instances are checked against the float64 oracle; the other full sizes
 through FP32 vs FP64 kernels, adjointness and linearity.
 """
 import numpy as np
@@ -181,3 +184,39 @@ def test_wide_2048x4E_gradient_fp32_vs_fp64(ja):
         assert err <= 1e-5
         n64 = res[torch.float64][0][s]
         assert abs(res[torch.float32][0][s] - n64) <= 1e-6 * abs(n64)
+
+
+_WORKLOAD = {}
+
+
+def _workload_oracle():
+    """The 1024^2 x 4E problem and its float64 oracle NLL, gradient and one
+    metric product (computed once for both precisions)."""
+    if not _WORKLOAD:
+        prob, (xr, xa) = make_problem(H=1024, n=8, margin=64, E=4, M=1, seed=75,
+                                      expo_range=(5e2, 2e3), mask_frac=0.05, moffat=2.5,
+                                      sigma=(1.0, 4.0))
+        want_nll, want_g = om.nll_and_grad(prob, xr, xa)
+        rng = np.random.default_rng(76)
+        v = (rng.normal(size=xr.shape), rng.normal(size=xa.shape))
+        _WORKLOAD.update(prob=prob, x=(xr, xa), nll=want_nll, g=want_g, v=v,
+                         m=om.metric_product(prob, xr, xa, *v))
+    return _WORKLOAD
+
+
+@pytest.mark.parametrize("dtype,tol", [(torch.float64, 1e-10), (torch.float32, 1e-5)])
+def test_gradient_workload_1024x4E_vs_oracle(ja, dtype, tol):
+    """The benchmarked evaluation at full size (1024^2 x 4 energies, n = 8,
+    margin 64, L = 384, spectral sky, Gaussian+Moffat PSFs of core sigma
+    1-4 px, exposure ramp 5e2-2e3 s, 5 % mask): NLL, gradient and one metric
+    product against the float64 oracle, the FP32 kernels within the 1e-5
+    gradient budget."""
+    w = _workload_oracle()
+    model = to_product(ja, w["prob"], dtype)
+    nll, g_r, g_a = model.nll_and_grad(*w["x"])
+    assert abs(nll - w["nll"]) <= tol * abs(w["nll"])
+    assert _rel(np.concatenate([np.ravel(g_r), np.ravel(g_a)]),
+                np.concatenate([np.ravel(g) for g in w["g"]])) <= tol
+    got_m = model.metric(*w["v"])
+    assert _rel(np.concatenate([np.ravel(g) for g in got_m]),
+                np.concatenate([np.ravel(m) for m in w["m"]])) <= tol
