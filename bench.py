@@ -56,6 +56,9 @@ def parse():
     ap.add_argument("--transform", default="fft", choices=("fft", "mfma_bf16x3"),
                     help="column-pass transform of the timed FP32 path (with --profile-only); "
                          "the full run times both at FFT length 384")
+    ap.add_argument("--profile-samples", type=int, default=1,
+                    help="with --profile-only: S > 1 runs the batched gradient of S VI-sample "
+                         "positions per call instead of single evaluations")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="with --profile-only: first copy a buffer of this many MiB "
                          "(a known-size reference for the HBM byte counters)")
@@ -514,6 +517,16 @@ def main():
             dst.copy_(src)
             torch.cuda.synchronize()
             del src, dst
+        if args.profile_samples > 1:
+            S = args.profile_samples
+            bm = make_model(ja, torch, inp, torch.float32, counts, max_samples=S)
+            bm.set_tile(*(fixed if fixed is not None else (1, 1)))
+            brng = np.random.default_rng(17)
+            xb = [torch.as_tensor(v[None] + 0.05 * brng.normal(size=(S,) + v.shape),
+                                  dtype=torch.float32, device="cuda") for v in inp["xi"]]
+            t = time_steps(torch, lambda: bm.nll_and_grad_batch(*xb), args.steps, args.warmup)
+            print(json.dumps({"profile_only": True, "samples": S, "ms_per_call": t * 1e3}))
+            return
         t = time_steps(torch, step, args.steps, args.warmup)
         print(json.dumps({"profile_only": True, "ms_per_step": t * 1e3}))
         return
